@@ -1,0 +1,173 @@
+/* xadv.h — C ABI of the MI355X lattice-Boltzmann cold-plate solver (libxadv.so).
+ *
+ * The reference (tgautam03/xAdvCool) has no C ABI: its hot path is a set of
+ * Warp kernels (src/fluid_engine3d.py, src/thermal_engine3d.py) launched by
+ * the Python driver src/simulation.py.  Each entry point below names the
+ * reference code it replaces.  Conventions:
+ *   - every function returns 0 on success, non-zero on error (XADV_E_*), and
+ *     never throws; xadv_last_error() gives the message (thread-local);
+ *   - host pointers are marked "host", device pointers "device"; device
+ *     buffers passed in are owned by the caller, buffers inside a batch by the
+ *     batch;
+ *   - all work runs on the batch's own HIP stream (created with default
+ *     priority); calls that hand data back to the caller synchronise it;
+ *   - one batch per host thread; calls on one batch must not overlap.
+ * Lattice values are FP32 (precision 32) or FP64 (precision 64); the
+ * "lattice precision" below is that type.
+ */
+#ifndef XADV_H
+#define XADV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define XADV_OK 0
+#define XADV_E_ARG 1     /* invalid argument or unsupported mask layout */
+#define XADV_E_HIP 2     /* a HIP runtime call failed */
+#define XADV_E_STATE 3   /* call out of order (e.g. step before init) */
+
+/* Cell codes of the mask: src/fluid_engine3d.py:11-14 (BC_FLUID/SOLID/INLET/OUTLET). */
+#define XADV_FLUID 0
+#define XADV_SOLID 1
+#define XADV_INLET 2
+#define XADV_OUTLET 3
+
+typedef struct xadv_batch xadv_batch;
+
+/* A batch of lattice slots advanced together.  A single solve is a batch of
+ * one slot sized to its lattice.  Replaces the per-solve allocations of
+ * src/simulation.py:60-75 and the captured six-launch graph of
+ * src/simulation.py:93-100. */
+typedef struct {
+  int precision;        /* 32 or 64 */
+  int fluid;            /* 1: advance the D3Q19 fluid lattice */
+  int thermal;          /* 1: advance the D3Q19 thermal lattice */
+  int general;          /* 1: build the kernels that also take periodic axes, a moving
+                           top wall, a body force and a Dirichlet plane (test modes of
+                           validate_analytical.py); 0: channel boundaries only */
+  int slots;            /* number of lattice slots */
+  int cap_nx, cap_ny, cap_nz;  /* capacity of each slot (largest lattice it can hold) */
+  int block;            /* threads per block: 64, 128 or 256; 0 = default (256) */
+  int xcd_remap;        /* 1: map consecutive lattice blocks to one XCD */
+  int graph_chunk;      /* steps captured per hipGraph (even, >= 2); 0 = no graphs */
+} xadv_batch_desc;
+
+/* One lattice instance.  Mirrors the arguments of src/simulation.py:14-29
+ * after its derivations (:70-71 thermal taus, :77-86 heat map normalised,
+ * :61 rho_outlet), plus the boundary options of validate_analytical.py's
+ * test kernels. */
+typedef struct {
+  int nx, ny, nz;
+  const int8_t* mask;   /* host, (nx,ny,nz) C-order, XADV_* codes */
+  const float* heat;    /* host, (nx,ny,nz) C-order, normalised heat map, or NULL (none) */
+  double tau;           /* fluid relaxation time */
+  double tau_th_fluid, tau_th_solid;
+  double heat_power;    /* q = heat * heat_power (thermal_engine3d.py:116-117) */
+  double T_in;          /* inlet temperature */
+  double rho_out;       /* outlet density (fluid_engine3d.py:186) */
+  double u_in;          /* inlet velocity (device-resident, see xadv_set_inlet_velocity) */
+  /* general mode only */
+  int periodic[3];      /* periodic x, y, z (validate_analytical.py:255-273, :397-420) */
+  double force_x;       /* body force (validate_analytical.py:377-395) */
+  double u_wall;        /* top-z moving wall velocity (validate_analytical.py:502-538) */
+  int dirichlet_x;      /* -1 or the x-plane held at dirichlet_T (validate_analytical.py:21-30) */
+  double dirichlet_T;
+  int frozen_velocity;  /* thermal-only: collide with the velocity given at init */
+} xadv_instance_desc;
+
+/* Create / destroy a batch. */
+int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out);
+int xadv_batch_destroy(xadv_batch* b);
+
+/* Upload one instance into a slot (mask, heat map and parameters).  Checks
+ * the mask: with the fluid on, INLET cells only on x=0 and OUTLET cells only
+ * on x=nx-1; with the thermal on, OUTLET cells only on x=nx-1; no INLET or
+ * OUTLET on a periodic x axis.  The slot is inactive until xadv_instance_init.
+ * Replaces the host->device copies of src/simulation.py:62-86. */
+int xadv_instance_load(xadv_batch* b, int slot, const xadv_instance_desc* d);
+
+/* Initial populations from macroscopic fields: f = f_eq(rho, u), g = g_eq(T, u)
+ * with u = 0 in solids.  rho (nx,ny,nz), u (nx,ny,nz,3), T (nx,ny,nz) are
+ * device pointers in lattice precision, or NULL for rho = 1, u = 0, T = T_in.
+ * Activates the slot and restarts its step count.
+ * Replaces compute_equilibrium_kernel and compute_thermal_equilibrium_kernel
+ * (src/fluid_engine3d.py:58-80, src/thermal_engine3d.py:7-47) as launched at
+ * src/simulation.py:89-90. */
+int xadv_instance_init(xadv_batch* b, int slot, const void* rho, const void* u, const void* T);
+
+/* Mark a slot active (1) or retired (0); retired slots are skipped by every kernel. */
+int xadv_set_active(xadv_batch* b, int slot, int active);
+
+/* Inlet velocity of a slot; device-resident, so captured graphs stay valid.
+ * Replaces u_inlet_array.assign (src/simulation.py:146). */
+int xadv_set_inlet_velocity(xadv_batch* b, int slot, double u_in);
+
+/* Advance every active slot by nsteps time steps (one fused AA pass per step).
+ * With want_fields != 0 the last pass also stores rho and u.
+ * Replaces the graph replay loop of src/simulation.py:120-122 (the six
+ * kernels of :94-99: collide, stream, macroscopic for both lattices). */
+int xadv_step(xadv_batch* b, int nsteps, int want_fields);
+
+/* After xadv_step(..., want_fields=1): form T (thermal streaming + macroscopic
+ * of the current step) and, if with_stats, the 16 reduction slots per active
+ * slot (11 fluid + 5 thermal, same meaning as the reference).
+ * Replaces compute_stats_kernel and compute_thermal_stats_kernel
+ * (src/fluid_engine3d.py:277-312, src/thermal_engine3d.py:226-243). */
+int xadv_readout(xadv_batch* b, int with_stats);
+
+/* Copy the 16 stats of every slot to host out[slots*16] (doubles; slots of
+ * retired instances are zero).  Synchronises.  Replaces stats_warp.numpy()
+ * (src/simulation.py:128-129). */
+int xadv_stats(xadv_batch* b, double* out);
+
+/* Copy one slot's macroscopic fields into caller device buffers (any may be
+ * NULL): rho (nx,ny,nz), u (nx,ny,nz,3), T (nx,ny,nz), lattice precision.
+ * Synchronises.  Replaces u.numpy(), rho_warp.numpy(), T.numpy()
+ * (src/simulation.py:161). */
+int xadv_fields(xadv_batch* b, int slot, void* rho, void* u, void* T);
+
+/* Post-solve reductions for one slot, from the fields of the last readout.
+ * out[XADV_NMETRIC] doubles, see XADV_M_* (host, synchronises).  core_x0 is
+ * the inlet buffer length and core_nx, core_ny the heated core size.
+ * Replaces the numpy reductions of compute_macroscopic_metrics
+ * (generate_dataset.py:80-179) and the pressure_drop column (:556). */
+#define XADV_NMETRIC 24
+enum {
+  XADV_M_QVOL = 0,        /* sum u_x over OUTLET cells on x=nx-1 */
+  XADV_M_RHO_IN_SUM,      /* sum rho over non-solid cells on x=1 */
+  XADV_M_RHO_IN_N,
+  XADV_M_RHO_OUT_SUM,     /* sum rho over non-solid cells on x=nx-2 */
+  XADV_M_RHO_OUT_N,
+  XADV_M_RHO_P1_SUM,      /* sum rho over the whole plane x=1 */
+  XADV_M_RHO_P2_SUM,      /* sum rho over the whole plane x=nx-2 */
+  XADV_M_PLANE_N,         /* ny*nz */
+  XADV_M_TS_MAX,          /* heated surface T[x0:x0+cnx, :cny, 0]: max, min, sum, sum of squares, count */
+  XADV_M_TS_MIN,
+  XADV_M_TS_SUM,
+  XADV_M_TS_SUMSQ,
+  XADV_M_TS_N,
+  XADV_M_QOUT_FLUX,       /* sum u_x*T over OUTLET cells */
+  XADV_M_TF_SUM,          /* sum T over non-solid cells of [x0:-x0, 1:-1, 4:-2] */
+  XADV_M_TF_N,
+  XADV_M_UMAX,            /* max |u| over all cells */
+  XADV_M_TMAX,            /* max T over all cells */
+  XADV_M_TSUM,            /* sum T over all cells */
+  XADV_M_NCELL,
+  XADV_M_NAN              /* count of cells with a non-finite rho, u or T */
+};
+int xadv_metrics(xadv_batch* b, int slot, int core_x0, int core_nx, int core_ny, double* out);
+
+/* Message of the last error on this thread ("" if none). */
+const char* xadv_last_error(void);
+
+/* Library version string. */
+const char* xadv_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* XADV_H */

@@ -1,0 +1,551 @@
+// Host side of libxadv.so: the C ABI declared in include/xadv.h.
+//
+// A batch owns one HIP stream (default priority), the slot table in device
+// memory, and per-slot arenas sized for the slot capacity.  Steps are fused
+// AA passes (lbm_kernels.h); runs of graph_chunk steps that start on an odd
+// pass are captured once into a hipGraph and replayed.
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/xadv.h"
+#include "lattice.h"
+#include "lbm_kernels.h"
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                            \
+  do {                                                                                          \
+    hipError_t e_ = (expr);                                                                     \
+    if (e_ != hipSuccess) return fail(XADV_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+}  // namespace
+
+struct xadv_batch {
+  xadv_batch_desc d{};
+  hipStream_t stream = nullptr;
+  int64_t cap_cells = 0, cap_npad = 0;
+  int cap_P = 0;
+  int bps = 0, G = 0, G_launch = 0;
+  size_t rsize = 4;
+  void* slots_dev = nullptr;
+  void* slots_host = nullptr;  // pinned copy of the slot table
+  size_t slot_bytes = 0;
+  // arenas (bytes)
+  char *f = nullptr, *g = nullptr, *flags = nullptr, *q = nullptr, *fin = nullptr, *fsnap = nullptr, *gsnap = nullptr,
+       *gxh = nullptr, *rho = nullptr, *u = nullptr, *T = nullptr;
+  double* partials = nullptr;
+  double* stats_dev = nullptr;
+  double* met_partials = nullptr;
+  double* met_dev = nullptr;
+  double* host_small = nullptr;  // pinned, for stats/metrics read-back
+  int64_t npass = 0;             // passes since the batch's last init of all slots
+  bool fields_fresh = false;
+  std::vector<int> loaded, inited;
+  std::vector<int> frozen;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+};
+
+namespace {
+
+using xadv::Slot;
+
+template <class R>
+Slot<R>* host_slots(xadv_batch* b) { return reinterpret_cast<Slot<R>*>(b->slots_host); }
+
+int push_slots(xadv_batch* b) {
+  HIPCHK(hipMemcpyAsync(b->slots_dev, b->slots_host, b->slot_bytes * b->d.slots, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XADV_OK;
+}
+
+template <class R>
+int push_slot(xadv_batch* b, int s) {
+  Slot<R>* hs = host_slots<R>(b);
+  HIPCHK(hipMemcpyAsync(reinterpret_cast<Slot<R>*>(b->slots_dev) + s, hs + s, sizeof(Slot<R>), hipMemcpyHostToDevice,
+                        b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XADV_OK;
+}
+
+// Push one field of a slot.  Passes change `raw` in the device copy, so
+// after init only single fields are pushed, never the whole (stale) struct.
+template <class R>
+int push_field(xadv_batch* b, int s, size_t off, size_t bytes) {
+  const char* src = reinterpret_cast<const char*>(host_slots<R>(b) + s) + off;
+  char* dst = reinterpret_cast<char*>(reinterpret_cast<Slot<R>*>(b->slots_dev) + s) + off;
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XADV_OK;
+}
+
+// ----------------------------------------------------------- dispatch -----
+
+template <class R, int FL, int TH, int GEN>
+void launch_pass_t(xadv_batch* b, int nodd, int want_fields) {
+  auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
+  dim3 grid(b->G_launch), block(b->d.block);
+  if (nodd)
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps, b->G, b->d.xcd_remap,
+                       want_fields);
+  else
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps, b->G, b->d.xcd_remap,
+                       want_fields);
+}
+
+template <class R, int FL, int TH, int GEN>
+void launch_readout_t(xadv_batch* b, int nodd, int with_stats) {
+  auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
+  dim3 grid(b->G_launch), block(b->d.block);
+  if (nodd)
+    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps, b->G,
+                       b->d.xcd_remap, with_stats, b->partials);
+  else
+    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps, b->G,
+                       b->d.xcd_remap, with_stats, b->partials);
+}
+
+template <class R, int FL, int TH, int GEN>
+void launch_init_t(xadv_batch* b, int slot, int64_t ncell, int frozen) {
+  auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
+  const int bs = 256;
+  dim3 grid((unsigned)((ncell + bs - 1) / bs)), block(bs);
+  hipLaunchKernelGGL((xadv::k_init<R, FL, TH, GEN>), grid, block, 0, b->stream, sl, slot, frozen);
+}
+
+#define XADV_DISPATCH(FN, R, ...)                                        \
+  do {                                                                   \
+    const int fl_ = b->d.fluid, th_ = b->d.thermal, ge_ = b->d.general;  \
+    if (fl_ && th_ && ge_) FN<R, 1, 1, 1>(__VA_ARGS__);                  \
+    else if (fl_ && th_) FN<R, 1, 1, 0>(__VA_ARGS__);                    \
+    else if (fl_ && ge_) FN<R, 1, 0, 1>(__VA_ARGS__);                    \
+    else if (fl_) FN<R, 1, 0, 0>(__VA_ARGS__);                           \
+    else if (ge_) FN<R, 0, 1, 1>(__VA_ARGS__);                           \
+    else FN<R, 0, 1, 0>(__VA_ARGS__);                                    \
+  } while (0)
+
+void launch_pass(xadv_batch* b, int nodd, int want_fields) {
+  if (b->d.precision == 64)
+    XADV_DISPATCH(launch_pass_t, double, b, nodd, want_fields);
+  else
+    XADV_DISPATCH(launch_pass_t, float, b, nodd, want_fields);
+}
+
+void launch_readout(xadv_batch* b, int nodd, int with_stats) {
+  if (b->d.precision == 64)
+    XADV_DISPATCH(launch_readout_t, double, b, nodd, with_stats);
+  else
+    XADV_DISPATCH(launch_readout_t, float, b, nodd, with_stats);
+}
+
+// ------------------------------------------------------------- flags ------
+
+// Per-cell flag words for one instance (see lattice.h).  Also validates the
+// mask against what the AA side planes support.
+int build_flags(const xadv_instance_desc* d, int fluid, int thermal, int general, std::vector<uint32_t>& out) {
+  using namespace xadv;
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const int px = general ? d->periodic[0] : 0, py = general ? d->periodic[1] : 0, pz = general ? d->periodic[2] : 0;
+  const int64_t n = (int64_t)nx * ny * nz;
+  auto at = [&](int x, int y, int z) { return ((int64_t)x * ny + y) * nz + z; };
+  out.assign(n, 0u);
+  bool any_io = false;
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const int m = d->mask[at(x, y, z)];
+        if (m < 0 || m > 3) return fail(XADV_E_ARG, "mask holds a code other than 0..3");
+        if (m == BC_INLET && fluid && x != 0)
+          return fail(XADV_E_ARG, "INLET cells must lie on x = 0 when the fluid lattice runs");
+        if (m == BC_OUTLET && (fluid || thermal) && x != nx - 1)
+          return fail(XADV_E_ARG, "OUTLET cells must lie on x = nx-1");
+        if (m == BC_INLET || m == BC_OUTLET) any_io = true;
+      }
+  if (any_io && px) return fail(XADV_E_ARG, "INLET/OUTLET cells need a non-periodic x axis");
+  if ((fluid || thermal) && nx < 3) return fail(XADV_E_ARG, "nx must be at least 3");
+  for (int x = 0; x < nx; ++x)
+    for (int y = 0; y < ny; ++y)
+      for (int z = 0; z < nz; ++z) {
+        const int64_t c = at(x, y, z);
+        const int m = d->mask[c];
+        uint32_t w = (uint32_t)m;
+        for (int i = 1; i < 19; ++i) {
+          int sx = x - ex_of(i), sy = y - ey_of(i), sz = z - ez_of(i);
+          bool oob = false;
+          if (sx < 0 || sx >= nx) { if (px) sx = (sx + nx) % nx; else oob = true; }
+          if (sy < 0 || sy >= ny) { if (py) sy = (sy + ny) % ny; else oob = true; }
+          if (sz < 0 || sz >= nz) { if (pz) sz = (sz + nz) % nz; else oob = true; }
+          const bool bb = oob || d->mask[at(sx, sy, sz)] == BC_SOLID;
+          if (bb) w |= 1u << (F_BOUNCE0 + i);
+        }
+        if (!px && x == 0) w |= F_FX0;
+        if (!px && x == nx - 1) w |= F_FXN;
+        if (!py && y == 0) w |= F_FY0;
+        if (!py && y == ny - 1) w |= F_FYN;
+        if (!pz && z == 0) w |= F_FZ0;
+        if (!pz && z == nz - 1) w |= F_FZN;
+        if (d->heat && (double)d->heat[c] * d->heat_power > 0.0) w |= F_HEAT;
+        if (general && d->dirichlet_x >= 0 && x == d->dirichlet_x) w |= F_DIR;
+        if (x + 1 < nx && d->mask[at(x + 1, y, z)] == BC_OUTLET) w |= F_SNAP;
+        if (!px && x == nx - 1 && m != BC_INLET && m != BC_OUTLET) w |= F_XH;
+        out[c] = w;
+      }
+  return XADV_OK;
+}
+
+template <class R>
+int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
+  std::vector<uint32_t> fl;
+  int rc = build_flags(d, b->d.fluid, b->d.thermal, b->d.general, fl);
+  if (rc) return rc;
+  const int64_t n = (int64_t)d->nx * d->ny * d->nz;
+  std::vector<R> q(n, (R)0);
+  if (d->heat) {
+    for (int64_t c = 0; c < n; ++c) q[c] = (R)d->heat[c] * (R)d->heat_power;  // thermal_engine3d.py:116-117
+  }
+  Slot<R>& S = host_slots<R>(b)[s];
+  const int64_t cap = b->cap_cells, capn = b->cap_npad, capP = b->cap_P;
+  const size_t rs = sizeof(R);
+  S.nx = d->nx;
+  S.ny = d->ny;
+  S.nz = d->nz;
+  S.P = d->ny * d->nz;
+  S.ncell = n;
+  S.npad = round_up(n, 64);
+  S.active = 0;
+  S.raw = 0;
+  S.px = b->d.general ? d->periodic[0] : 0;
+  S.py = b->d.general ? d->periodic[1] : 0;
+  S.pz = b->d.general ? d->periodic[2] : 0;
+  S.dir_x = b->d.general ? d->dirichlet_x : -1;
+  // Scalars rounded to the lattice precision and derived there, as the
+  // reference's kernels do with their float arguments.
+  const R tau = (R)d->tau, ttf = (R)d->tau_th_fluid, tts = (R)d->tau_th_solid;
+  S.omega = (R)1.0 / tau;
+  S.rho_out = (R)d->rho_out;
+  S.u_in = (R)d->u_in;
+  S.force_x = b->d.general ? (R)d->force_x : (R)0;
+  S.u_wall = b->d.general ? (R)d->u_wall : (R)0;
+  S.omega_tf = (R)1.0 / ttf;
+  S.omega_ts = (R)1.0 / tts;
+  S.sf_tf = (R)1.0 - (R)0.5 * S.omega_tf;
+  S.sf_ts = (R)1.0 - (R)0.5 * S.omega_ts;
+  S.T_in = (R)d->T_in;
+  S.dir_T = (R)d->dirichlet_T;
+  S.f = reinterpret_cast<R*>(b->f + (size_t)s * 19 * capn * rs);
+  S.g = reinterpret_cast<R*>(b->g + (size_t)s * 19 * capn * rs);
+  S.flags = reinterpret_cast<const uint32_t*>(b->flags + (size_t)s * cap * 4);
+  S.q = reinterpret_cast<const R*>(b->q + (size_t)s * cap * rs);
+  S.fin = reinterpret_cast<R*>(b->fin + (size_t)s * 19 * capP * rs);
+  S.fsnap = reinterpret_cast<R*>(b->fsnap + (size_t)s * 2 * 19 * capP * rs);
+  S.gsnap = reinterpret_cast<R*>(b->gsnap + (size_t)s * 2 * 19 * capP * rs);
+  S.gxh = reinterpret_cast<R*>(b->gxh + (size_t)s * 5 * capP * rs);
+  S.rho = reinterpret_cast<R*>(b->rho + (size_t)s * cap * rs);
+  S.u = reinterpret_cast<R*>(b->u + (size_t)s * 3 * cap * rs);
+  S.T = reinterpret_cast<R*>(b->T + (size_t)s * cap * rs);
+  HIPCHK(hipMemcpyAsync((void*)S.flags, fl.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipMemcpyAsync((void*)S.q, q.data(), n * rs, hipMemcpyHostToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  b->loaded[s] = 1;
+  b->inited[s] = 0;
+  b->frozen[s] = d->frozen_velocity;
+  return push_slot<R>(b, s);
+}
+
+template <class R>
+int init_t(xadv_batch* b, int s, const void* rho, const void* u, const void* T) {
+  Slot<R>& S = host_slots<R>(b)[s];
+  const int64_t n = S.ncell;
+  const size_t rs = sizeof(R);
+  const int bs = 256;
+  const unsigned gb = (unsigned)((n + bs - 1) / bs), gb3 = (unsigned)((3 * n + bs - 1) / bs);
+  if (rho)
+    HIPCHK(hipMemcpyAsync(S.rho, rho, n * rs, hipMemcpyDeviceToDevice, b->stream));
+  else
+    hipLaunchKernelGGL((xadv::k_fill<R>), dim3(gb), dim3(bs), 0, b->stream, S.rho, n, (R)1);
+  if (u)
+    HIPCHK(hipMemcpyAsync(S.u, u, 3 * n * rs, hipMemcpyDeviceToDevice, b->stream));
+  else
+    hipLaunchKernelGGL((xadv::k_fill<R>), dim3(gb3), dim3(bs), 0, b->stream, S.u, 3 * n, (R)0);
+  if (T)
+    HIPCHK(hipMemcpyAsync(S.T, T, n * rs, hipMemcpyDeviceToDevice, b->stream));
+  else
+    hipLaunchKernelGGL((xadv::k_fill<R>), dim3(gb), dim3(bs), 0, b->stream, S.T, n, S.T_in);
+  HIPCHK(hipGetLastError());
+  XADV_DISPATCH(launch_init_t, R, b, s, n, b->frozen[s]);
+  HIPCHK(hipGetLastError());
+  S.active = 1;
+  S.raw = b->d.thermal ? 1 : 0;
+  b->inited[s] = 1;
+  return push_slot<R>(b, s);
+}
+
+}  // namespace
+
+// ================================================================ C ABI ====
+
+extern "C" {
+
+const char* xadv_last_error(void) { return g_err.c_str(); }
+const char* xadv_version(void) { return "xadvcool-amd 0.1 (gfx950)"; }
+
+int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
+  if (!desc || !out) return fail(XADV_E_ARG, "null argument");
+  *out = nullptr;
+  xadv_batch_desc d = *desc;
+  if (d.precision != 32 && d.precision != 64) return fail(XADV_E_ARG, "precision must be 32 or 64");
+  if (!d.fluid && !d.thermal) return fail(XADV_E_ARG, "enable the fluid or the thermal lattice");
+  if (d.slots < 1) return fail(XADV_E_ARG, "slots must be >= 1");
+  if (d.cap_nx < 1 || d.cap_ny < 1 || d.cap_nz < 1) return fail(XADV_E_ARG, "capacity dims must be >= 1");
+  if (d.block == 0) d.block = 256;
+  if (d.block != 64 && d.block != 128 && d.block != 256) return fail(XADV_E_ARG, "block must be 64, 128 or 256");
+  if (d.graph_chunk < 0 || (d.graph_chunk % 2)) return fail(XADV_E_ARG, "graph_chunk must be even and >= 0");
+  auto* b = new xadv_batch();
+  b->d = d;
+  b->rsize = d.precision == 64 ? 8 : 4;
+  b->cap_cells = (int64_t)d.cap_nx * d.cap_ny * d.cap_nz;
+  b->cap_npad = round_up(b->cap_cells, 64);
+  b->cap_P = d.cap_ny * d.cap_nz;
+  b->bps = (int)((b->cap_cells + d.block - 1) / d.block);
+  b->G = b->bps * d.slots;
+  b->G_launch = d.xcd_remap ? ((b->G + 7) / 8) * 8 : b->G;
+  b->slot_bytes = d.precision == 64 ? sizeof(Slot<double>) : sizeof(Slot<float>);
+  b->loaded.assign(d.slots, 0);
+  b->inited.assign(d.slots, 0);
+  b->frozen.assign(d.slots, 0);
+  const size_t rs = b->rsize, S = d.slots;
+  const size_t capn = b->cap_npad, cap = b->cap_cells, capP = b->cap_P;
+  auto cleanup = [&](int rc) {
+    xadv_batch_destroy(b);
+    return rc;
+  };
+  hipError_t e;
+  if ((e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking)) != hipSuccess)
+    return cleanup(fail(XADV_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+  struct A {
+    char** p;
+    size_t bytes;
+  } allocs[] = {
+      {&b->f, d.fluid ? S * 19 * capn * rs : 0},
+      {&b->g, d.thermal ? S * 19 * capn * rs : 0},
+      {&b->flags, S * cap * 4},
+      {&b->q, S * cap * rs},
+      {&b->fin, S * 19 * capP * rs},
+      {&b->fsnap, S * 2 * 19 * capP * rs},
+      {&b->gsnap, S * 2 * 19 * capP * rs},
+      {&b->gxh, S * 5 * capP * rs},
+      {&b->rho, S * cap * rs},
+      {&b->u, S * 3 * cap * rs},
+      {&b->T, S * cap * rs},
+  };
+  for (auto& a : allocs) {
+    if (!a.bytes) continue;
+    if ((e = hipMalloc((void**)a.p, a.bytes)) != hipSuccess)
+      return cleanup(fail(XADV_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e)));
+    if ((e = hipMemsetAsync(*a.p, 0, a.bytes, b->stream)) != hipSuccess)
+      return cleanup(fail(XADV_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));
+  }
+  const int nmb = (int)((b->cap_cells + 255) / 256);
+  if ((e = hipMalloc((void**)&b->partials, (size_t)b->G * 16 * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc((void**)&b->stats_dev, S * 16 * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc((void**)&b->met_partials, (size_t)nmb * xadv::kNMet * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc((void**)&b->met_dev, xadv::kNMet * sizeof(double))) != hipSuccess ||
+      (e = hipMalloc(&b->slots_dev, b->slot_bytes * S)) != hipSuccess ||
+      (e = hipHostMalloc(&b->slots_host, b->slot_bytes * S, hipHostMallocDefault)) != hipSuccess ||
+      (e = hipHostMalloc((void**)&b->host_small, S * 16 * sizeof(double) + 64 * sizeof(double),
+                         hipHostMallocDefault)) != hipSuccess)
+    return cleanup(fail(XADV_E_HIP, std::string("allocation: ") + hipGetErrorString(e)));
+  memset(b->slots_host, 0, b->slot_bytes * S);
+  if ((e = hipMemsetAsync(b->stats_dev, 0, S * 16 * sizeof(double), b->stream)) != hipSuccess)
+    return cleanup(fail(XADV_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));
+  int rc = push_slots(b);
+  if (rc) return cleanup(rc);
+  *out = b;
+  return XADV_OK;
+}
+
+int xadv_batch_destroy(xadv_batch* b) {
+  if (!b) return XADV_OK;
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
+  char** ps[] = {&b->f, &b->g, &b->flags, &b->q, &b->fin, &b->fsnap, &b->gsnap, &b->gxh, &b->rho, &b->u, &b->T};
+  for (char** p : ps)
+    if (*p) hipFree(*p);
+  if (b->partials) hipFree(b->partials);
+  if (b->stats_dev) hipFree(b->stats_dev);
+  if (b->met_partials) hipFree(b->met_partials);
+  if (b->met_dev) hipFree(b->met_dev);
+  if (b->slots_dev) hipFree(b->slots_dev);
+  if (b->slots_host) hipHostFree(b->slots_host);
+  if (b->host_small) hipHostFree(b->host_small);
+  if (b->stream) hipStreamDestroy(b->stream);
+  delete b;
+  return XADV_OK;
+}
+
+int xadv_instance_load(xadv_batch* b, int slot, const xadv_instance_desc* d) {
+  if (!b || !d || !d->mask) return fail(XADV_E_ARG, "null argument");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(XADV_E_ARG, "dims must be >= 1");
+  if ((int64_t)d->nx * d->ny * d->nz > b->cap_cells || (int64_t)d->ny * d->nz > b->cap_P)
+    return fail(XADV_E_ARG, "lattice larger than the slot capacity");
+  if (!(d->tau > 0.5) || !(d->tau_th_fluid > 0.5) || !(d->tau_th_solid > 0.5))
+    return fail(XADV_E_ARG, "relaxation times must exceed 0.5");
+  if (d->frozen_velocity && b->d.fluid) return fail(XADV_E_ARG, "frozen_velocity needs a thermal-only batch");
+  return b->d.precision == 64 ? load_t<double>(b, slot, d) : load_t<float>(b, slot, d);
+}
+
+int xadv_instance_init(xadv_batch* b, int slot, const void* rho, const void* u, const void* T) {
+  if (!b) return fail(XADV_E_ARG, "null batch");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (!b->loaded[slot]) return fail(XADV_E_STATE, "slot not loaded");
+  if (b->d.slots == 1) b->npass = 0;
+  if (b->npass % 2) return fail(XADV_E_STATE, "init needs an even step count of the batch; step once more first");
+  return b->d.precision == 64 ? init_t<double>(b, slot, rho, u, T) : init_t<float>(b, slot, rho, u, T);
+}
+
+int xadv_set_active(xadv_batch* b, int slot, int active) {
+  if (!b) return fail(XADV_E_ARG, "null batch");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (active && !b->inited[slot]) return fail(XADV_E_STATE, "slot not initialised");
+  if (b->d.precision == 64) {
+    host_slots<double>(b)[slot].active = active ? 1 : 0;
+    return push_field<double>(b, slot, offsetof(Slot<double>, active), sizeof(int));
+  }
+  host_slots<float>(b)[slot].active = active ? 1 : 0;
+  return push_field<float>(b, slot, offsetof(Slot<float>, active), sizeof(int));
+}
+
+int xadv_set_inlet_velocity(xadv_batch* b, int slot, double u_in) {
+  if (!b) return fail(XADV_E_ARG, "null batch");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (b->d.precision == 64) {
+    host_slots<double>(b)[slot].u_in = (double)u_in;
+    return push_field<double>(b, slot, offsetof(Slot<double>, u_in), sizeof(double));
+  }
+  host_slots<float>(b)[slot].u_in = (float)u_in;
+  return push_field<float>(b, slot, offsetof(Slot<float>, u_in), sizeof(float));
+}
+
+int xadv_step(xadv_batch* b, int nsteps, int want_fields) {
+  if (!b) return fail(XADV_E_ARG, "null batch");
+  if (nsteps < 0) return fail(XADV_E_ARG, "nsteps must be >= 0");
+  if (nsteps == 0) return XADV_OK;
+  const int chunk = b->d.graph_chunk;
+  if (chunk && nsteps % chunk == 0 && b->npass % 2 == 0) {
+    const std::pair<int, int> key(chunk, want_fields ? 1 : 0);
+    const std::pair<int, int> key0(chunk, 0);
+    for (int k = 0; k < nsteps / chunk; ++k) {
+      const bool last = k == nsteps / chunk - 1;
+      const auto kk = (last && want_fields) ? key : key0;
+      auto it = b->graphs.find(kk);
+      if (it == b->graphs.end()) {
+        hipGraph_t gr;
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        for (int j = 0; j < chunk; ++j) launch_pass(b, (j + 1) & 1, (kk.second && j == chunk - 1) ? 1 : 0);
+        hipError_t le = hipGetLastError();
+        HIPCHK(hipStreamEndCapture(b->stream, &gr));
+        if (le != hipSuccess) return fail(XADV_E_HIP, std::string("launch in capture: ") + hipGetErrorString(le));
+        hipGraphExec_t ex;
+        HIPCHK(hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0));
+        HIPCHK(hipGraphDestroy(gr));
+        it = b->graphs.emplace(kk, ex).first;
+      }
+      HIPCHK(hipGraphLaunch(it->second, b->stream));
+      b->npass += chunk;
+    }
+  } else {
+    for (int k = 0; k < nsteps; ++k) {
+      b->npass += 1;
+      launch_pass(b, (int)(b->npass & 1), (want_fields && k == nsteps - 1) ? 1 : 0);
+    }
+    HIPCHK(hipGetLastError());
+  }
+  b->fields_fresh = want_fields != 0;
+  return XADV_OK;
+}
+
+int xadv_readout(xadv_batch* b, int with_stats) {
+  if (!b) return fail(XADV_E_ARG, "null batch");
+  if (b->npass < 1) return fail(XADV_E_STATE, "readout needs at least one step");
+  if (!b->fields_fresh) return fail(XADV_E_STATE, "readout needs the last xadv_step with want_fields = 1");
+  launch_readout(b, (int)((b->npass + 1) & 1), with_stats);
+  HIPCHK(hipGetLastError());
+  if (with_stats) {
+    if (b->d.precision == 64)
+      hipLaunchKernelGGL((xadv::k_stats_final<double>), dim3(b->d.slots), dim3(64), 0, b->stream,
+                         reinterpret_cast<const Slot<double>*>(b->slots_dev), b->bps, b->partials, b->stats_dev);
+    else
+      hipLaunchKernelGGL((xadv::k_stats_final<float>), dim3(b->d.slots), dim3(64), 0, b->stream,
+                         reinterpret_cast<const Slot<float>*>(b->slots_dev), b->bps, b->partials, b->stats_dev);
+    HIPCHK(hipGetLastError());
+  }
+  return XADV_OK;
+}
+
+int xadv_stats(xadv_batch* b, double* out) {
+  if (!b || !out) return fail(XADV_E_ARG, "null argument");
+  const size_t bytes = (size_t)b->d.slots * 16 * sizeof(double);
+  HIPCHK(hipMemcpyAsync(b->host_small, b->stats_dev, bytes, hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  memcpy(out, b->host_small, bytes);
+  return XADV_OK;
+}
+
+int xadv_fields(xadv_batch* b, int slot, void* rho, void* u, void* T) {
+  if (!b) return fail(XADV_E_ARG, "null batch");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (!b->inited[slot]) return fail(XADV_E_STATE, "slot not initialised");
+  const size_t rs = b->rsize;
+  int64_t n;
+  void *dr, *du, *dT;
+  if (b->d.precision == 64) {
+    auto& S = host_slots<double>(b)[slot];
+    n = S.ncell; dr = S.rho; du = S.u; dT = S.T;
+  } else {
+    auto& S = host_slots<float>(b)[slot];
+    n = S.ncell; dr = S.rho; du = S.u; dT = S.T;
+  }
+  if (rho) HIPCHK(hipMemcpyAsync(rho, dr, n * rs, hipMemcpyDeviceToDevice, b->stream));
+  if (u) HIPCHK(hipMemcpyAsync(u, du, 3 * n * rs, hipMemcpyDeviceToDevice, b->stream));
+  if (T) HIPCHK(hipMemcpyAsync(T, dT, n * rs, hipMemcpyDeviceToDevice, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XADV_OK;
+}
+
+int xadv_metrics(xadv_batch* b, int slot, int core_x0, int core_nx, int core_ny, double* out) {
+  if (!b || !out) return fail(XADV_E_ARG, "null argument");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (!b->inited[slot]) return fail(XADV_E_STATE, "slot not initialised");
+  int64_t n = b->d.precision == 64 ? host_slots<double>(b)[slot].ncell : host_slots<float>(b)[slot].ncell;
+  const int nb = (int)((n + 255) / 256);
+  if (b->d.precision == 64)
+    hipLaunchKernelGGL((xadv::k_metrics<double>), dim3(nb), dim3(256), 0, b->stream,
+                       reinterpret_cast<Slot<double>*>(b->slots_dev), slot, core_x0, core_nx, core_ny, b->met_partials);
+  else
+    hipLaunchKernelGGL((xadv::k_metrics<float>), dim3(nb), dim3(256), 0, b->stream,
+                       reinterpret_cast<Slot<float>*>(b->slots_dev), slot, core_x0, core_nx, core_ny, b->met_partials);
+  hipLaunchKernelGGL(xadv::k_metrics_final, dim3(1), dim3(64), 0, b->stream, nb, b->met_partials, b->met_dev);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->host_small, b->met_dev, xadv::kNMet * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  memcpy(out, b->host_small, XADV_NMETRIC * sizeof(double));
+  return XADV_OK;
+}
+
+}  // extern "C"
