@@ -160,6 +160,11 @@ enum {
 };
 int xadv_metrics(xadv_batch* b, int slot, int core_x0, int core_nx, int core_ny, double* out);
 
+/* The batch's HIP stream (hipStream_t), for callers that time or order work
+ * against it (e.g. hipEventRecord around xadv_step).  No reference
+ * counterpart: Warp's default stream was implicit (src/simulation.py:93-121). */
+int xadv_get_stream(xadv_batch* b, void** stream);
+
 /* Message of the last error on this thread ("" if none). */
 const char* xadv_last_error(void);
 

@@ -28,6 +28,9 @@
 #include <vector>
 #include <algorithm>
 
+// Planted-mistake selector for the mutation tests (see orc_cfg.mutation).
+static int g_mutation = 0;
+
 namespace {
 
 enum { BC_FLUID = 0, BC_SOLID = 1, BC_INLET = 2, BC_OUTLET = 3 };  // fluid_engine3d.py:11-14
@@ -244,7 +247,7 @@ void periodic_streaming_solids(const Dims& d, const R* fpc, R* fs, const int* ma
           int sx = (x - EX[i] + d.nx) % d.nx, sy = (y - EY[i] + d.ny) % d.ny, sz = (z - EZ[i] + d.nz) % d.nz;
           int64_t s = d.c(sx, sy, sz);
           if (mask[s] == BC_SOLID)
-            fs[i * N + c] = fpc[OPP[i] * N + c];
+            fs[i * N + c] = fpc[(g_mutation == 2 ? i : OPP[i]) * N + c];
           else
             fs[i * N + c] = fpc[i * N + s];
         }
@@ -268,6 +271,7 @@ void couette_streaming(const Dims& d, const R* fpc, R* fs, const int* mask, R u_
             fs[i * N + c] = fpc[OPP[i] * N + c];
           } else if (sz >= d.nz) {
             R mom = (R)6.0 * W.w[i] * (R)EX[i] * u_wall_x;
+            if (g_mutation == 3) mom = -mom;
             fs[i * N + c] = fpc[OPP[i] * N + c] + mom;
           } else {
             int64_t s = d.c(sx, sy, sz);
@@ -313,7 +317,7 @@ void thermal_bgk(const Dims& d, const R* go, const R* geq, R* gn, const int* mas
     R q = hs[c] * power;
     for (int i = 0; i < 19; ++i) {
       R v = go[i * N + c] - omega * (go[i * N + c] - geq[i * N + c]);
-      if (q > (R)0.0) v += W.w[i] * ((R)1.0 - (R)0.5 * omega) * q;
+      if (q > (R)0.0) v += (g_mutation == 1) ? W.w[i] * q : W.w[i] * ((R)1.0 - (R)0.5 * omega) * q;
       gn[i * N + c] = v;
     }
   }
@@ -437,6 +441,11 @@ typedef struct {
   int dirichlet_x;     // -1: none; else dirichlet_thermal_bc_kernel plane after streaming
   double tau, force_x, u_wall, u_in, rho_out;
   double tau_th_fluid, tau_th_solid, heat_power, T_in, dirichlet_T;
+  // Planted mistakes, for tests that show the closed-form cases catch them:
+  // 1 drops the (1 - omega/2) factor of the thermal source, 2 bounces back
+  // the same direction instead of the opposite one, 3 flips the sign of the
+  // moving-wall momentum.  0 = the reference.
+  int mutation;
 } orc_cfg;
 }  // extern "C"
 
@@ -444,6 +453,7 @@ template <class R>
 static int run_steps(const orc_cfg* cfg, int nsteps, const int* mask, const R* hs, R* f_old, R* f_new, R* f_eq,
                      R* rho, R* u, R* g_old, R* g_new, R* g_eq, R* T) {
   Dims d{cfg->nx, cfg->ny, cfg->nz};
+  g_mutation = cfg->mutation;
   const R tau = (R)cfg->tau, ttf = (R)cfg->tau_th_fluid, tts = (R)cfg->tau_th_solid;
   const R P = (R)cfg->heat_power, Tin = (R)cfg->T_in;
   for (int s = 0; s < nsteps; ++s) {

@@ -35,6 +35,7 @@ class OrcCfg(ctypes.Structure):
         ("u_in", ctypes.c_double), ("rho_out", ctypes.c_double),
         ("tau_th_fluid", ctypes.c_double), ("tau_th_solid", ctypes.c_double),
         ("heat_power", ctypes.c_double), ("T_in", ctypes.c_double), ("dirichlet_T", ctypes.c_double),
+        ("mutation", ctypes.c_int),
     ]
 
 
@@ -103,11 +104,12 @@ class State:
 
 def run(state: State, mask, nsteps, *, fluid=0, fluid_stream=0, forced=0, thermal=0, thermal_stream=0,
         dirichlet_x=-1, tau=0.6, force_x=0.0, u_wall=0.0, u_in=0.0, rho_out=1.0,
-        tau_th_fluid=0.6, tau_th_solid=0.6, heat_power=0.0, T_in=0.0, dirichlet_T=0.0, heat=None):
+        tau_th_fluid=0.6, tau_th_solid=0.6, heat_power=0.0, T_in=0.0, dirichlet_T=0.0, heat=None, mutation=0):
     """Advance ``nsteps`` with the chosen reference schedule (see orc_cfg in lbm_oracle.cpp)."""
     nx, ny, nz = state.shape
     cfg = OrcCfg(nx, ny, nz, fluid, fluid_stream, forced, thermal, thermal_stream, dirichlet_x,
-                 tau, force_x, u_wall, u_in, rho_out, tau_th_fluid, tau_th_solid, heat_power, T_in, dirichlet_T)
+                 tau, force_x, u_wall, u_in, rho_out, tau_th_fluid, tau_th_solid, heat_power, T_in, dirichlet_T,
+                 mutation)
     m = np.ascontiguousarray(mask, dtype=np.int32)
     hs = np.zeros(state.shape, state.dtype) if heat is None else np.ascontiguousarray(heat, dtype=state.dtype)
     fn = getattr(lib(), "orc_run_" + _sfx(state.dtype))

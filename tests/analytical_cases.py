@@ -281,17 +281,18 @@ class OracleBackend:
     _FS = {"channel": 0, "periodic": 1, "periodic_solids": 2, "couette": 3}
     _TS = {"channel": 0, "periodic": 1}
 
-    def __init__(self, dtype=np.float32):
+    def __init__(self, dtype=np.float32, mutation=0):
         from oracle import oracle as orc
         self.orc = orc
         self.dtype = dtype
+        self.mutation = mutation
 
     def fluid(self, mask, rho0, u0, tau, steps, stream="channel", force_x=0.0, u_wall=0.0, u_in=0.0, rho_out=1.0,
               every=None, callback=None):
         st = self.orc.State(mask.shape, self.dtype)
         st.init_fluid(rho0, u0)
         kw = dict(fluid=2, fluid_stream=self._FS[stream], forced=int(force_x != 0.0), tau=tau, force_x=force_x,
-                  u_wall=u_wall, u_in=u_in, rho_out=rho_out)
+                  u_wall=u_wall, u_in=u_in, rho_out=rho_out, mutation=self.mutation)
         done = 0
         while done < steps:
             n = min(every or steps, steps - done)
@@ -308,5 +309,6 @@ class OracleBackend:
         st.init_thermal(T0, mask)
         dx, dT = dirichlet if dirichlet is not None else (-1, 0.0)
         self.orc.run(st, mask, steps, thermal=2, thermal_stream=self._TS[stream], dirichlet_x=dx, dirichlet_T=dT,
-                     tau_th_fluid=tau_f, tau_th_solid=tau_s, heat_power=power, T_in=T_in, heat=heat)
+                     tau_th_fluid=tau_f, tau_th_solid=tau_s, heat_power=power, T_in=T_in, heat=heat,
+                     mutation=self.mutation)
         return st.T.copy()

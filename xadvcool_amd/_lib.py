@@ -18,7 +18,7 @@ NMETRIC = 24
 EXPORTS = (
     "xadv_batch_create", "xadv_batch_destroy", "xadv_instance_load", "xadv_instance_init", "xadv_set_active",
     "xadv_set_inlet_velocity", "xadv_step", "xadv_readout", "xadv_stats", "xadv_fields", "xadv_metrics",
-    "xadv_last_error", "xadv_version",
+    "xadv_get_stream", "xadv_last_error", "xadv_version",
 )
 
 
@@ -70,6 +70,7 @@ def load():
     lib.xadv_stats.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
     lib.xadv_fields.argtypes = [vp, ip, vp, vp, vp]
     lib.xadv_metrics.argtypes = [vp, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_double)]
+    lib.xadv_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
     lib.xadv_last_error.restype = ctypes.c_char_p
     lib.xadv_version.restype = ctypes.c_char_p
     for name in EXPORTS:

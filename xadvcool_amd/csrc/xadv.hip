@@ -304,6 +304,12 @@ extern "C" {
 const char* xadv_last_error(void) { return g_err.c_str(); }
 const char* xadv_version(void) { return "xadvcool-amd 0.1 (gfx950)"; }
 
+int xadv_get_stream(xadv_batch* b, void** stream) {
+  if (!b || !stream) return fail(XADV_E_ARG, "null argument");
+  *stream = (void*)b->stream;
+  return XADV_OK;
+}
+
 int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   if (!desc || !out) return fail(XADV_E_ARG, "null argument");
   *out = nullptr;

@@ -117,6 +117,12 @@ class LatticeBatch:
     def set_inlet_velocity(self, slot, u_in):
         _lib.check(self._lib.xadv_set_inlet_velocity(self._h, int(slot), float(u_in)), "xadv_set_inlet_velocity")
 
+    def stream(self):
+        """The batch's HIP stream as a torch ExternalStream (events recorded there time the kernels)."""
+        ptr = ctypes.c_void_p()
+        _lib.check(self._lib.xadv_get_stream(self._h, ctypes.byref(ptr)), "xadv_get_stream")
+        return torch.cuda.ExternalStream(ptr.value)
+
     # -- stepping -------------------------------------------------------
     def step(self, n, want_fields=True):
         _lib.check(self._lib.xadv_step(self._h, int(n), int(bool(want_fields))), "xadv_step")

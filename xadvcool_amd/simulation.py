@@ -17,7 +17,7 @@ from __future__ import annotations
 import numpy as np
 
 from .constants import BC_SOLID
-from .engine import CoupledSolver
+from .engine import STAT_NAMES, CoupledSolver
 
 
 def hydraulic_diameter(mask, buf=10):
@@ -31,6 +31,87 @@ def hydraulic_diameter(mask, buf=10):
             np.sum(fluid[:, :, 1:] & solid[:, :, :-1]) + np.sum(solid[:, :, 1:] & fluid[:, :, :-1]))
     Dh = 4.0 * V / A if A > 0 else float(mask.shape[1])
     return Dh, V, A
+
+
+class _Residuals:
+    """Residual bookkeeping of src/simulation.py:107-137 and the stop rules of :150-156."""
+
+    def __init__(self, stats_every, tol_ke, tol_T, min_steps):
+        self.every, self.tol_ke, self.tol_T, self.min_steps = stats_every, tol_ke, tol_T, min_steps
+        self.prev_ke = self.prev_max_T = self.prev_Q_out = 0.0
+        self.d_ke = self.dT_conv = 0.0
+        self.dQ_conv = 1.0
+        self.max_v = self.max_T = 0.0
+
+    def update(self, step, s):
+        self.max_v, ke, self.max_T, Q_out = s["max_v"], s["sum_ke"], s["max_T"], s["Q_out"]
+        first = step <= self.every
+        self.d_ke = 0.0 if first else abs(ke - self.prev_ke) / (self.prev_ke + 1e-9) / self.every
+        self.dT_conv = 0.0 if first else abs(self.max_T - self.prev_max_T) / (self.prev_max_T + 1e-9) / self.every
+        self.dQ_conv = 1.0 if first else abs(Q_out - self.prev_Q_out) / (abs(self.prev_Q_out) + 1e-12) / self.every
+        self.prev_ke, self.prev_max_T, self.prev_Q_out = ke, self.max_T, Q_out
+
+    def converged(self, step):
+        return (step > self.min_steps and self.d_ke < self.tol_ke and self.dT_conv < self.tol_T
+                and self.dQ_conv < self.tol_T)
+
+    def diverged(self):
+        return bool(np.isnan(self.max_v) or self.max_v > 0.17 or np.isnan(self.max_T) or self.max_T > 1e4)
+
+
+def min_steps_floor(nx, max_steps, u_inlet_val):
+    """Flow-through floor of src/simulation.py:104."""
+    return min(int(0.6 * max_steps), max(5000, int(2.0 * nx / max(u_inlet_val, 1e-6))))
+
+
+def solve_on_batch(batch, slot, *, nx, max_steps, tau_fluid, u_inlet_val, stats_every=100, tol_ke=1e-7, tol_T=1e-7,
+                   pump_mode="constant_flow", target_power=1.0, min_steps_rule=True, pbar=None, Dh=0.0,
+                   timer=None):
+    """The step loop of src/simulation.py:102-160 on slot ``slot`` of a one-slot, initialised batch.
+
+    ``timer`` (optional) gets ``begin()`` before and ``end(n)`` after every
+    block of n steps (events on the batch stream).  Returns the loop's outcome.
+    """
+    min_steps = min_steps_floor(nx, max_steps, u_inlet_val) if min_steps_rule else 0
+    res = _Residuals(stats_every, tol_ke, tol_T, min_steps)
+    u_in_eff = float(np.float32(u_inlet_val))
+    step = 0
+    diverged = False
+    while step < max_steps:
+        n = min(stats_every - (step % stats_every), max_steps - step)
+        if timer is not None:
+            timer.begin()
+        batch.step(n, want_fields=True)
+        if timer is not None:
+            timer.end(n)
+        step += n
+        if pbar is not None:
+            pbar.update(n)
+        if step % stats_every != 0:
+            continue
+        batch.readout(with_stats=True)
+        s = dict(zip(STAT_NAMES, batch.stats()[slot].tolist()))
+        res.update(step, s)
+        if pump_mode == "constant_power" and step > 2000:
+            rho_in_avg = s["sum_rho_in"] / s["c_in"] if s["c_in"] > 0 else 1.0
+            rho_out_avg = s["sum_rho_out"] / s["c_out"] if s["c_out"] > 0 else 1.0
+            dP_measured = (rho_in_avg - rho_out_avg) / 3.0
+            power_measured = dP_measured * (u_in_eff * s["c_in"])
+            u_in_eff = float(np.float32(np.clip(u_in_eff + 0.005 * (target_power - power_measured), 0.0001, 0.15)))
+            batch.set_inlet_velocity(slot, u_in_eff)
+        if pbar is not None:
+            pbar.set_postfix({"Re": f"{(u_in_eff * Dh) / ((tau_fluid - 0.5) / 3.0):.1f}", "MaxV": f"{res.max_v:.4f}",
+                              "MaxT": f"{res.max_T:.2f}", "dKE": f"{res.d_ke:.1e}", "dQ": f"{res.dQ_conv:.1e}"})
+        if res.converged(step):
+            break
+        if res.diverged():
+            diverged = True
+            break
+    if step % stats_every != 0:
+        batch.readout(with_stats=False)
+    return {"steps": step, "d_ke": res.d_ke, "dT_conv": res.dT_conv, "dQ_conv": res.dQ_conv,
+            "converged": bool(res.converged(step)), "diverged": diverged, "max_v": res.max_v, "max_T": res.max_T,
+            "u_inlet": u_in_eff}
 
 
 def run_simulation(mask,
@@ -71,58 +152,22 @@ def run_simulation(mask,
             solver.init_equilibrium(*init_fields)
         else:
             solver.init_equilibrium()
-
-        min_steps = min(int(0.6 * max_steps), max(5000, int(2.0 * nx / max(u_inlet_val, 1e-6))))
-        if not min_steps_rule:
-            min_steps = 0
-
         pbar = None
         if not silent:
             from tqdm import tqdm
             pbar = tqdm(total=max_steps)
-        prev_ke = prev_max_T = prev_Q_out = 0.0
-        d_ke = dT_conv = 0.0
-        dQ_conv = 1.0
-        u_in_eff = float(np.float32(u_inlet_val))
-        step = 0
-        while step < max_steps:
-            n = min(stats_every - (step % stats_every), max_steps - step)
-            solver.step(n)
-            step += n
-            if pbar is not None:
-                pbar.update(n)
-            if step % stats_every != 0:
-                continue
-            s = solver.stats()
-            max_v, ke, max_T, Q_out = s["max_v"], s["sum_ke"], s["max_T"], s["Q_out"]
-            d_ke = abs(ke - prev_ke) / (prev_ke + 1e-9) / stats_every if step > stats_every else 0.0
-            dT_conv = abs(max_T - prev_max_T) / (prev_max_T + 1e-9) / stats_every if step > stats_every else 0.0
-            dQ_conv = abs(Q_out - prev_Q_out) / (abs(prev_Q_out) + 1e-12) / stats_every if step > stats_every else 1.0
-            prev_ke, prev_max_T, prev_Q_out = ke, max_T, Q_out
-
-            if pump_mode == "constant_power" and step > 2000:
-                rho_in_avg = s["sum_rho_in"] / s["c_in"] if s["c_in"] > 0 else 1.0
-                rho_out_avg = s["sum_rho_out"] / s["c_out"] if s["c_out"] > 0 else 1.0
-                dP_measured = (rho_in_avg - rho_out_avg) / 3.0
-                power_measured = dP_measured * (u_in_eff * s["c_in"])
-                u_in_eff = float(np.float32(np.clip(u_in_eff + 0.005 * (target_power - power_measured), 0.0001, 0.15)))
-                solver.set_inlet_velocity(u_in_eff)
-
-            if pbar is not None:
-                pbar.set_postfix({"Re": f"{(u_in_eff * Dh) / ((tau_fluid - 0.5) / 3.0):.1f}", "MaxV": f"{max_v:.4f}",
-                                  "MaxT": f"{max_T:.2f}", "dKE": f"{d_ke:.1e}", "dQ": f"{dQ_conv:.1e}"})
-            if step > min_steps and d_ke < tol_ke and dT_conv < tol_T and dQ_conv < tol_T:
-                break
-            if np.isnan(max_v) or max_v > 0.17 or np.isnan(max_T) or max_T > 1e4:
-                if not silent:
-                    print(f"\nSimulation diverged at step {step}: MaxV={max_v:.2e}, Ma={max_v * 3 ** 0.5:.2f}, "
-                          f"MaxT={max_T:.2e}")
-                break
+        out = solve_on_batch(solver.batch, 0, nx=nx, max_steps=max_steps, tau_fluid=tau_fluid,
+                             u_inlet_val=u_inlet_val, stats_every=stats_every, tol_ke=tol_ke, tol_T=tol_T,
+                             pump_mode=pump_mode, target_power=target_power, min_steps_rule=min_steps_rule,
+                             pbar=pbar, Dh=Dh)
         if pbar is not None:
             pbar.close()
-        converged = step > min_steps and d_ke < tol_ke and dT_conv < tol_T and dQ_conv < tol_T
-        rho, u, T = solver.fields_torch()
+            if out["diverged"]:
+                print(f"\nSimulation diverged at step {out['steps']}: MaxV={out['max_v']:.2e}, "
+                      f"Ma={out['max_v'] * 3 ** 0.5:.2f}, MaxT={out['max_T']:.2e}")
+        rho, u, T = solver.batch.fields(0)
         return {"u": u.cpu().numpy(), "rho": rho.cpu().numpy(), "T": T.cpu().numpy(), "mask": mask,
-                "steps": step, "d_ke": d_ke, "dT_conv": dT_conv, "dQ_conv": dQ_conv, "converged": converged}
+                "steps": out["steps"], "d_ke": out["d_ke"], "dT_conv": out["dT_conv"], "dQ_conv": out["dQ_conv"],
+                "converged": out["converged"]}
     finally:
         solver.close()
