@@ -49,6 +49,6 @@ def test_run_simulation_reference_defaults(gpu):
     assert set(got) >= {"u", "rho", "T", "mask", "steps", "d_ke", "dT_conv", "dQ_conv", "converged"}
     assert got["u"].shape == mask.shape + (3,) and got["u"].dtype == np.float32
     assert got["rho"].shape == mask.shape and got["T"].shape == mask.shape
-    assert got["steps"] == 300 and got["converged"] is False
     ref = orc.run_simulation(mask, max_steps=300)
+    assert got["steps"] == ref["steps"] and got["converged"] == ref["converged"]
     assert np.array_equal(got["T"], ref["T"]) and np.array_equal(got["u"], ref["u"])
