@@ -46,15 +46,22 @@ class PassTimer:
         self._e0 = self.torch.cuda.Event(enable_timing=True)
         self._e0.record(self.stream)
 
-    def end(self, n):
+    def end(self, n, bytes_per_pass=None):
         e1 = self.torch.cuda.Event(enable_timing=True)
         e1.record(self.stream)
-        self.pairs.append((self._e0, e1, n))
+        self.pairs.append((self._e0, e1, n, bytes_per_pass))
         self.passes += n
 
     def total_ms(self):
         self.torch.cuda.synchronize()
-        return sum(a.elapsed_time(b) for a, b, _ in self.pairs)
+        return sum(p[0].elapsed_time(p[1]) for p in self.pairs)
+
+    def first(self, k):
+        """(ms, passes, bytes) over the first k chunks."""
+        self.torch.cuda.synchronize()
+        sel = self.pairs[:k]
+        return (sum(p[0].elapsed_time(p[1]) for p in sel), sum(p[2] for p in sel),
+                sum(p[2] * (p[3] or 0) for p in sel))
 
 
 def algorithmic_bytes_per_pass(mask, rsize=4):
@@ -118,6 +125,80 @@ def cpu_baseline(design, seconds, threads):
     return mask.size * steps / dt / 1e6, steps, dt
 
 
+def profile_passes(npasses, seed=1000):
+    """Only fused passes on one config-2 design (graph chunks of 100), for kernel traces and counters."""
+    import torch
+    from xadvcool_amd.engine import LatticeBatch
+    from xadvcool_amd.geometry import config2_design
+    d = config2_design(seed)
+    batch = LatticeBatch(slots=1, cap_shape=LATTICE, graph_chunk=100)
+    load_design(batch, d)
+    batch.step(200, want_fields=False)  # past the start-up transient
+    torch.cuda.synchronize()
+    timer = PassTimer(batch.stream())
+    done = 0
+    while done < npasses:
+        n = min(100, npasses - done)
+        timer.begin()
+        batch.step(n, want_fields=False)
+        timer.end(n)
+        done += n
+    ms = timer.total_ms() / max(done, 1)
+    nb = algorithmic_bytes_per_pass(d["mask"])
+    print(json.dumps({"mode": "passes", "passes": done, "avg_pass_us": round(1000 * ms, 2),
+                      "alg_bytes_per_pass": nb, "achieved_GBs": round(nb / (ms * 1e-3) / 1e9, 1)}), flush=True)
+    batch.close()
+
+
+def profile_copy(mbytes, reps=20):
+    """A plain device copy of known size (torch): calibrates the FETCH_SIZE / WRITE_SIZE counters and gives the
+    copy bandwidth for that working set (read + write bytes / time)."""
+    import torch
+    n = int(mbytes * (1 << 20)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    print(json.dumps({"mode": "copy", "bytes_read": 4 * n, "bytes_written": 4 * n, "launches": reps + 3,
+                      "copy_GBs": round(8 * n / (ms * 1e-3) / 1e9, 1), "us": round(1000 * ms, 2)}), flush=True)
+
+
+def batch_bench(n_designs, slots, max_steps, tol, seed0=2000, **shape):
+    """Config-3 style batch: mixed families and heat maps, uneven lattices, one launch per step."""
+    import torch
+    from xadvcool_amd.batch import BatchedSolver
+    from xadvcool_amd.geometry import family_design
+    designs = [dict(family_design(seed0 + i), key=seed0 + i) for i in range(n_designs)]
+    bs = BatchedSolver(slots, (148, 128, 32), tol_ke=tol, tol_T=tol, max_steps=max_steps, **shape)
+    timer = PassTimer(bs.batch.stream())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    res = bs.run(designs, timer=timer)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms, npass, nbytes = timer.first(3)
+    total_ms = timer.total_ms()
+    out = {"designs": n_designs, "slots": slots, "wall_s": round(wall, 2),
+           "converged": sum(r["converged"] for r in res),
+           "status": {k: sum(r["status"] == k for r in res) for k in ("converged", "diverged", "max_steps", "stalled")},
+           "solves_per_hr": round(3600.0 * sum(r["converged"] for r in res) / wall, 1),
+           "mlups": round(bs.lups / wall / 1e6, 1),
+           "full_batch_pass_us": round(1000 * ms / max(npass, 1), 2),
+           "full_batch_alg_GBs": round(nbytes / (ms * 1e-3) / 1e9, 1),
+           "all_passes_ms": round(total_ms, 1), "passes": bs.passes,
+           "cells_total": int(sum(d["mask"].size for d in designs)),
+           "steps": sorted(r["steps"] for r in res)}
+    bs.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -127,6 +208,12 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["solves", "passes", "copy", "batch"], default="solves",
+                    help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy)")
+    ap.add_argument("--passes", type=int, default=400)
+    ap.add_argument("--copy-mb", type=float, default=512.0)
+    ap.add_argument("--designs", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=64)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -134,7 +221,15 @@ def main():
               "multi-GPU path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
-    import torch
+    if args.mode == "passes":
+        return profile_passes(args.passes)
+    if args.mode == "copy":
+        return profile_copy(args.copy_mb)
+    if args.mode == "batch":
+        print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots, args.max_steps, args.tol)}),
+              flush=True)
+        return
+
     from xadvcool_amd.geometry import config2_design
 
     warm = [config2_design(900 + i) for i in range(args.warmup)]

@@ -53,6 +53,7 @@ typedef struct {
   int block;            /* threads per block: 64, 128 or 256; 0 = default (256) */
   int xcd_remap;        /* 1: map consecutive lattice blocks to one XCD */
   int graph_chunk;      /* steps captured per hipGraph (even, >= 2); 0 = no graphs */
+  int cells_per_thread; /* lattice cells per thread in the fused pass: 1 or 2; 0 = default (2) */
 } xadv_batch_desc;
 
 /* One lattice instance.  Mirrors the arguments of src/simulation.py:14-29

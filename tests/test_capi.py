@@ -41,7 +41,7 @@ def test_bad_arguments_are_errors_not_crashes():
     """Argument checks run before any device call."""
     from xadvcool_amd import _lib
     lib = _lib.load()
-    d = _lib.BatchDesc(16, 1, 1, 0, 1, 4, 4, 4, 256, 0, 0)  # precision 16 is not supported
+    d = _lib.BatchDesc(16, 1, 1, 0, 1, 4, 4, 4, 256, 0, 0, 0)  # precision 16 is not supported
     h = ctypes.c_void_p()
     assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
     assert b"precision" in lib.xadv_last_error()

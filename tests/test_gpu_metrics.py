@@ -1,0 +1,39 @@
+"""On-device metric reductions against the numpy restatement of compute_macroscopic_metrics.
+
+The reference reduces the converged fields on the host (generate_dataset.py:80-179,
+:438-441, :553-556) in FP32 numpy; the solver reduces them on the GPU in FP64
+(wavefront shuffles + LDS, fixed-order block sums).  Same fields, so the two
+agree to FP32 summation error (rtol 2e-5).
+"""
+import numpy as np
+import pytest
+
+from oracle import metrics_ref
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("family", ["gyroid", "straight"])
+def test_device_metrics_match_numpy(gpu, family):
+    from xadvcool_amd import geometry as G
+    from xadvcool_amd.engine import CoupledSolver
+    print(f"[metrics] {family}", flush=True)
+    core = (40, 24, 12)
+    d = G.family_design(7, core=core, family=family, heat_name="dual_core")
+    mask = d["mask"]
+    s = CoupledSolver(mask, d["heat"], tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
+                      u_inlet_val=d["u_inlet_val"])
+    try:
+        s.init_equilibrium()
+        s.step(1500)
+        got = s.metrics(tau_fluid=d["tau_fluid"], u_inlet_val=d["u_inlet_val"], core=core[:2])
+        rho, u, T = (t.cpu().numpy() for t in s.fields_torch())
+    finally:
+        s.close()
+    Dh, _, A = G.compute_hydraulic_diameter(mask)
+    res = {"u": u, "rho": rho, "T": T}
+    want = metrics_ref.compute_macroscopic_metrics(res, mask, d, Dh, A, core=core[:2])
+    want.update(metrics_ref.metadata_reductions(res))
+    for k, v in want.items():
+        assert got[k] == pytest.approx(v, rel=2e-5, abs=1e-9), k
+    assert got["nonfinite_cells"] == 0

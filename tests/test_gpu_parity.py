@@ -78,11 +78,11 @@ def oracle_coupled(p, steps, dtype):
     return st
 
 
-def hip_coupled(p, steps, precision, chunks=None, graph_chunk=100):
+def hip_coupled(p, steps, precision, chunks=None, graph_chunk=100, **shape):
     from xadvcool_amd.engine import LatticeSolver
     s = LatticeSolver(p["mask"], fluid=True, thermal=True, heat=p["heat"], tau=p["tau"], tau_th_fluid=p["ttf"],
                       tau_th_solid=p["tts"], heat_power=p["P"], T_in=p["T_in"], u_in=p["u_in"], precision=precision,
-                      graph_chunk=graph_chunk)
+                      graph_chunk=graph_chunk, **shape)
     try:
         s.init_equilibrium()
         for n in (chunks or [steps]):
@@ -99,6 +99,18 @@ def test_coupled_fp32_matches_oracle(gpu, steps):
     p = coupled_params()
     ref = oracle_coupled(p, steps, np.float32)
     r, u, T = hip_coupled(p, steps, 32)
+    assert_match("rho", r, ref.rho)
+    assert_match("u", u, ref.u)
+    assert_match("T", T, ref.T)
+
+
+@pytest.mark.parametrize("block,cpt,xcd", [(64, 1, False), (128, 2, True), (256, 1, True), (64, 2, False)])
+def test_coupled_launch_shapes(gpu, block, cpt, xcd):
+    """Every launch shape the autotuner may pick computes the same bits."""
+    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd}", flush=True)
+    p = coupled_params(seed=2, shape=(23, 13, 11))
+    ref = oracle_coupled(p, 13, np.float32)
+    r, u, T = hip_coupled(p, 13, 32, block=block, cells_per_thread=cpt, xcd_remap=xcd)
     assert_match("rho", r, ref.rho)
     assert_match("u", u, ref.u)
     assert_match("T", T, ref.T)

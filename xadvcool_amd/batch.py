@@ -1,0 +1,143 @@
+"""Batched solving: many designs of uneven lattice sizes advanced by one fused launch per step.
+
+``BatchedSolver`` keeps ``slots`` lattice slots (each sized for the largest
+lattice) in one C-ABI batch.  Every step is one launch of the fused AA pass
+over all active slots; blocks of retired or empty slots exit at once.  Steps
+run in graph-captured chunks of ``stats_every``; after each chunk one
+reduction launch forms T and the 16 stats of every slot, and the host applies
+the reference's residual rule (src/simulation.py:130-156) per slot:
+
+* converged or diverged (or at ``max_steps``) slots retire: their fields and
+  on-device metrics are collected and the slot takes the next queued design
+  (continuous batching), so the launch stays full until the queue drains;
+* all slots share the AA parity, so a new design starts at a chunk boundary.
+
+The queue holds design dicts (``mask``, ``heat``, ``tau_fluid``,
+``u_inlet_val``, ``heat_power``; optional ``init_fields`` = (rho, u, T) for a
+warm start, and ``key``).
+"""
+from __future__ import annotations
+
+import time
+from collections import deque
+
+import numpy as np
+
+from .engine import STAT_NAMES, LatticeBatch, normalized_heat, thermal_taus
+from .simulation import _Residuals, min_steps_floor
+
+
+class BatchedSolver:
+    def __init__(self, slots, cap_shape=(148, 128, 32), *, stats_every=100, tol_ke=1e-6, tol_T=1e-6,
+                 max_steps=30000, min_steps_rule=False, Pr=7.0, k_ratio=628.0, t_inlet=25.0, precision=32,
+                 block=256, cells_per_thread=1, xcd_remap=False, collect_fields=False, metrics=True,
+                 stall_check=None):
+        self.slots = int(slots)
+        self.cap_shape = tuple(cap_shape)
+        self.batch = LatticeBatch(slots=self.slots, cap_shape=self.cap_shape, precision=precision, block=block,
+                                  cells_per_thread=cells_per_thread, xcd_remap=xcd_remap, graph_chunk=stats_every)
+        self.every = int(stats_every)
+        self.tol_ke, self.tol_T = tol_ke, tol_T
+        self.max_steps = int(max_steps)
+        self.min_steps_rule = min_steps_rule
+        self.Pr, self.k_ratio, self.t_inlet = Pr, k_ratio, t_inlet
+        self.collect_fields = collect_fields
+        self.metrics = metrics
+        # Optional callable(design, step, residuals) -> True to abandon a run
+        # (used by the warm-start fallback to detect a stalled or growing residual).
+        self.stall_check = stall_check
+        self.state = [None] * self.slots
+        self.lups = 0
+        self.passes = 0
+
+    def close(self):
+        self.batch.close()
+
+    # -- slots ----------------------------------------------------------------
+    def _start(self, s, d):
+        mask = d["mask"]
+        ttf, tts = thermal_taus(d["tau_fluid"], self.Pr, self.k_ratio)
+        self.batch.load(s, mask, heat=normalized_heat(d.get("heat"), mask.shape), tau=d["tau_fluid"],
+                        tau_th_fluid=ttf, tau_th_solid=tts, heat_power=d["heat_power"], T_in=self.t_inlet,
+                        rho_out=1.0, u_in=d["u_inlet_val"])
+        init = d.get("init_fields")
+        if init is not None:
+            self.batch.init(s, *init)
+        else:
+            self.batch.init(s)
+        nx = mask.shape[0]
+        min_steps = min_steps_floor(nx, self.max_steps, d["u_inlet_val"]) if self.min_steps_rule else 0
+        self.state[s] = dict(design=d, steps=0, res=_Residuals(self.every, self.tol_ke, self.tol_T, min_steps),
+                             cells=int(mask.size), t0=time.perf_counter(), history=[],
+                             bytes=19 * 4 * 2 * (int(np.count_nonzero(mask != 1)) + int(mask.size)))
+
+    def _finish(self, s, status):
+        st = self.state[s]
+        d = st["design"]
+        out = dict(key=d.get("key"), steps=st["steps"], status=status, converged=status == "converged",
+                   d_ke=st["res"].d_ke, dT_conv=st["res"].dT_conv, dQ_conv=st["res"].dQ_conv,
+                   max_v=st["res"].max_v, max_T=st["res"].max_T, history=st["history"],
+                   wall_s=time.perf_counter() - st["t0"])
+        if self.metrics:
+            from .metrics import device_metrics
+            mask = d["mask"]
+            core = (mask.shape[0] - 20, mask.shape[1])
+            out["metrics"] = device_metrics(self.batch, s, mask, tau_fluid=d["tau_fluid"],
+                                            u_inlet_val=d["u_inlet_val"], core=core, Pr=self.Pr)
+        if self.collect_fields:
+            rho, u, T = self.batch.fields(s)
+            out["fields"] = (rho.cpu().numpy(), u.cpu().numpy(), T.cpu().numpy())
+        self.batch.set_active(s, False)
+        self.state[s] = None
+        return out
+
+    # -- driver ---------------------------------------------------------------
+    def active_bytes_per_pass(self):
+        """Algorithmic bytes of one FP32 pass over the active slots: 152 B per non-solid cell (fluid lattice)
+        plus 152 B per cell (thermal lattice)."""
+        return sum(st["bytes"] for st in self.state if st is not None)
+
+    def run(self, designs, on_result=None, timer=None):
+        """Solve every design in ``designs``; returns results in completion order.
+
+        ``timer`` (optional) gets ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes."""
+        queue = deque(designs)
+        results = []
+        for s in range(self.slots):
+            if queue:
+                self._start(s, queue.popleft())
+        while any(st is not None for st in self.state):
+            if timer is not None:
+                timer.begin()
+            self.batch.step(self.every, want_fields=True)
+            if timer is not None:
+                timer.end(self.every, self.active_bytes_per_pass())
+            self.passes += self.every
+            self.batch.readout(with_stats=True)
+            stats = self.batch.stats()
+            for s, st in enumerate(self.state):
+                if st is None:
+                    continue
+                st["steps"] += self.every
+                self.lups += st["cells"] * self.every
+                sd = dict(zip(STAT_NAMES, stats[s].tolist()))
+                res = st["res"]
+                res.update(st["steps"], sd)
+                st["history"].append((st["steps"], res.d_ke, res.dT_conv, res.dQ_conv))
+                status = None
+                if res.converged(st["steps"]):
+                    status = "converged"
+                elif res.diverged():
+                    status = "diverged"
+                elif self.stall_check is not None and self.stall_check(st["design"], st["steps"], res):
+                    status = "stalled"
+                elif st["steps"] >= self.max_steps:
+                    status = "max_steps"
+                if status is not None:
+                    r = self._finish(s, status)
+                    results.append(r)
+                    if on_result is not None:
+                        on_result(r)
+                    if queue:
+                        self._start(s, queue.popleft())
+        return results

@@ -76,10 +76,28 @@ __device__ __forceinline__ int map_block(int G, int xcd) {
   return b;
 }
 
+// e_i . u with the zero components left out.  The reference forms
+// ex*ux + ey*uy + ez*uz; a 0*u term only adds a signed zero, which cannot
+// change a non-zero sum and disappears in 1 + 3*eu, so the result is the
+// reference's (the parity tests check the bits).  Non-zero terms keep the
+// reference's x, y, z order; -1 factors are free negation modifiers.
+template <class R, int i>
+__device__ __forceinline__ R edot(R ux, R uy, R uz) {
+  constexpr int a = ex_of(i), b = ey_of(i), c = ez_of(i);
+  const R tx = a > 0 ? ux : -ux, ty = b > 0 ? uy : -uy, tz = c > 0 ? uz : -uz;
+  if (a != 0 && b != 0) return tx + ty;
+  if (a != 0 && c != 0) return tx + tz;
+  if (b != 0 && c != 0) return ty + tz;
+  if (a != 0) return tx;
+  if (b != 0) return ty;
+  if (c != 0) return tz;
+  return (R)0;
+}
+
 template <class R, int i>
 __device__ __forceinline__ R feq_i(R rho, R ux, R uy, R uz, R usq) {
   // src/fluid_engine3d.py:74-78 (same expression for the thermal lattice with T)
-  const R eu = (R)ex_of(i) * ux + (R)ey_of(i) * uy + (R)ez_of(i) * uz;
+  const R eu = edot<R, i>(ux, uy, uz);
   return (R)w_of(i) * rho * ((R)1.0 + (R)3.0 * eu + (R)4.5 * (eu * eu) - (R)1.5 * usq);
 }
 
@@ -89,162 +107,387 @@ __device__ __forceinline__ R wall_mom(int i, R u_wall) {
   return (R)6.0 * (R)w_of(i) * (R)ex_of(i) * u_wall;
 }
 
+// Loads and stores through the global address space with a wave-uniform
+// base (SGPRs) and a 32-bit per-lane byte offset, so the compiler emits
+// global_load/store with saddr instead of FLAT with 64-bit address math.
+// Every lattice array of a slot stays below 4 GiB (checked on the host).
+#define XADV_GLOBAL __attribute__((address_space(1)))
+template <class R>
+__device__ __forceinline__ R ldb(const void* base, uint32_t byte_off) {
+  // C-style cast: an address-space cast from the generic pointer.
+  const XADV_GLOBAL char* g = (const XADV_GLOBAL char*)base;
+  return *(const XADV_GLOBAL R*)(g + byte_off);
+}
+template <class R>
+__device__ __forceinline__ void stb(void* base, uint32_t byte_off, R v) {
+  XADV_GLOBAL char* g = (XADV_GLOBAL char*)base;
+  *(XADV_GLOBAL R*)(g + byte_off) = v;
+}
+template <class R>
+__device__ __forceinline__ R ld(const R* base, uint32_t idx) {
+  return ldb<R>(base, idx * (uint32_t)sizeof(R));
+}
+template <class R>
+__device__ __forceinline__ void st(R* base, uint32_t idx, R v) {
+  stb<R>(base, idx * (uint32_t)sizeof(R), v);
+}
+// All-ones if bit `b` of w is set, else 0 (one v_bfe_i32).
+__device__ __forceinline__ uint32_t bitmask(uint32_t w, int b) { return (uint32_t)(((int32_t)(w << (31 - b))) >> 31); }
+// All-ones if any bit of m is set in w, else 0.
+__device__ __forceinline__ uint32_t anymask(uint32_t w, uint32_t m) { return (w & m) ? 0xffffffffu : 0u; }
+
+// Element c + ka (lanes with sel == 0) or c + kb (lanes with sel == ~0) of A,
+// where ka and kb are wave-uniform and KB_GT says kb > ka.  The base is the
+// lower of the two, so the per-lane byte offset is non-negative and below the
+// array's size (< 4 GiB).
+template <class R, bool KB_GT>
+__device__ __forceinline__ R ld2(const R* A, int64_t ka, int64_t kb, uint32_t c, uint32_t sel) {
+  constexpr uint32_t rs = sizeof(R);
+  const char* a = reinterpret_cast<const char*>(A);
+  if (KB_GT) return ldb<R>(a + ka * (int64_t)rs, c * rs + (sel & (uint32_t)((kb - ka) * rs)));
+  return ldb<R>(a + kb * (int64_t)rs, c * rs + (~sel & (uint32_t)((ka - kb) * rs)));
+}
+template <class R, bool KB_GT>
+__device__ __forceinline__ void st2(R* A, int64_t ka, int64_t kb, uint32_t c, uint32_t sel, R v) {
+  constexpr uint32_t rs = sizeof(R);
+  char* a = reinterpret_cast<char*>(A);
+  if (KB_GT)
+    stb<R>(a + ka * (int64_t)rs, c * rs + (sel & (uint32_t)((kb - ka) * rs)), v);
+  else
+    stb<R>(a + kb * (int64_t)rs, c * rs + (~sel & (uint32_t)((ka - kb) * rs)), v);
+}
+
 template <int i>
-__device__ __forceinline__ int64_t nb_off(int P, int nz) {
-  return (int64_t)ex_of(i) * P + (int64_t)ey_of(i) * nz + ez_of(i);
+__device__ __forceinline__ int nb_off(int P, int nz) {
+  return ex_of(i) * P + ey_of(i) * nz + ez_of(i);
 }
 
 // Index of c + s*e_i with periodic wrap on the periodic axes (general mode).
 // On a non-periodic axis the result may leave the lattice; callers only use it
 // where the precomputed bits say the neighbour exists.
 template <int i>
-__device__ __forceinline__ int64_t nb_wrap(int x, int y, int z, int s, int nx, int ny, int nz) {
+__device__ __forceinline__ uint32_t nb_wrap(int x, int y, int z, int s, int nx, int ny, int nz) {
   int sx = x + s * ex_of(i), sy = y + s * ey_of(i), sz = z + s * ez_of(i);
   sx = sx < 0 ? sx + nx : (sx >= nx ? sx - nx : sx);
   sy = sy < 0 ? sy + ny : (sy >= ny ? sy - ny : sy);
   sz = sz < 0 ? sz + nz : (sz >= nz ? sz - nz : sz);
-  return ((int64_t)sx * ny + sy) * nz + sz;
+  return (uint32_t)((sx * ny + sy) * nz + sz);
 }
 
-template <int i>
-__device__ __forceinline__ bool th_src_xlow(uint32_t fl) { return ex_of(i) == 1 && (fl & F_FX0); }
-template <int i>
-__device__ __forceinline__ bool th_src_xhigh(uint32_t fl) { return ex_of(i) == -1 && (fl & F_FXN); }
-template <int i>
-__device__ __forceinline__ bool th_src_yz(uint32_t fl) {
-  return (ey_of(i) == 1 && (fl & F_FY0)) || (ey_of(i) == -1 && (fl & F_FYN)) ||
-         (ez_of(i) == 1 && (fl & F_FZ0)) || (ez_of(i) == -1 && (fl & F_FZN));
+// Face bits under which the source x - e_i of direction i lies outside a
+// non-periodic axis (the face flags are only set on non-periodic axes).
+__host__ __device__ constexpr uint32_t src_face_mask(int i) {
+  return (ex_of(i) == 1 ? F_FX0 : 0u) | (ex_of(i) == -1 ? F_FXN : 0u) | (ey_of(i) == 1 ? F_FY0 : 0u) |
+         (ey_of(i) == -1 ? F_FYN : 0u) | (ez_of(i) == 1 ? F_FZ0 : 0u) | (ez_of(i) == -1 ? F_FZN : 0u);
 }
-// Target c + e_i outside a non-periodic axis.
-template <int i>
-__device__ __forceinline__ bool th_dst_oob(uint32_t fl) {
-  return (ex_of(i) == 1 && (fl & F_FXN)) || (ex_of(i) == -1 && (fl & F_FX0)) ||
-         (ey_of(i) == 1 && (fl & F_FYN)) || (ey_of(i) == -1 && (fl & F_FY0)) ||
-         (ez_of(i) == 1 && (fl & F_FZN)) || (ez_of(i) == -1 && (fl & F_FZ0));
+
+// Per-cell context of one thread's cell.
+struct Cell {
+  uint32_t c;   // cell index within the slot
+  uint32_t fl;  // flag word
+  int type;
+  int x, y, z;  // coordinates (general mode only)
+  bool valid;
+};
+
+__device__ __forceinline__ void coords(uint32_t c, int P, int nz, int& x, int& y, int& z) {
+  x = (int)(c / (uint32_t)P);
+  const int r = (int)(c - (uint32_t)x * (uint32_t)P);
+  y = r / nz;
+  z = r - y * nz;
 }
 
 // ------------------------------------------------------------------ fluid ---
 
 template <class R, int GEN, int NODD, int i>
-__device__ __forceinline__ R fluid_load(const R* __restrict__ A, int64_t N, int64_t c, uint32_t fl, int P, int nz,
-                                        int x, int y, int zc, int nx, int ny, R u_wall) {
-  if (NODD == 0 || i == 0) return A[(int64_t)i * N + c];
-  const bool bb = (fl >> (F_BOUNCE0 + i)) & 1u;
-  int64_t src;
-  if (GEN)
-    src = nb_wrap<i>(x, y, zc, -1, nx, ny, nz);
-  else
-    src = c - nb_off<i>(P, nz);
-  const int64_t a = bb ? ((int64_t)i * N + c) : ((int64_t)opp_of(i) * N + src);
-  R v = A[a];
-  if (GEN && ez_of(i) == -1) {
-    if (bb && (fl & F_FZN)) v = v + wall_mom<R>(i, u_wall);
+__device__ __forceinline__ R fluid_load(const R* __restrict__ A, uint32_t N, const Cell& k, int P, int nz, int nx,
+                                        int ny, R u_wall) {
+  constexpr uint32_t rs = sizeof(R);
+  if (NODD == 0 || i == 0) return ldb<R>(reinterpret_cast<const char*>(A) + (size_t)i * N * rs, k.c * rs);
+  if (!GEN) {
+    // Pull from the neighbour's opposite slot, or (bounce-back) the own slot i.
+    const int64_t pull = (int64_t)opp_of(i) * N - nb_off<i>(P, nz);
+    return ld2<R, (i > opp_of(i))>(A, pull, (int64_t)i * N, k.c, bitmask(k.fl, F_BOUNCE0 + i));
+  }
+  const bool bb = (k.fl >> (F_BOUNCE0 + i)) & 1u;
+  const uint32_t src = nb_wrap<i>(k.x, k.y, k.z, -1, nx, ny, nz);
+  const uint32_t a = bb ? ((uint32_t)i * N + k.c) : ((uint32_t)opp_of(i) * N + src);
+  R v = ld(A, a);
+  if (ez_of(i) == -1) {
+    if (bb && (k.fl & F_FZN)) v = v + wall_mom<R>(i, u_wall);
   }
   return v;
 }
 
 template <class R, int GEN, int NODD, int i>
-__device__ __forceinline__ void fluid_store(R* __restrict__ A, int64_t N, int64_t c, uint32_t fl, int P, int nz, int x,
-                                            int y, int zc, int nx, int ny, R u_wall, R v) {
+__device__ __forceinline__ void fluid_store(R* __restrict__ A, uint32_t N, const Cell& k, int P, int nz, int nx,
+                                            int ny, R u_wall, R v) {
+  constexpr uint32_t rs = sizeof(R);
   if (NODD == 0 || i == 0) {
-    A[(int64_t)opp_of(i) * N + c] = v;
+    stb<R>(reinterpret_cast<char*>(A) + (size_t)opp_of(i) * N * rs, k.c * rs, v);
     return;
   }
-  const bool rb = (fl >> (F_BOUNCE0 + opp_of(i))) & 1u;  // target c + e_i is outside or SOLID
-  int64_t dst;
-  if (GEN)
-    dst = nb_wrap<i>(x, y, zc, +1, nx, ny, nz);
-  else
-    dst = c + nb_off<i>(P, nz);
-  if (GEN && ez_of(i) == 1) {
-    if (rb && (fl & F_FZN)) v = v + wall_mom<R>(opp_of(i), u_wall);
+  if (!GEN) {
+    // Push into the neighbour's slot i, or (target outside / SOLID) into the
+    // own opposite slot for the next pass's bounce-back.
+    const int64_t push = (int64_t)i * N + nb_off<i>(P, nz);
+    st2<R, (opp_of(i) > i)>(A, push, (int64_t)opp_of(i) * N, k.c, bitmask(k.fl, F_BOUNCE0 + opp_of(i)), v);
+    return;
   }
-  const int64_t a = rb ? ((int64_t)opp_of(i) * N + c) : ((int64_t)i * N + dst);
-  A[a] = v;
+  const bool rb = (k.fl >> (F_BOUNCE0 + opp_of(i))) & 1u;  // target c + e_i is outside or SOLID
+  const uint32_t dst = nb_wrap<i>(k.x, k.y, k.z, +1, nx, ny, nz);
+  if (ez_of(i) == 1) {
+    if (rb && (k.fl & F_FZN)) v = v + wall_mom<R>(opp_of(i), u_wall);
+  }
+  const uint32_t a = rb ? ((uint32_t)opp_of(i) * N + k.c) : ((uint32_t)i * N + dst);
+  st(A, a, v);
+}
+
+// Streamed fluid populations of a non-solid cell for the pass of parity NODD.
+template <class R, int GEN, int NODD>
+__device__ __forceinline__ void fluid_gather(const Slot<R>& S, const Cell& k, R* f) {
+  const int P = S.P, nz = S.nz;
+  const uint32_t N = (uint32_t)S.npad;
+  if (k.type == BC_INLET) {
+    // Velocity inlet from the cell's own post-collision set
+    // (src/fluid_engine3d.py:197-244); x = 0 so the plane index is c.
+#pragma unroll
+    for (int i = 0; i < 19; ++i) f[i] = ld(S.fin, (uint32_t)(i * P) + k.c);
+    const R uin = S.u_in;
+    const R rho_new = (f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18] +
+                       (R)2.0 * (f[2] + f[8] + f[10] + f[12] + f[14])) /
+                      ((R)1.0 - uin);
+    const R c23 = (R)(2.0 / 3.0), c16 = (R)(1.0 / 6.0);
+    f[1] = f[2] + c23 * rho_new * uin;
+    f[7] = f[10] + c16 * rho_new * uin;
+    f[9] = f[8] + c16 * rho_new * uin;
+    f[11] = f[14] + c16 * rho_new * uin;
+    f[13] = f[12] + c16 * rho_new * uin;
+  } else if (k.type == BC_OUTLET) {
+    // Pressure outlet: all 19 post-collision values of x-1, rescaled
+    // (src/fluid_engine3d.py:247-259).
+    const uint32_t p = k.c - (uint32_t)((S.nx - 1) * P);
+    const R* __restrict__ sn = S.fsnap + (1 - NODD) * 19 * P;
+    R rs = 0;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) {
+      f[i] = ld(sn, (uint32_t)(i * P) + p);
+      rs += f[i];
+    }
+    if (rs > (R)1e-6) {
+      const R sc = S.rho_out / rs;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) f[i] *= sc;
+    }
+  } else {
+    const R* __restrict__ A = S.f;
+    const R uw = GEN ? S.u_wall : (R)0;
+    const int nx = S.nx, ny = S.ny;
+#define XADV_FL(i) f[i] = fluid_load<R, GEN, NODD, i>(A, N, k, P, nz, nx, ny, uw);
+    XADV_FL(0) XADV_FL(1) XADV_FL(2) XADV_FL(3) XADV_FL(4) XADV_FL(5) XADV_FL(6) XADV_FL(7) XADV_FL(8)
+    XADV_FL(9) XADV_FL(10) XADV_FL(11) XADV_FL(12) XADV_FL(13) XADV_FL(14) XADV_FL(15) XADV_FL(16)
+    XADV_FL(17) XADV_FL(18)
+#undef XADV_FL
+  }
+}
+
+// Moments (src/fluid_engine3d.py:92-113), then BGK with the equilibrium in
+// registers (:157-173) and, in general mode, the body force
+// (validate_analytical.py:388-390).
+template <class R, int GEN>
+__device__ __forceinline__ void fluid_collide(const Slot<R>& S, R* f, R& lr, R& ux, R& uy, R& uz) {
+  // The reference sums f_i * e_i over i = 0..18; the zero terms only add
+  // signed zeros, so the sums keep its order over the non-zero terms.
+  lr = f[0] + f[1] + f[2] + f[3] + f[4] + f[5] + f[6] + f[7] + f[8] + f[9] + f[10] + f[11] + f[12] + f[13] +
+       f[14] + f[15] + f[16] + f[17] + f[18];
+  const R mx = f[1] - f[2] + f[7] - f[8] + f[9] - f[10] + f[11] - f[12] + f[13] - f[14];
+  const R my = f[3] - f[4] + f[7] - f[8] - f[9] + f[10] + f[15] - f[16] + f[17] - f[18];
+  const R mz = f[5] - f[6] + f[11] - f[12] - f[13] + f[14] + f[15] - f[16] - f[17] + f[18];
+  ux = uy = uz = 0;
+  if (lr > (R)1e-10) {
+    ux = mx / lr;
+    uy = my / lr;
+    uz = mz / lr;
+  }
+  const R om = S.omega;
+  const R usq = ux * ux + uy * uy + uz * uz;
+#define XADV_FC(i) f[i] = f[i] - om * (f[i] - feq_i<R, i>(lr, ux, uy, uz, usq));
+  XADV_FC(0) XADV_FC(1) XADV_FC(2) XADV_FC(3) XADV_FC(4) XADV_FC(5) XADV_FC(6) XADV_FC(7) XADV_FC(8)
+  XADV_FC(9) XADV_FC(10) XADV_FC(11) XADV_FC(12) XADV_FC(13) XADV_FC(14) XADV_FC(15) XADV_FC(16)
+  XADV_FC(17) XADV_FC(18)
+#undef XADV_FC
+  if (GEN) {
+    const R F = S.force_x;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) f[i] = f[i] + (R)w_of(i) * (R)3.0 * (R)ex_of(i) * F;
+  }
+}
+
+template <class R, int GEN, int NODD>
+__device__ __forceinline__ void fluid_scatter(const Slot<R>& S, const Cell& k, const R* f) {
+  R* __restrict__ A = S.f;
+  const uint32_t N = (uint32_t)S.npad;
+  const int P = S.P, nz = S.nz, nx = S.nx, ny = S.ny;
+  const R uw = GEN ? S.u_wall : (R)0;
+#define XADV_FS(i) fluid_store<R, GEN, NODD, i>(A, N, k, P, nz, nx, ny, uw, f[i]);
+  XADV_FS(0) XADV_FS(1) XADV_FS(2) XADV_FS(3) XADV_FS(4) XADV_FS(5) XADV_FS(6) XADV_FS(7) XADV_FS(8)
+  XADV_FS(9) XADV_FS(10) XADV_FS(11) XADV_FS(12) XADV_FS(13) XADV_FS(14) XADV_FS(15) XADV_FS(16)
+  XADV_FS(17) XADV_FS(18)
+#undef XADV_FS
+  if (k.type == BC_INLET) {
+#pragma unroll
+    for (int i = 0; i < 19; ++i) st(S.fin, (uint32_t)(i * P) + k.c, f[i]);
+  }
+  if (k.fl & F_SNAP) {
+    const uint32_t p = k.c - (uint32_t)((nx - 2) * P);
+    R* __restrict__ sn = S.fsnap + NODD * 19 * P;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) st(sn, (uint32_t)(i * P) + p, f[i]);
+  }
 }
 
 // ---------------------------------------------------------------- thermal ---
 
-// Streamed thermal populations of cell c for the pass of parity NODD (the
+// Streamed thermal populations of cell k for the pass of parity NODD (the
 // thermal half uses the opposite AA parity: TODD = !NODD).
 template <class R, int GEN, int NODD>
-__device__ __forceinline__ void thermal_gather(const Slot<R>& S, int64_t c, uint32_t fl, int type, int x, int y,
-                                               int zc, R* g) {
+__device__ __forceinline__ void thermal_gather(const Slot<R>& S, const Cell& k, R* g) {
   constexpr int TODD = 1 - NODD;
   const R* __restrict__ B = S.g;
-  const int64_t N = S.npad;
+  const uint32_t N = (uint32_t)S.npad;
   const int P = S.P, nz = S.nz;
-  if (type == BC_INLET) {  // thermal_engine3d.py:160-165
+  if (k.type == BC_INLET) {  // thermal_engine3d.py:160-165
 #pragma unroll
     for (int i = 0; i < 19; ++i) g[i] = (R)w_of(i) * S.T_in;
-  } else if (type == BC_OUTLET) {  // :170-177, values of x-1 published into gsnap
-    const int64_t p = c - (int64_t)(S.nx - 1) * P;
-    const R* __restrict__ sn = S.gsnap + (int64_t)NODD * 19 * P;
+  } else if (k.type == BC_OUTLET) {  // :170-177, values of x-1 published into gsnap
+    const uint32_t p = k.c - (uint32_t)((S.nx - 1) * P);
+    const R* __restrict__ sn = S.gsnap + NODD * 19 * P;
 #pragma unroll
-    for (int i = 0; i < 19; ++i) g[i] = sn[(int64_t)i * P + p];
+    for (int i = 0; i < 19; ++i) g[i] = ld(sn, (uint32_t)(i * P) + p);
   } else {
-    const int64_t pxh = c - (int64_t)(S.nx - 1) * P;
-#define XADV_TG(i)                                                                     \
-  {                                                                                    \
-    R v;                                                                               \
-    if (TODD == 0 || i == 0) {                                                         \
-      v = B[(int64_t)i * N + c];                                                       \
-    } else {                                                                           \
-      const bool oob = th_src_xlow<i>(fl) || th_src_xhigh<i>(fl) || th_src_yz<i>(fl);   \
-      int64_t src;                                                                     \
-      if (GEN)                                                                         \
-        src = nb_wrap<i>(x, y, zc, -1, S.nx, S.ny, nz);                                \
-      else                                                                             \
-        src = c - nb_off<i>(P, nz);                                                    \
-      const int64_t a = oob ? ((int64_t)i * N + c) : ((int64_t)opp_of(i) * N + src);   \
-      v = B[a];                                                                        \
-    }                                                                                  \
-    if (th_src_xlow<i>(fl)) v = (R)w_of(i) * S.T_in;                                   \
-    if (xm_slot(i) >= 0) {                                                             \
-      if (th_src_xhigh<i>(fl)) v = S.gxh[(int64_t)(xm_slot(i) < 0 ? 0 : xm_slot(i)) * P + pxh]; \
-    }                                                                                  \
-    g[i] = v;                                                                          \
+    const uint32_t fl = k.fl;
+    constexpr uint32_t rs = sizeof(R);
+#define XADV_TG(i)                                                                                          \
+  {                                                                                                         \
+    if (TODD == 0 || i == 0) {                                                                              \
+      g[i] = ldb<R>(reinterpret_cast<const char*>(B) + (size_t)i * N * rs, k.c * rs);                        \
+    } else if (!GEN) {                                                                                      \
+      const int64_t pull = (int64_t)opp_of(i) * N - nb_off<i>(P, nz);                                       \
+      g[i] = ld2<R, (i > opp_of(i))>(B, pull, (int64_t)i * N, k.c, anymask(fl, src_face_mask(i)));          \
+    } else {                                                                                                \
+      const bool oob = (fl & src_face_mask(i)) != 0;                                                        \
+      const uint32_t src = nb_wrap<i>(k.x, k.y, k.z, -1, S.nx, S.ny, nz);                                   \
+      g[i] = ld(B, oob ? ((uint32_t)i * N + k.c) : ((uint32_t)opp_of(i) * N + src));                        \
+    }                                                                                                       \
   }
     XADV_TG(0) XADV_TG(1) XADV_TG(2) XADV_TG(3) XADV_TG(4) XADV_TG(5) XADV_TG(6) XADV_TG(7) XADV_TG(8)
     XADV_TG(9) XADV_TG(10) XADV_TG(11) XADV_TG(12) XADV_TG(13) XADV_TG(14) XADV_TG(15) XADV_TG(16)
     XADV_TG(17) XADV_TG(18)
 #undef XADV_TG
+    if (fl & (F_FX0 | F_FXN)) {
+      // x = 0: e_x = +1 sources lie before the inlet (w_i T_in); x = nx-1:
+      // e_x = -1 sources lie past the outlet (the cell's own values, kept in
+      // gxh).  These rules take precedence over the y/z walls
+      // (thermal_engine3d.py:193-207).
+      if (fl & F_FX0) {
+        g[1] = (R)w_of(1) * S.T_in;
+        g[7] = (R)w_of(7) * S.T_in;
+        g[9] = (R)w_of(9) * S.T_in;
+        g[11] = (R)w_of(11) * S.T_in;
+        g[13] = (R)w_of(13) * S.T_in;
+      }
+      if (fl & F_FXN) {
+        const uint32_t pxh = k.c - (uint32_t)((S.nx - 1) * P);
+        g[2] = ld(S.gxh, 0u * P + pxh);
+        g[8] = ld(S.gxh, 1u * P + pxh);
+        g[10] = ld(S.gxh, 2u * P + pxh);
+        g[12] = ld(S.gxh, 3u * P + pxh);
+        g[14] = ld(S.gxh, 4u * P + pxh);
+      }
+    }
   }
-  if (GEN && (fl & F_DIR)) {  // validate_analytical.py:21-30, applied after streaming
+  if (GEN && (k.fl & F_DIR)) {  // validate_analytical.py:21-30, applied after streaming
 #pragma unroll
     for (int i = 0; i < 19; ++i) g[i] = (R)w_of(i) * S.dir_T;
   }
 }
 
 template <class R, int GEN, int NODD, int i>
-__device__ __forceinline__ void thermal_store(R* __restrict__ B, int64_t N, int64_t c, uint32_t fl, int P, int nz,
-                                              int x, int y, int zc, int nx, int ny, R v) {
+__device__ __forceinline__ void thermal_store(R* __restrict__ B, uint32_t N, const Cell& k, int P, int nz, int nx,
+                                              int ny, R v) {
   constexpr int TODD = 1 - NODD;
+  constexpr uint32_t rs = sizeof(R);
   if (TODD == 0 || i == 0) {
-    B[(int64_t)opp_of(i) * N + c] = v;
+    stb<R>(reinterpret_cast<char*>(B) + (size_t)opp_of(i) * N * rs, k.c * rs, v);
     return;
   }
-  const bool oob = th_dst_oob<i>(fl);
-  int64_t dst;
-  if (GEN)
-    dst = nb_wrap<i>(x, y, zc, +1, nx, ny, nz);
-  else
-    dst = c + nb_off<i>(P, nz);
-  const int64_t a = oob ? ((int64_t)opp_of(i) * N + c) : ((int64_t)i * N + dst);
-  B[a] = v;
+  // Target c + e_i outside a non-periodic axis = source of opp(i) outside.
+  if (!GEN) {
+    const int64_t push = (int64_t)i * N + nb_off<i>(P, nz);
+    st2<R, (opp_of(i) > i)>(B, push, (int64_t)opp_of(i) * N, k.c, anymask(k.fl, src_face_mask(opp_of(i))), v);
+    return;
+  }
+  const bool oob = (k.fl & src_face_mask(opp_of(i))) != 0;
+  const uint32_t dst = nb_wrap<i>(k.x, k.y, k.z, +1, nx, ny, nz);
+  st(B, oob ? ((uint32_t)opp_of(i) * N + k.c) : ((uint32_t)i * N + dst), v);
 }
 
+// BGK with the source term (thermal_engine3d.py:107-145).
 template <class R>
-__device__ __forceinline__ void coords(int64_t c, int P, int nz, int& x, int& y, int& z) {
-  x = (int)(c / P);
-  const int r = (int)(c - (int64_t)x * P);
-  y = r / nz;
-  z = r - y * nz;
+__device__ __forceinline__ void thermal_collide(const Slot<R>& S, const Cell& k, R* g, R Tl, R ux, R uy, R uz) {
+  const bool solid = k.type == BC_SOLID;
+  if (solid) ux = uy = uz = 0;  // thermal_engine3d.py:123-125
+  const R om = solid ? S.omega_ts : S.omega_tf;
+  const R sf = solid ? S.sf_ts : S.sf_tf;
+  const R q = (k.fl & F_HEAT) ? ld(S.q, k.c) : (R)0;
+  const R usq = ux * ux + uy * uy + uz * uz;
+#define XADV_TC(i)                                               \
+  {                                                              \
+    R v = g[i] - om * (g[i] - feq_i<R, i>(Tl, ux, uy, uz, usq)); \
+    if (q > (R)0.0) v += (R)w_of(i) * sf * q;                    \
+    g[i] = v;                                                    \
+  }
+  XADV_TC(0) XADV_TC(1) XADV_TC(2) XADV_TC(3) XADV_TC(4) XADV_TC(5) XADV_TC(6) XADV_TC(7) XADV_TC(8)
+  XADV_TC(9) XADV_TC(10) XADV_TC(11) XADV_TC(12) XADV_TC(13) XADV_TC(14) XADV_TC(15) XADV_TC(16)
+  XADV_TC(17) XADV_TC(18)
+#undef XADV_TC
+}
+
+template <class R, int GEN, int NODD>
+__device__ __forceinline__ void thermal_scatter(const Slot<R>& S, const Cell& k, const R* g) {
+  R* __restrict__ B = S.g;
+  const uint32_t N = (uint32_t)S.npad;
+  const int P = S.P, nz = S.nz, nx = S.nx, ny = S.ny;
+#define XADV_TS(i) thermal_store<R, GEN, NODD, i>(B, N, k, P, nz, nx, ny, g[i]);
+  XADV_TS(0) XADV_TS(1) XADV_TS(2) XADV_TS(3) XADV_TS(4) XADV_TS(5) XADV_TS(6) XADV_TS(7) XADV_TS(8)
+  XADV_TS(9) XADV_TS(10) XADV_TS(11) XADV_TS(12) XADV_TS(13) XADV_TS(14) XADV_TS(15) XADV_TS(16)
+  XADV_TS(17) XADV_TS(18)
+#undef XADV_TS
+  if (k.fl & F_XH) {
+    const uint32_t p = k.c - (uint32_t)((nx - 1) * P);
+    st(S.gxh, 0u * P + p, g[2]);
+    st(S.gxh, 1u * P + p, g[8]);
+    st(S.gxh, 2u * P + p, g[10]);
+    st(S.gxh, 3u * P + p, g[12]);
+    st(S.gxh, 4u * P + p, g[14]);
+  }
+  if (k.fl & F_SNAP) {
+    const uint32_t p = k.c - (uint32_t)((nx - 2) * P);
+    R* __restrict__ sn = S.gsnap + (1 - NODD) * 19 * P;
+#pragma unroll
+    for (int i = 0; i < 19; ++i) st(sn, (uint32_t)(i * P) + p, g[i]);
+  }
 }
 
 // ------------------------------------------------------------- the pass ----
 
-template <class R, int FL, int TH, int GEN, int NODD>
+// One time step for CPT cells per thread.  The phases (fluid gather, fluid
+// collide + scatter, thermal gather, thermal collide + scatter) run for all
+// of a thread's cells before the next phase, so each thread keeps
+// CPT x 19 loads in flight.  Cell k of a thread is block_base + k*blockDim +
+// threadIdx, so every load instruction of a wavefront covers 64 consecutive
+// cells.
+template <class R, int FL, int TH, int GEN, int NODD, int CPT>
 __global__ __launch_bounds__(256) void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int xcd,
                                               int want_fields) {
   const int b = map_block(G, xcd);
@@ -262,196 +505,108 @@ __global__ __launch_bounds__(256) void k_pass(Slot<R>* __restrict__ slots, int b
     else if (lb == 0 && threadIdx.x == 0 && S.raw)
       S.raw = 0;
   }
-  const int64_t c = (int64_t)lb * blockDim.x + threadIdx.x;
-  if (c >= S.ncell) return;
+  const uint32_t ncell = (uint32_t)S.ncell;
+  const uint32_t base = (uint32_t)lb * blockDim.x * CPT + threadIdx.x;
+  if (base >= ncell) return;
 
-  const uint32_t fl = S.flags[c];
-  const int type = (int)(fl & F_TYPE);
-  const int P = S.P, nz = S.nz, nx = S.nx, ny = S.ny;
-  const int64_t N = S.npad;
-  int x = 0, y = 0, zc = 0;
-  if (GEN) coords<R>(c, P, nz, x, y, zc);
-
-  R ux = 0, uy = 0, uz = 0;
-  if (FL) {
-    R* __restrict__ A = S.f;
-    if (type != BC_SOLID) {
-      R f[19];
-      if (type == BC_INLET) {
-        // Velocity inlet from the cell's own post-collision set
-        // (src/fluid_engine3d.py:197-244); x = 0 so the plane index is c.
-        const R* __restrict__ fi = S.fin;
+  Cell cell[CPT];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = fi[(int64_t)i * P + c];
-        const R uin = S.u_in;
-        const R rho_new = (f[0] + f[3] + f[4] + f[5] + f[6] + f[15] + f[16] + f[17] + f[18] +
-                           (R)2.0 * (f[2] + f[8] + f[10] + f[12] + f[14])) /
-                          ((R)1.0 - uin);
-        const R c23 = (R)(2.0 / 3.0), c16 = (R)(1.0 / 6.0);
-        f[1] = f[2] + c23 * rho_new * uin;
-        f[7] = f[10] + c16 * rho_new * uin;
-        f[9] = f[8] + c16 * rho_new * uin;
-        f[11] = f[14] + c16 * rho_new * uin;
-        f[13] = f[12] + c16 * rho_new * uin;
-      } else if (type == BC_OUTLET) {
-        // Pressure outlet: all 19 post-collision values of x-1, rescaled
-        // (src/fluid_engine3d.py:247-259).
-        const int64_t p = c - (int64_t)(nx - 1) * P;
-        const R* __restrict__ sn = S.fsnap + (int64_t)(1 - NODD) * 19 * P;
-        R rs = 0;
+  for (int k = 0; k < CPT; ++k) {
+    Cell& q = cell[k];
+    q.c = base + (uint32_t)k * blockDim.x;
+    q.valid = q.c < ncell;
+    q.fl = q.valid ? ld(S.flags, q.c) : (uint32_t)BC_SOLID;
+    q.type = (int)(q.fl & F_TYPE);
+    q.x = q.y = q.z = 0;
+    if (GEN && q.valid) coords(q.c, S.P, S.nz, q.x, q.y, q.z);
+  }
+
+  R ux[CPT], uy[CPT], uz[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) ux[k] = uy[k] = uz[k] = 0;
+
+  if (FL) {
+    R f[CPT][19];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+      if (cell[k].valid && cell[k].type != BC_SOLID) fluid_gather<R, GEN, NODD>(S, cell[k], f[k]);
+#pragma unroll
+    for (int k = 0; k < CPT; ++k) {
+      const Cell& q = cell[k];
+      if (!q.valid) continue;
+      if (q.type != BC_SOLID) {
+        R lr;
+        fluid_collide<R, GEN>(S, f[k], lr, ux[k], uy[k], uz[k]);
+        if (want_fields) {
+          st(S.rho, q.c, lr);
+          st(S.u, 3 * q.c, ux[k]);
+          st(S.u, 3 * q.c + 1, uy[k]);
+          st(S.u, 3 * q.c + 2, uz[k]);
+        }
+        fluid_scatter<R, GEN, NODD>(S, q, f[k]);
+      } else if (want_fields) {
+        // Solid cells keep their initial populations, stored in the even
+        // layout (slot opp(i)); the reference's moment kernel still runs on them.
+        const uint32_t N = (uint32_t)S.npad;
+        R lr = 0, mx = 0, my = 0, mz = 0;
 #pragma unroll
         for (int i = 0; i < 19; ++i) {
-          f[i] = sn[(int64_t)i * P + p];
-          rs += f[i];
+          const R v = ld(S.f, (uint32_t)opp_of(i) * N + q.c);
+          lr += v;
+          mx += v * (R)ex_of(i);
+          my += v * (R)ey_of(i);
+          mz += v * (R)ez_of(i);
         }
-        if (rs > (R)1e-6) {
-          const R sc = S.rho_out / rs;
-#pragma unroll
-          for (int i = 0; i < 19; ++i) f[i] *= sc;
+        R sx = 0, sy = 0, sz = 0;
+        if (lr > (R)1e-10) {
+          sx = mx / lr;
+          sy = my / lr;
+          sz = mz / lr;
         }
-      } else {
-        const R uw = GEN ? S.u_wall : (R)0;
-#define XADV_FL(i) f[i] = fluid_load<R, GEN, NODD, i>(A, N, c, fl, P, nz, x, y, zc, nx, ny, uw);
-        XADV_FL(0) XADV_FL(1) XADV_FL(2) XADV_FL(3) XADV_FL(4) XADV_FL(5) XADV_FL(6) XADV_FL(7) XADV_FL(8)
-        XADV_FL(9) XADV_FL(10) XADV_FL(11) XADV_FL(12) XADV_FL(13) XADV_FL(14) XADV_FL(15) XADV_FL(16)
-        XADV_FL(17) XADV_FL(18)
-#undef XADV_FL
+        st(S.rho, q.c, lr);
+        st(S.u, 3 * q.c, sx);
+        st(S.u, 3 * q.c + 1, sy);
+        st(S.u, 3 * q.c + 2, sz);
       }
-      // Moments (src/fluid_engine3d.py:92-113)
-      R lr = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-      for (int i = 0; i < 19; ++i) {
-        lr += f[i];
-        mx += f[i] * (R)ex_of(i);
-        my += f[i] * (R)ey_of(i);
-        mz += f[i] * (R)ez_of(i);
-      }
-      if (lr > (R)1e-10) {
-        ux = mx / lr;
-        uy = my / lr;
-        uz = mz / lr;
-      }
-      if (want_fields) {
-        S.rho[c] = lr;
-        S.u[3 * c] = ux;
-        S.u[3 * c + 1] = uy;
-        S.u[3 * c + 2] = uz;
-      }
-      // BGK with the equilibrium in registers (src/fluid_engine3d.py:157-173)
-      const R om = S.omega;
-      const R usq = ux * ux + uy * uy + uz * uz;
-#define XADV_FC(i) f[i] = f[i] - om * (f[i] - feq_i<R, i>(lr, ux, uy, uz, usq));
-      XADV_FC(0) XADV_FC(1) XADV_FC(2) XADV_FC(3) XADV_FC(4) XADV_FC(5) XADV_FC(6) XADV_FC(7) XADV_FC(8)
-      XADV_FC(9) XADV_FC(10) XADV_FC(11) XADV_FC(12) XADV_FC(13) XADV_FC(14) XADV_FC(15) XADV_FC(16)
-      XADV_FC(17) XADV_FC(18)
-#undef XADV_FC
-      if (GEN) {  // validate_analytical.py:388-390
-        const R F = S.force_x;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) f[i] = f[i] + (R)w_of(i) * (R)3.0 * (R)ex_of(i) * F;
-      }
-      {
-        const R uw = GEN ? S.u_wall : (R)0;
-#define XADV_FS(i) fluid_store<R, GEN, NODD, i>(A, N, c, fl, P, nz, x, y, zc, nx, ny, uw, f[i]);
-        XADV_FS(0) XADV_FS(1) XADV_FS(2) XADV_FS(3) XADV_FS(4) XADV_FS(5) XADV_FS(6) XADV_FS(7) XADV_FS(8)
-        XADV_FS(9) XADV_FS(10) XADV_FS(11) XADV_FS(12) XADV_FS(13) XADV_FS(14) XADV_FS(15) XADV_FS(16)
-        XADV_FS(17) XADV_FS(18)
-#undef XADV_FS
-      }
-      if (type == BC_INLET) {
-        R* __restrict__ fi = S.fin;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) fi[(int64_t)i * P + c] = f[i];
-      }
-      if (fl & F_SNAP) {
-        const int64_t p = c - (int64_t)(nx - 2) * P;
-        R* __restrict__ sn = S.fsnap + (int64_t)NODD * 19 * P;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) sn[(int64_t)i * P + p] = f[i];
-      }
-    } else if (want_fields) {
-      // Solid cells keep their initial populations, stored in the even layout
-      // (slot opp(i)); the reference's moment kernel still runs on them.
-      R lr = 0, mx = 0, my = 0, mz = 0;
-#pragma unroll
-      for (int i = 0; i < 19; ++i) {
-        const R v = A[(int64_t)opp_of(i) * N + c];
-        lr += v;
-        mx += v * (R)ex_of(i);
-        my += v * (R)ey_of(i);
-        mz += v * (R)ez_of(i);
-      }
-      R sx = 0, sy = 0, sz = 0;
-      if (lr > (R)1e-10) {
-        sx = mx / lr;
-        sy = my / lr;
-        sz = mz / lr;
-      }
-      S.rho[c] = lr;
-      S.u[3 * c] = sx;
-      S.u[3 * c + 1] = sy;
-      S.u[3 * c + 2] = sz;
     }
   } else if (TH) {
     // Thermal lattice alone: collide with the velocity field given at init.
-    ux = S.u[3 * c];
-    uy = S.u[3 * c + 1];
-    uz = S.u[3 * c + 2];
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+      if (cell[k].valid) {
+        ux[k] = ld(S.u, 3 * cell[k].c);
+        uy[k] = ld(S.u, 3 * cell[k].c + 1);
+        uz[k] = ld(S.u, 3 * cell[k].c + 2);
+      }
   }
 
   if (TH) {
-    R g[19];
-    R Tl;
-    if (raw) {
-      // First thermal half: the initial populations and T as given
-      // (src/simulation.py:73,89), natural layout.
-      const R* __restrict__ B = S.g;
+    R g[CPT][19];
+    R Tl[CPT];
 #pragma unroll
-      for (int i = 0; i < 19; ++i) g[i] = B[(int64_t)i * N + c];
-      Tl = S.T[c];
-    } else {
-      thermal_gather<R, GEN, NODD>(S, c, fl, type, x, y, zc, g);
-      Tl = 0;
+    for (int k = 0; k < CPT; ++k) {
+      if (!cell[k].valid) continue;
+      if (raw) {
+        // First thermal half: the initial populations and T as given
+        // (src/simulation.py:73,89), natural layout.
+        const uint32_t N = (uint32_t)S.npad;
 #pragma unroll
-      for (int i = 0; i < 19; ++i) Tl += g[i];  // thermal_engine3d.py:220-224
+        for (int i = 0; i < 19; ++i) g[k][i] = ld(S.g, (uint32_t)i * N + cell[k].c);
+        Tl[k] = ld(S.T, cell[k].c);
+      } else {
+        thermal_gather<R, GEN, NODD>(S, cell[k], g[k]);
+      }
     }
-    const bool solid = type == BC_SOLID;
-    if (solid) ux = uy = uz = 0;  // thermal_engine3d.py:123-125
-    const R om = solid ? S.omega_ts : S.omega_tf;
-    const R sf = solid ? S.sf_ts : S.sf_tf;
-    const R q = (fl & F_HEAT) ? S.q[c] : (R)0;
-    const R usq = ux * ux + uy * uy + uz * uz;
-    // thermal_engine3d.py:128-145
-#define XADV_TC(i)                                                  \
-  {                                                                 \
-    R v = g[i] - om * (g[i] - feq_i<R, i>(Tl, ux, uy, uz, usq));    \
-    if (q > (R)0.0) v += (R)w_of(i) * sf * q;                       \
-    g[i] = v;                                                       \
-  }
-    XADV_TC(0) XADV_TC(1) XADV_TC(2) XADV_TC(3) XADV_TC(4) XADV_TC(5) XADV_TC(6) XADV_TC(7) XADV_TC(8)
-    XADV_TC(9) XADV_TC(10) XADV_TC(11) XADV_TC(12) XADV_TC(13) XADV_TC(14) XADV_TC(15) XADV_TC(16)
-    XADV_TC(17) XADV_TC(18)
-#undef XADV_TC
-    R* __restrict__ B = S.g;
-#define XADV_TS(i) thermal_store<R, GEN, NODD, i>(B, N, c, fl, P, nz, x, y, zc, nx, ny, g[i]);
-    XADV_TS(0) XADV_TS(1) XADV_TS(2) XADV_TS(3) XADV_TS(4) XADV_TS(5) XADV_TS(6) XADV_TS(7) XADV_TS(8)
-    XADV_TS(9) XADV_TS(10) XADV_TS(11) XADV_TS(12) XADV_TS(13) XADV_TS(14) XADV_TS(15) XADV_TS(16)
-    XADV_TS(17) XADV_TS(18)
-#undef XADV_TS
-    if (fl & F_XH) {
-      const int64_t p = c - (int64_t)(nx - 1) * P;
-      R* __restrict__ xh = S.gxh;
-      xh[0 * (int64_t)P + p] = g[2];
-      xh[1 * (int64_t)P + p] = g[8];
-      xh[2 * (int64_t)P + p] = g[10];
-      xh[3 * (int64_t)P + p] = g[12];
-      xh[4 * (int64_t)P + p] = g[14];
-    }
-    if (fl & F_SNAP) {
-      const int64_t p = c - (int64_t)(nx - 2) * P;
-      R* __restrict__ sn = S.gsnap + (int64_t)(1 - NODD) * 19 * P;
 #pragma unroll
-      for (int i = 0; i < 19; ++i) sn[(int64_t)i * P + p] = g[i];
+    for (int k = 0; k < CPT; ++k) {
+      if (!cell[k].valid) continue;
+      if (!raw) {  // thermal_engine3d.py:220-224, in order from g_0
+        Tl[k] = g[k][0];
+#pragma unroll
+        for (int i = 1; i < 19; ++i) Tl[k] += g[k][i];
+      }
+      thermal_collide<R>(S, cell[k], g[k], Tl[k], ux[k], uy[k], uz[k]);
+      thermal_scatter<R, GEN, NODD>(S, cell[k], g[k]);
     }
   }
 }
@@ -572,23 +727,27 @@ __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, in
   const int lb = b - si * bps;
   Slot<R>& S = slots[si];
   if (!S.active) return;
-  const int64_t c = (int64_t)lb * blockDim.x + threadIdx.x;
-  const bool valid = c < S.ncell;
+  const uint32_t c = (uint32_t)lb * blockDim.x + threadIdx.x;
+  const bool valid = c < (uint32_t)S.ncell;
   double v[16];
 #pragma unroll
   for (int k = 0; k < 16; ++k) v[k] = 0.0;
   if (valid) {
-    const uint32_t fl = S.flags[c];
-    const int type = (int)(fl & F_TYPE);
-    int x, y, zc;
-    coords<R>(c, S.P, S.nz, x, y, zc);
+    Cell q;
+    q.c = c;
+    q.valid = true;
+    q.fl = ld(S.flags, c);
+    q.type = (int)(q.fl & F_TYPE);
+    coords(c, S.P, S.nz, q.x, q.y, q.z);
+    const int type = q.type;
+    const int x = q.x;
     R Tl = 0;
     if (TH) {
       R g[19];
-      thermal_gather<R, GEN, NODD>(S, c, fl, type, x, y, zc, g);
+      thermal_gather<R, GEN, NODD>(S, q, g);
 #pragma unroll
       for (int i = 0; i < 19; ++i) Tl += g[i];
-      S.T[c] = Tl;
+      st(S.T, c, Tl);
     }
     if (with_stats) {
       const float ux = (float)S.u[3 * c], uy = (float)S.u[3 * c + 1], uz = (float)S.u[3 * c + 2];
@@ -663,7 +822,7 @@ template <class R>
 __global__ __launch_bounds__(256) void k_metrics(Slot<R>* __restrict__ slots, int slot, int x0, int cnx, int cny,
                                                  double* __restrict__ partials) {
   const Slot<R>& S = slots[slot];
-  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t c = (uint32_t)blockIdx.x * blockDim.x + threadIdx.x;
   double v[kNMet];
   int op[kNMet];
 #pragma unroll
@@ -671,9 +830,9 @@ __global__ __launch_bounds__(256) void k_metrics(Slot<R>* __restrict__ slots, in
     op[k] = kMetOp[k];
     v[k] = op[k] == 1 ? -INFINITY : (op[k] == 2 ? INFINITY : 0.0);
   }
-  if (c < S.ncell) {
+  if (c < (uint32_t)S.ncell) {
     int x, y, z;
-    coords<R>(c, S.P, S.nz, x, y, z);
+    coords(c, S.P, S.nz, x, y, z);
     const int type = (int)(S.flags[c] & F_TYPE);
     const R r = S.rho[c];
     const float ux = (float)S.u[3 * c], uy = (float)S.u[3 * c + 1], uz = (float)S.u[3 * c + 2];

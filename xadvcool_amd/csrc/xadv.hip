@@ -34,6 +34,8 @@ int fail(int code, const std::string& msg) {
 
 int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
 
+int cleanup_early(xadv_batch* b, int rc);
+
 }  // namespace
 
 struct xadv_batch {
@@ -41,7 +43,8 @@ struct xadv_batch {
   hipStream_t stream = nullptr;
   int64_t cap_cells = 0, cap_npad = 0;
   int cap_P = 0;
-  int bps = 0, G = 0, G_launch = 0;
+  int bps = 0, G = 0, G_launch = 0;        // pass grid (block * cpt cells per block)
+  int bps_r = 0, G_r = 0, G_r_launch = 0;  // readout grid (one cell per thread)
   size_t rsize = 4;
   void* slots_dev = nullptr;
   void* slots_host = nullptr;  // pinned copy of the slot table
@@ -96,27 +99,35 @@ int push_field(xadv_batch* b, int s, size_t off, size_t bytes) {
 
 // ----------------------------------------------------------- dispatch -----
 
-template <class R, int FL, int TH, int GEN>
-void launch_pass_t(xadv_batch* b, int nodd, int want_fields) {
+template <class R, int FL, int TH, int GEN, int CPT>
+void launch_pass_c(xadv_batch* b, int nodd, int want_fields) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
   dim3 grid(b->G_launch), block(b->d.block);
   if (nodd)
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps, b->G, b->d.xcd_remap,
-                       want_fields);
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT>), grid, block, 0, b->stream, sl, b->bps, b->G,
+                       b->d.xcd_remap, want_fields);
   else
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps, b->G, b->d.xcd_remap,
-                       want_fields);
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT>), grid, block, 0, b->stream, sl, b->bps, b->G,
+                       b->d.xcd_remap, want_fields);
+}
+
+template <class R, int FL, int TH, int GEN>
+void launch_pass_t(xadv_batch* b, int nodd, int want_fields) {
+  if (b->d.cells_per_thread == 2)
+    launch_pass_c<R, FL, TH, GEN, 2>(b, nodd, want_fields);
+  else
+    launch_pass_c<R, FL, TH, GEN, 1>(b, nodd, want_fields);
 }
 
 template <class R, int FL, int TH, int GEN>
 void launch_readout_t(xadv_batch* b, int nodd, int with_stats) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
-  dim3 grid(b->G_launch), block(b->d.block);
+  dim3 grid(b->G_r_launch), block(b->d.block);
   if (nodd)
-    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps, b->G,
+    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps_r, b->G_r,
                        b->d.xcd_remap, with_stats, b->partials);
   else
-    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps, b->G,
+    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps_r, b->G_r,
                        b->d.xcd_remap, with_stats, b->partials);
 }
 
@@ -297,6 +308,13 @@ int init_t(xadv_batch* b, int s, const void* rho, const void* u, const void* T) 
 
 }  // namespace
 
+namespace {
+int cleanup_early(xadv_batch* b, int rc) {
+  delete b;
+  return rc;
+}
+}  // namespace
+
 // ================================================================ C ABI ====
 
 extern "C" {
@@ -321,15 +339,24 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   if (d.block == 0) d.block = 256;
   if (d.block != 64 && d.block != 128 && d.block != 256) return fail(XADV_E_ARG, "block must be 64, 128 or 256");
   if (d.graph_chunk < 0 || (d.graph_chunk % 2)) return fail(XADV_E_ARG, "graph_chunk must be even and >= 0");
+  if (d.cells_per_thread == 0) d.cells_per_thread = 2;
+  if (d.cells_per_thread != 1 && d.cells_per_thread != 2) return fail(XADV_E_ARG, "cells_per_thread must be 1 or 2");
   auto* b = new xadv_batch();
   b->d = d;
   b->rsize = d.precision == 64 ? 8 : 4;
   b->cap_cells = (int64_t)d.cap_nx * d.cap_ny * d.cap_nz;
   b->cap_npad = round_up(b->cap_cells, 64);
   b->cap_P = d.cap_ny * d.cap_nz;
-  b->bps = (int)((b->cap_cells + d.block - 1) / d.block);
+  // Every lattice array of a slot is addressed with 32-bit byte offsets.
+  if ((double)19 * round_up(b->cap_cells, 64) * b->rsize >= 4294967296.0)
+    return cleanup_early(b, fail(XADV_E_ARG, "slot capacity too large for 32-bit addressing (max ~56M cells FP32)"));
+  const int cpb = d.block * d.cells_per_thread;
+  b->bps = (int)((b->cap_cells + cpb - 1) / cpb);
   b->G = b->bps * d.slots;
   b->G_launch = d.xcd_remap ? ((b->G + 7) / 8) * 8 : b->G;
+  b->bps_r = (int)((b->cap_cells + d.block - 1) / d.block);
+  b->G_r = b->bps_r * d.slots;
+  b->G_r_launch = d.xcd_remap ? ((b->G_r + 7) / 8) * 8 : b->G_r;
   b->slot_bytes = d.precision == 64 ? sizeof(Slot<double>) : sizeof(Slot<float>);
   b->loaded.assign(d.slots, 0);
   b->inited.assign(d.slots, 0);
@@ -367,7 +394,7 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
       return cleanup(fail(XADV_E_HIP, std::string("hipMemset: ") + hipGetErrorString(e)));
   }
   const int nmb = (int)((b->cap_cells + 255) / 256);
-  if ((e = hipMalloc((void**)&b->partials, (size_t)b->G * 16 * sizeof(double))) != hipSuccess ||
+  if ((e = hipMalloc((void**)&b->partials, (size_t)b->G_r * 16 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc((void**)&b->stats_dev, S * 16 * sizeof(double))) != hipSuccess ||
       (e = hipMalloc((void**)&b->met_partials, (size_t)nmb * xadv::kNMet * sizeof(double))) != hipSuccess ||
       (e = hipMalloc((void**)&b->met_dev, xadv::kNMet * sizeof(double))) != hipSuccess ||
@@ -495,10 +522,10 @@ int xadv_readout(xadv_batch* b, int with_stats) {
   if (with_stats) {
     if (b->d.precision == 64)
       hipLaunchKernelGGL((xadv::k_stats_final<double>), dim3(b->d.slots), dim3(64), 0, b->stream,
-                         reinterpret_cast<const Slot<double>*>(b->slots_dev), b->bps, b->partials, b->stats_dev);
+                         reinterpret_cast<const Slot<double>*>(b->slots_dev), b->bps_r, b->partials, b->stats_dev);
     else
       hipLaunchKernelGGL((xadv::k_stats_final<float>), dim3(b->d.slots), dim3(64), 0, b->stream,
-                         reinterpret_cast<const Slot<float>*>(b->slots_dev), b->bps, b->partials, b->stats_dev);
+                         reinterpret_cast<const Slot<float>*>(b->slots_dev), b->bps_r, b->partials, b->stats_dev);
     HIPCHK(hipGetLastError());
   }
   return XADV_OK;
