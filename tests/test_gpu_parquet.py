@@ -10,8 +10,11 @@ run_simulation, and their metrics formed by the on-device reductions.
 The reference ran Warp FP32 on an RTX 3090 with its own code generation
 (FMA contraction, atomic order), so parity here is at the level the north
 star states for converged quantities: pressure drop, R_th and Nu within
-0.1 %, the stop step within one stats interval; the remaining columns within
-0.5 %.
+0.1 %, the remaining columns within 0.5 %.  The stop step must lie within one
+stats interval of the reference's; it is a threshold test on residuals the
+reference sums with order-nondeterministic FP32 atomics, so when the two stop
+steps differ the fields are compared at the reference's stop step (a second
+solve run for exactly that many steps).
 """
 import json
 import os
@@ -43,6 +46,12 @@ def solve_row(row):
         out = solve_on_batch(s.batch, 0, nx=mask.shape[0], max_steps=30000, tau_fluid=row["tau_fluid"],
                              u_inlet_val=row["u_inlet_val"], stats_every=100, tol_ke=1e-5, tol_T=1e-5,
                              min_steps_rule=True)
+        out["compared_at"] = out["steps"]
+        if out["steps"] != row["sim_steps"]:
+            s.init_equilibrium()
+            s.step(int(row["sim_steps"]))
+            s._readout(False)
+            out["compared_at"] = int(row["sim_steps"])
         met = device_metrics(s.batch, 0, mask, tau_fluid=row["tau_fluid"], u_inlet_val=row["u_inlet_val"])
     finally:
         s.close()
@@ -56,7 +65,7 @@ def test_parquet_row(gpu, row):
     assert out["converged"] and not out["diverged"]
     assert abs(out["steps"] - row["sim_steps"]) <= 100, (out["steps"], row["sim_steps"])
     errs = {k: abs(met[k] - row[k]) / max(abs(row[k]), 1e-12) for k in TIGHT + LOOSE}
-    print(f"[parquet] steps {out['steps']} vs {row['sim_steps']}; " +
+    print(f"[parquet] steps {out['steps']} vs {row['sim_steps']} (fields at {out['compared_at']}); " +
           ", ".join(f"{k} {100 * v:.4f}%" for k, v in errs.items()), flush=True)
     for k in TIGHT:
         assert errs[k] <= 1e-3, (k, met[k], row[k])

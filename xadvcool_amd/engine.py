@@ -40,7 +40,7 @@ class LatticeBatch:
     """One batch of lattice slots advanced together by the fused HIP pass."""
 
     def __init__(self, *, slots=1, cap_shape, fluid=True, thermal=True, general=False, precision=32, block=256,
-                 xcd_remap=True, graph_chunk=100, cells_per_thread=2):
+                 xcd_remap=False, graph_chunk=100, cells_per_thread=1):
         _require_gpu()
         self._lib = _lib.load()
         if graph_chunk and graph_chunk % 2:
@@ -163,8 +163,8 @@ class LatticeSolver:
 
     def __init__(self, mask, *, fluid=True, thermal=True, heat=None, tau=0.6, tau_th_fluid=0.6, tau_th_solid=0.6,
                  heat_power=0.0, T_in=0.0, rho_out=1.0, u_in=0.0, periodic=(0, 0, 0), force_x=0.0, u_wall=0.0,
-                 dirichlet=None, frozen_velocity=False, precision=32, block=256, xcd_remap=True, graph_chunk=100,
-                 cells_per_thread=2):
+                 dirichlet=None, frozen_velocity=False, precision=32, block=256, xcd_remap=False, graph_chunk=100,
+                 cells_per_thread=1):
         mask = np.asarray(mask)
         general = any(periodic) or force_x != 0.0 or u_wall != 0.0 or dirichlet is not None
         self.mask = mask
