@@ -1,0 +1,51 @@
+/* xadv_nn.h — C ABI of the warm-start surrogate kernels (libxadv_nn.so).
+ *
+ * No counterpart in the reference's solver path: the reference has no warm
+ * start.  Its benchmark suite's 3-D U-Net (benchmark/models/unet3d.py:49-125,
+ * PyTorch) is the design template for the network these kernels train and
+ * run.  Tensors are device pointers, channels-last (N, X, Y, Z, C); bf16
+ * tensors are passed as void*; `stream` is a hipStream_t (NULL = default).
+ * Return 0 on success, 1 for an unsupported shape, 2 for a launch error.
+ */
+#ifndef XADV_NN_H
+#define XADV_NN_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* out[v][co] = bias[co] + sum_{tap,ci} W[co][tap*Cin+ci] * in[v+tap][ci]; 3x3x3, pad 1.
+ * Cin, Cout multiples of 16, Cout <= 128, Kp = K padded to 32 (K = 27*Cin). BF16 MFMA, FP32 out. */
+int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, int N, int X, int Y, int Z, int Cin,
+                 int Cout, int Kp, void* stream);
+/* Weights of the data-gradient convolution: Wt[ci][tap'*Cout+co] = W[co][(26-tap')*Cin+ci]. */
+int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void* stream);
+/* dW[co][tap*Cin+ci] = sum_v gout[v][co] * in[v+tap][ci] over voxel chunks of vchunk (multiple of 32);
+ * part holds xnn_wgrad_chunks(...) * Cout * Kp floats (padded K entries must be zero on entry). */
+int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int N, int X, int Y, int Z, int Cin,
+                   int Cout, int Kp, int vchunk, void* stream);
+int xnn_wgrad_chunks(int N, int X, int Y, int Z, int vchunk);
+int xnn_bias_grad(const void* g, float* db, int64_t V, int C, void* stream);
+int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream);
+int xnn_silu_bwd(const float* x, const float* gy, void* gxb, int64_t n, void* stream);
+int xnn_to_bf16(const float* x, void* y, int64_t n, void* stream);
+int xnn_add_f32(const float* a, const float* b, float* y, int64_t n, void* stream);
+int xnn_add_bf16(const void* a, const void* b, void* y, int64_t n, void* stream);
+/* 2x2x2 average pooling (coarse dims given) and its gradient (fine dims given). */
+int xnn_pool_fwd(const void* in, void* out, int N, int CX, int CY, int CZ, int C, void* stream);
+int xnn_pool_bwd(const float* gcoarse, float* gfine, int N, int FX, int FY, int FZ, int C, void* stream);
+/* nearest x2 upsampling (fine dims given) and its gradient (coarse dims given). */
+int xnn_up_fwd(const void* in, void* out, int N, int FX, int FY, int FZ, int C, void* stream);
+int xnn_up_bwd(const float* gfine, float* gcoarse, int N, int CX, int CY, int CZ, int C, void* stream);
+/* weighted MSE over the first nout channels; per-block loss partials (xnn_mse_blocks of them). */
+int xnn_mse(const float* pred, const float* target, const float* wmask, float* grad, float* part, int64_t V, int C,
+            int nout, float inv_norm, void* stream);
+int xnn_mse_blocks(int64_t V, int C);
+/* Adam with bias correction on FP32 master weights; wb (may be NULL) receives the BF16 copy. */
+int xnn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, float lr, float b1, float b2,
+             float eps, int step, float wd, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,166 @@
+"""The surrogate's HIP kernels (libxadv_nn.so) against PyTorch CPU FP32 as the oracle.
+
+Every operand the kernels see in BF16 is rounded to BF16 on the CPU side too;
+the CPU then computes in FP32, so differences are FP32 summation order only
+(conv forward / data grad / weight grad: rtol 1e-3 on the tensor norm).  The
+whole network's forward and parameter gradients are checked against
+torch.autograd on a CPU copy with the same BF16 rounding points, and a few
+Adam steps must lower the loss.
+"""
+import numpy as np
+import pytest
+import torch
+import torch.nn.functional as F
+
+pytestmark = pytest.mark.gpu
+
+
+def bf(x):
+    return x.to(torch.bfloat16).to(torch.float32)
+
+
+def rel(a, b):
+    a, b = a.double(), b.double()
+    return float((a - b).norm() / max(b.norm(), 1e-30))
+
+
+def torch_weight(W, cin, cout):
+    """Our W[co][tap*cin + ci] (tap = kx*9 + ky*3 + kz) -> torch conv3d weight [co, ci, kx, ky, kz]."""
+    return W[:, :27 * cin].reshape(cout, 27, cin).permute(0, 2, 1).reshape(cout, cin, 3, 3, 3)
+
+
+def ncdhw(x):
+    return x.permute(0, 4, 1, 2, 3)
+
+
+def ndhwc(x):
+    return x.permute(0, 2, 3, 4, 1)
+
+
+@pytest.mark.parametrize("cin,cout", [(16, 16), (16, 32), (32, 64), (64, 128), (128, 16), (32, 32)])
+def test_conv_fwd_dgrad_wgrad(gpu, cin, cout):
+    from xadvcool_amd import surrogate as S
+    print(f"[nn] conv {cin}->{cout}", flush=True)
+    gen = torch.Generator().manual_seed(cin * 7 + cout)
+    shape = (2, 8, 6, 10)
+    c = S.Conv(cin, cout, gen)
+    c.b.copy_(torch.randn(cout, generator=gen))
+    x = torch.randn(*shape, cin, generator=gen)
+    xb = x.to(torch.bfloat16).cuda()
+    out = c.fwd(xb, shape)
+    W = c.w.cpu()
+    ref = ndhwc(F.conv3d(ncdhw(bf(x)), torch_weight(bf(W), cin, cout), bias=c.b.cpu(), padding=1))
+    assert rel(out.cpu(), ref) < 1e-3
+    # backward
+    g = torch.randn(*shape, cout, generator=gen)
+    gb = g.to(torch.bfloat16).cuda()
+    gin = c.dgrad(gb, shape)
+    xr = ncdhw(bf(x)).clone().requires_grad_(True)
+    wr = torch_weight(bf(W), cin, cout).clone().requires_grad_(True)
+    br = c.b.cpu().clone().requires_grad_(True)
+    o = F.conv3d(xr, wr, bias=br, padding=1)
+    o.backward(ncdhw(bf(g)))
+    assert rel(gin.cpu(), ndhwc(xr.grad)) < 1e-3
+    c.zero_grad()
+    c.wgrad(xb, gb, shape, vchunk=96)
+    dw = torch_weight(c.dw.cpu(), cin, cout)
+    assert rel(dw, wr.grad) < 1e-3
+    assert rel(c.db.cpu(), br.grad) < 1e-3
+    assert torch.all(c.dw.cpu()[:, 27 * cin:] == 0)
+
+
+def test_pool_up_silu(gpu):
+    from xadvcool_amd import surrogate as S
+    gen = torch.Generator().manual_seed(3)
+    x = torch.randn(2, 8, 4, 6, 16, generator=gen)
+    xb = x.to(torch.bfloat16).cuda()
+    p = S.pool_fwd(xb, (2, 4, 2, 3))
+    pref = ndhwc(F.avg_pool3d(ncdhw(bf(x)), 2))
+    assert rel(p.float().cpu(), pref) < 5e-3
+    u = S.up_fwd(p, (2, 8, 4, 6))
+    uref = p.float().cpu().repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3)
+    assert torch.equal(u.float().cpu(), uref)
+    g = torch.randn(2, 8, 4, 6, 16, generator=gen).cuda()
+    gc = S.up_bwd(g, (2, 4, 2, 3))
+    gref = g.cpu().reshape(2, 4, 2, 2, 2, 3, 2, 16).sum(dim=(2, 4, 6))
+    assert rel(gc.cpu(), gref) < 1e-6
+    gf = S.pool_bwd(gc, (2, 8, 4, 6))
+    assert rel(gf.cpu(), gc.cpu().repeat_interleave(2, 1).repeat_interleave(2, 2).repeat_interleave(2, 3) / 8) < 1e-6
+    xf = x.cuda()
+    y = S.silu_fwd(xf)
+    assert rel(y.float().cpu(), bf(F.silu(x))) < 5e-3
+    gy = torch.randn_like(x).cuda()
+    gx = S.silu_bwd(xf, gy)
+    xr = x.clone().requires_grad_(True)
+    F.silu(xr).backward(gy.cpu())
+    assert rel(gx.float().cpu(), xr.grad) < 5e-3
+
+
+def _torch_net(net, x):
+    """CPU FP32 copy of UNet3D.forward with the same BF16 rounding points (autograd-capable)."""
+    Ws = {k: torch_weight(bf(c.w.cpu()), c.cin, c.cout).clone().requires_grad_(True) for k, c in net.conv.items()}
+    Bs = {k: c.b.cpu().clone().requires_grad_(True) for k, c in net.conv.items()}
+
+    def cv(k, h):
+        return F.conv3d(h, Ws[k], bias=Bs[k], padding=1)
+
+    def act(h):  # SiLU output stored in BF16
+        return bf(F.silu(h))
+
+    h = ncdhw(bf(x))
+    s1 = act(cv("c1b", act(cv("c1a", h))))
+    q1 = bf(F.avg_pool3d(s1, 2))
+    s2 = act(cv("c2b", act(cv("c2a", q1))))
+    q2 = bf(F.avg_pool3d(s2, 2))
+    s3 = act(cv("c3b", act(cv("c3a", q2))))
+    q3 = bf(F.avg_pool3d(s3, 2))
+    a4 = act(cv("c4b", act(cv("c4a", q3))))
+    up = lambda t: t.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
+    d3 = act(cv("d3b", bf(act(cv("d3a", up(a4))) + s3)))
+    d2 = act(cv("d2b", bf(act(cv("d2a", up(d3))) + s2)))
+    d1 = act(cv("d1b", bf(act(cv("d1a", up(d2))) + s1)))
+    return ndhwc(cv("head", d1)), Ws, Bs
+
+
+def test_network_forward_and_gradients(gpu):
+    from xadvcool_amd import surrogate as S
+    torch.manual_seed(0)
+    net = S.UNet3D(seed=1)
+    x = torch.randn(2, 16, 8, 8, 16) * 0.5
+    tgt = torch.randn(2, 16, 8, 8, 5)
+    w = (torch.rand(2, 16, 8, 8, 5) > 0.3).float()
+    out = net.forward(x.to(torch.bfloat16).cuda(), keep=True)
+    ref, Ws, Bs = _torch_net(net, x)
+    assert rel(out.cpu(), ref) < 2e-2
+    loss, g = S.mse_loss(out, tgt.cuda(), w.cuda())
+    net.zero_grad()
+    net.backward(g)
+    lref = ((ref[..., :5] - tgt) ** 2 * w).sum() / w.sum()
+    lref.backward()
+    assert abs(loss.item() - lref.item()) / lref.item() < 2e-2
+    for k, c in net.conv.items():
+        gw = torch_weight(c.dw.cpu(), c.cin, c.cout)
+        e = rel(gw, Ws[k].grad)
+        cos = float(torch.nn.functional.cosine_similarity(gw.flatten(), Ws[k].grad.flatten(), dim=0))
+        print(f"[nn] grad {k}: rel {e:.3e} cos {cos:.5f}", flush=True)
+        assert e < 6e-2 and cos > 0.998, k
+        assert rel(c.db.cpu(), Bs[k].grad) < 6e-2, k
+
+
+def test_adam_lowers_the_loss(gpu):
+    from xadvcool_amd import surrogate as S
+    torch.manual_seed(1)
+    net = S.UNet3D(seed=2)
+    x = (torch.randn(2, 16, 8, 8, 16) * 0.5).to(torch.bfloat16).cuda()
+    tgt = torch.randn(2, 16, 8, 8, 5).cuda() * 0.3
+    w = torch.ones(2, 16, 8, 8, 5).cuda()
+    losses = []
+    for _ in range(30):
+        out = net.forward(x, keep=True)
+        loss, g = S.mse_loss(out, tgt, w)
+        net.zero_grad()
+        net.backward(g)
+        net.adam_step(2e-3)
+        losses.append(loss.item())
+    print(f"[nn] adam losses {losses[0]:.4f} -> {losses[-1]:.4f}", flush=True)
+    assert losses[-1] < 0.5 * losses[0]

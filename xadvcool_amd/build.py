@@ -12,6 +12,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRC = os.path.join(HERE, "csrc", "xadv.hip")
 LIB = os.path.join(HERE, "libxadv.so")
+NN_SRC = os.path.join(HERE, "csrc", "nn_kernels.hip")
+NN_LIB = os.path.join(HERE, "libxadv_nn.so")
 ARCH = os.environ.get("XADV_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: every + and * rounds separately, in source order, so the
@@ -25,18 +27,24 @@ HIPCC_FLAGS = [
 ]
 
 
-def build(verbose: bool = False) -> str:
+def _hipcc(src, lib, verbose):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
-    tmp = LIB + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, SRC]
+    tmp = lib + ".tmp"
+    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, src]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         sys.stderr.write(r.stdout + r.stderr)
-        raise RuntimeError("hipcc failed building libxadv.so")
-    os.replace(tmp, LIB)
-    return LIB
+        raise RuntimeError(f"hipcc failed building {os.path.basename(lib)}")
+    os.replace(tmp, lib)
+    return lib
+
+
+def build(verbose: bool = False) -> str:
+    """Build the solver library and the surrogate library; returns the solver library's path."""
+    _hipcc(NN_SRC, NN_LIB, verbose)
+    return _hipcc(SRC, LIB, verbose)
 
 
 if __name__ == "__main__":

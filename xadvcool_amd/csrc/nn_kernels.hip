@@ -1,0 +1,474 @@
+// MI355X (gfx950) kernels of the warm-start surrogate: a 3-D convolutional
+// encoder-decoder trained and run with BF16 MFMA and FP32 accumulation.
+//
+// Tensors are channels-last (N, X, Y, Z, C) with C a multiple of 16 and the
+// spatial dims of each level given explicitly.  Convolutions are 3x3x3, stride
+// 1, zero padding 1, as implicit GEMMs on v_mfma_f32_16x16x32_bf16:
+//   forward   out[v][co] = b[co] + sum_k W[co][k] * in[v + tap(k)][ci(k)],
+//             k = tap * Cin + ci, K padded to a multiple of 32 with zero weights;
+//   data grad = forward with W'[ci][tap' * Cout + co] = W[co][(26 - tap') * Cin + ci];
+//   weight grad dW[co][k] = sum_v g[v][co] * in[v + tap(k)][ci(k)], split over
+//             voxel chunks (partials reduced in a fixed order), db[co] = sum_v g.
+// MFMA operand layout (checked on the device): lane l holds A[l%16][8*(l/16)+j],
+// B[8*(l/16)+j][l%16] (j = 0..7) and D[4*(l/16)+j][l%16] (j = 0..3).
+// Activations use SiLU, x * sigmoid(x) with HIP's expf.  Adam follows Kingma &
+// Ba (2015) with bias correction; master weights and moments are FP32.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+
+#include "../../include/xadv_nn.h"
+
+namespace {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ float bf2f(__bf16 h) { return (float)h; }
+__device__ __forceinline__ __bf16 f2bf(float f) { return (__bf16)f; }
+
+struct Geo {
+  int X, Y, Z;  // spatial dims of this level
+  int N;        // batch
+  __host__ __device__ __forceinline__ int64_t V() const { return (int64_t)N * X * Y * Z; }
+};
+
+// Neighbour voxel of (n,x,y,z) for tap t (t = (dx+1)*9 + (dy+1)*3 + (dz+1)),
+// or -1 outside the lattice (zero padding).
+__device__ __forceinline__ int64_t tap_voxel(const Geo& g, int n, int x, int y, int z, int t) {
+  const int dx = t / 9 - 1, dy = (t / 3) % 3 - 1, dz = t % 3 - 1;
+  const int xx = x + dx, yy = y + dy, zz = z + dz;
+  if (xx < 0 || xx >= g.X || yy < 0 || yy >= g.Y || zz < 0 || zz >= g.Z) return -1;
+  return (((int64_t)n * g.X + xx) * g.Y + yy) * g.Z + zz;
+}
+
+__device__ __forceinline__ void decompose(const Geo& g, int64_t v, int& n, int& x, int& y, int& z) {
+  z = (int)(v % g.Z);
+  int64_t r = v / g.Z;
+  y = (int)(r % g.Y);
+  r /= g.Y;
+  x = (int)(r % g.X);
+  n = (int)(r / g.X);
+}
+
+// --------------------------------------------------------------- forward ---
+// One block = 4 waves; each wave computes 16 voxels x NT*16 output channels.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_fwd(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
+                                                  const float* __restrict__ bias, float* __restrict__ out, Geo g,
+                                                  int Cin, int Kp) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t V = g.V();
+  const int64_t v0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
+  if (v0 >= V) return;
+  const int Cout = NT * 16;
+  // A-operand voxel of this lane
+  const int64_t va = v0 + (lane & 15);
+  int n = 0, x = 0, y = 0, z = 0;
+  const bool va_ok = va < V;
+  if (va_ok) decompose(g, va, n, x, y, z);
+  const int kg = (lane >> 4) * 8;
+  floatx4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0, 0, 0, 0};
+  const int K = 27 * Cin;
+  for (int kb = 0; kb < Kp; kb += 32) {
+    const int k0 = kb + kg;
+    bf16x8 a;
+    bool have = false;
+    if (va_ok && k0 < K) {
+      const int tap = k0 / Cin, ci = k0 - tap * Cin;
+      const int64_t nb = tap_voxel(g, n, x, y, z, tap);
+      if (nb >= 0) {
+        const u16x8 raw = *reinterpret_cast<const u16x8*>(in + nb * Cin + ci);
+        a = __builtin_bit_cast(bf16x8, raw);
+        have = true;
+      }
+    }
+    if (!have) a = __builtin_bit_cast(bf16x8, u16x8{0, 0, 0, 0, 0, 0, 0, 0});
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int co = t * 16 + (lane & 15);
+      const u16x8 rb = *reinterpret_cast<const u16x8*>(W + (int64_t)co * Kp + k0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, rb), acc[t], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + (lane & 15);
+    const float b = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t v = v0 + (lane >> 4) * 4 + j;
+      if (v < V) out[v * Cout + co] = acc[t][j] + b;
+    }
+  }
+}
+
+// W'[ci][tap' * Cout + co] = W[co][(26 - tap') * Cin + ci], K' padded with zeros.
+__global__ void k_wflip(const __bf16* __restrict__ W, __bf16* __restrict__ Wt, int Cin, int Cout, int Kp,
+                        int Kpt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)Cin * Kpt) return;
+  const int ci = (int)(i / Kpt), k = (int)(i - (int64_t)ci * Kpt);
+  __bf16 v = f2bf(0.0f);
+  if (k < 27 * Cout) {
+    const int tp = k / Cout, co = k - tp * Cout;
+    v = W[(int64_t)co * Kp + (26 - tp) * Cin + ci];
+  }
+  Wt[i] = v;
+}
+
+// ----------------------------------------------------------- weight grad ---
+// Block: one tap, one 16-wide Cout tile, all Cin tiles, a chunk of voxels.
+// Per 32-voxel step the block stages g (32 x 16) and the shifted input
+// (32 x Cin) in LDS transposed (channel-major), then each wave runs MFMAs for
+// its share of the Cin tiles.  Partials: part[chunk][co][tap*Cin + ci].
+template <int CIT>
+__global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ in, const __bf16* __restrict__ gout,
+                                                    float* __restrict__ part, Geo g, int Cout, int vchunk,
+                                                    int Kp) {
+  constexpr int Cin = CIT * 16;
+  __shared__ __attribute__((aligned(16))) __bf16 sg[16][40];       // [co][v], padded rows
+  __shared__ __attribute__((aligned(16))) __bf16 si[Cin][40];      // [ci][v]
+  const int tap = blockIdx.x;
+  const int cot = blockIdx.y;
+  const int chunk = blockIdx.z;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t V = g.V();
+  const int64_t vb = (int64_t)chunk * vchunk;
+  const int64_t ve = vb + vchunk < V ? vb + vchunk : V;
+  floatx4 acc[(CIT + 3) / 4];
+#pragma unroll
+  for (int t = 0; t < (CIT + 3) / 4; ++t) acc[t] = floatx4{0, 0, 0, 0};
+  for (int64_t v0 = vb; v0 < ve; v0 += 32) {
+    // stage: 32 voxels x (16 g channels + Cin input channels)
+    for (int e = threadIdx.x; e < 32 * (16 + Cin); e += 256) {
+      const int vi = e / (16 + Cin), c = e - vi * (16 + Cin);
+      const int64_t v = v0 + vi;
+      __bf16 val = f2bf(0.0f);
+      if (v < ve) {
+        if (c < 16) {
+          val = gout[v * Cout + cot * 16 + c];
+        } else {
+          int n, x, y, z;
+          decompose(g, v, n, x, y, z);
+          const int64_t nb = tap_voxel(g, n, x, y, z, tap);
+          if (nb >= 0) val = in[nb * Cin + (c - 16)];
+        }
+      }
+      if (c < 16)
+        sg[c][vi] = val;
+      else
+        si[c - 16][vi] = val;
+    }
+    __syncthreads();
+    const int kg = (lane >> 4) * 8;
+    const u16x8 ra = *reinterpret_cast<const u16x8*>(&sg[lane & 15][kg]);
+    const bf16x8 a = __builtin_bit_cast(bf16x8, ra);
+#pragma unroll
+    for (int t = 0; t < (CIT + 3) / 4; ++t) {
+      const int cit = wave + 4 * t;
+      if (cit < CIT) {
+        const u16x8 rb = *reinterpret_cast<const u16x8*>(&si[cit * 16 + (lane & 15)][kg]);
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, rb), acc[t], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+  }
+  // D[m = co][n = ci]: lane holds co = 4*(l/16)+j, ci = l%16
+#pragma unroll
+  for (int t = 0; t < (CIT + 3) / 4; ++t) {
+    const int cit = wave + 4 * t;
+    if (cit >= CIT) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int co = cot * 16 + (lane >> 4) * 4 + j;
+      const int ci = cit * 16 + (lane & 15);
+      part[((int64_t)chunk * Cout + co) * Kp + tap * Cin + ci] = acc[t][j];
+    }
+  }
+}
+
+__global__ void k_wgrad_reduce(const float* __restrict__ part, float* __restrict__ dW, int nchunk, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float s = 0.0f;
+  for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + i];
+  dW[i] = s;
+}
+
+// db[co] = sum_v g[v][co]: one block per channel, fixed-order tree.
+__global__ void k_bias_grad(const __bf16* __restrict__ g, float* __restrict__ db, int64_t V, int C) {
+  __shared__ float red[256];
+  const int co = blockIdx.x;
+  float s = 0.0f;
+  for (int64_t v = threadIdx.x; v < V; v += 256) s += bf2f(g[v * C + co]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) db[co] = red[0];
+}
+
+// ------------------------------------------------------------ elementwise ---
+
+__device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
+__device__ __forceinline__ float dsilu(float x) {
+  const float s = 1.0f / (1.0f + expf(-x));
+  return s * (1.0f + x * (1.0f - s));
+}
+
+__global__ void k_silu_fwd(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(silu(x[i]));
+}
+// gx = gy * silu'(x); written as bf16 (operand of the next MFMA) and optionally fp32.
+__global__ void k_silu_bwd(const float* __restrict__ x, const float* __restrict__ gy, __bf16* __restrict__ gxb,
+                           int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gxb[i] = f2bf(gy[i] * dsilu(x[i]));
+}
+__global__ void k_to_bf16(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(x[i]);
+}
+__global__ void k_add_f32(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ y,
+                          int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a[i] + b[i];
+}
+// y_bf16 = bf16(a_bf16 + b_bf16) computed in fp32 (additive skip of the decoder)
+__global__ void k_add_bf16(const __bf16* __restrict__ a, const __bf16* __restrict__ b, __bf16* __restrict__ y,
+                           int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = f2bf(bf2f(a[i]) + bf2f(b[i]));
+}
+
+// 2x2x2 average pooling, fine (N,X,Y,Z,C) -> coarse (N,X/2,Y/2,Z/2,C).
+__global__ void k_pool_fwd(const __bf16* __restrict__ in, __bf16* __restrict__ out, Geo gc, int C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t nco = gc.V() * C;
+  if (i >= nco) return;
+  const int c = (int)(i % C);
+  int n, x, y, z;
+  decompose(gc, i / C, n, x, y, z);
+  const int FX = gc.X * 2, FY = gc.Y * 2, FZ = gc.Z * 2;
+  float s = 0.0f;
+  for (int d = 0; d < 8; ++d) {
+    const int xx = 2 * x + (d >> 2), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d & 1);
+    s += bf2f(in[((((int64_t)n * FX + xx) * FY + yy) * FZ + zz) * C + c]);
+  }
+  out[i] = f2bf(s * 0.125f);
+}
+// grad of pooling: g_fine = g_coarse / 8 (fp32 in, fp32 out)
+__global__ void k_pool_bwd(const float* __restrict__ gc_, float* __restrict__ gf, Geo gf_geo, int C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= gf_geo.V() * C) return;
+  const int c = (int)(i % C);
+  int n, x, y, z;
+  decompose(gf_geo, i / C, n, x, y, z);
+  const int CX = gf_geo.X / 2, CY = gf_geo.Y / 2, CZ = gf_geo.Z / 2;
+  gf[i] = 0.125f * gc_[((((int64_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2) * C + c];
+}
+// nearest x2 upsampling coarse -> fine (bf16)
+__global__ void k_up_fwd(const __bf16* __restrict__ in, __bf16* __restrict__ out, Geo gf_geo, int C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= gf_geo.V() * C) return;
+  const int c = (int)(i % C);
+  int n, x, y, z;
+  decompose(gf_geo, i / C, n, x, y, z);
+  const int CX = gf_geo.X / 2, CY = gf_geo.Y / 2, CZ = gf_geo.Z / 2;
+  out[i] = in[((((int64_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2) * C + c];
+}
+// grad of upsampling: g_coarse = sum of the 8 fine children (fp32)
+__global__ void k_up_bwd(const float* __restrict__ gf, float* __restrict__ gc_, Geo gc, int C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= gc.V() * C) return;
+  const int c = (int)(i % C);
+  int n, x, y, z;
+  decompose(gc, i / C, n, x, y, z);
+  const int FX = gc.X * 2, FY = gc.Y * 2, FZ = gc.Z * 2;
+  float s = 0.0f;
+  for (int d = 0; d < 8; ++d) {
+    const int xx = 2 * x + (d >> 2), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d & 1);
+    s += gf[((((int64_t)n * FX + xx) * FY + yy) * FZ + zz) * C + c];
+  }
+  gc_[i] = s;
+}
+
+// Weighted MSE over the first `nout` of C channels:
+// loss partials (one per block) and grad = 2 w (pred - target) / norm.
+__global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ target,
+                      const float* __restrict__ wmask, float* __restrict__ grad, float* __restrict__ part, int64_t V,
+                      int C, int nout, float inv_norm) {
+  __shared__ float red[256];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < V * C) {
+    const int64_t v = i / C;
+    const int c = (int)(i - v * C);
+    float gr = 0.0f;
+    if (c < nout) {
+      const float w = wmask[v * nout + c];
+      const float d = pred[i] - target[v * nout + c];
+      l = w * d * d * inv_norm;
+      gr = 2.0f * w * d * inv_norm;
+    }
+    grad[i] = gr;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy.
+__global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
+                       float* __restrict__ v, __bf16* __restrict__ wb, int64_t n, float lr, float b1, float b2,
+                       float eps, float c1, float c2, float wd) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float gi = g[i] + wd * w[i];
+  const float mi = b1 * m[i] + (1.0f - b1) * gi;
+  const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  const float wi = w[i] - lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+  w[i] = wi;
+  if (wb) wb[i] = f2bf(wi);
+}
+
+inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+}  // namespace
+
+// ================================================================= C ABI ===
+
+extern "C" {
+
+int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, int N, int X, int Y, int Z, int Cin,
+                 int Cout, int Kp, void* stream) {
+  if (Cin % 16 || Cout % 16 || Cout > 128 || Kp % 32 || Kp < 27 * Cin) return 1;
+  Geo g{X, Y, Z, N};
+  const int64_t V = g.V();
+  const unsigned grid = (unsigned)((V + 63) / 64);
+  hipStream_t s = (hipStream_t)stream;
+  const __bf16* i = (const __bf16*)in;
+  const __bf16* w = (const __bf16*)W;
+  switch (Cout / 16) {
+    case 1: hipLaunchKernelGGL(k_conv_fwd<1>, dim3(grid), dim3(256), 0, s, i, w, bias, out, g, Cin, Kp); break;
+    case 2: hipLaunchKernelGGL(k_conv_fwd<2>, dim3(grid), dim3(256), 0, s, i, w, bias, out, g, Cin, Kp); break;
+    case 4: hipLaunchKernelGGL(k_conv_fwd<4>, dim3(grid), dim3(256), 0, s, i, w, bias, out, g, Cin, Kp); break;
+    case 8: hipLaunchKernelGGL(k_conv_fwd<8>, dim3(grid), dim3(256), 0, s, i, w, bias, out, g, Cin, Kp); break;
+    default: return 1;
+  }
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void* stream) {
+  if (Kpt % 32 || Kpt < 27 * Cout) return 1;
+  hipLaunchKernelGGL(k_wflip, dim3(nblk((int64_t)Cin * Kpt)), dim3(256), 0, (hipStream_t)stream,
+                     (const __bf16*)W, (__bf16*)Wt, Cin, Cout, Kp, Kpt);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int N, int X, int Y, int Z, int Cin,
+                   int Cout, int Kp, int vchunk, void* stream) {
+  if (Cin % 16 || Cout % 16 || Cin > 128 || vchunk % 32 || vchunk <= 0) return 1;
+  Geo g{X, Y, Z, N};
+  const int64_t V = g.V();
+  const int nchunk = (int)((V + vchunk - 1) / vchunk);
+  hipStream_t s = (hipStream_t)stream;
+  dim3 grid(27, Cout / 16, nchunk);
+  const __bf16* i = (const __bf16*)in;
+  const __bf16* go = (const __bf16*)gout;
+  // padded K entries of the partials stay zero: clear once via the reduce input
+  switch (Cin / 16) {
+    case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
+    case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
+    case 4: hipLaunchKernelGGL(k_conv_wgrad<4>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
+    case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
+    default: return 1;
+  }
+  const int64_t n = (int64_t)Cout * Kp;
+  hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblk(n)), dim3(256), 0, s, part, dW, nchunk, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int xnn_wgrad_chunks(int N, int X, int Y, int Z, int vchunk) {
+  const int64_t V = (int64_t)N * X * Y * Z;
+  return (int)((V + vchunk - 1) / vchunk);
+}
+
+int xnn_bias_grad(const void* g, float* db, int64_t V, int C, void* stream) {
+  hipLaunchKernelGGL(k_bias_grad, dim3(C), dim3(256), 0, (hipStream_t)stream, (const __bf16*)g, db, V, C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream) {
+  hipLaunchKernelGGL(k_silu_fwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_silu_bwd(const float* x, const float* gy, void* gxb, int64_t n, void* stream) {
+  hipLaunchKernelGGL(k_silu_bwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, gy, (__bf16*)gxb, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_to_bf16(const float* x, void* y, int64_t n, void* stream) {
+  hipLaunchKernelGGL(k_to_bf16, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_add_f32(const float* a, const float* b, float* y, int64_t n, void* stream) {
+  hipLaunchKernelGGL(k_add_f32, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, a, b, y, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_add_bf16(const void* a, const void* b, void* y, int64_t n, void* stream) {
+  hipLaunchKernelGGL(k_add_bf16, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)a,
+                     (const __bf16*)b, (__bf16*)y, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_pool_fwd(const void* in, void* out, int N, int CX, int CY, int CZ, int C, void* stream) {
+  Geo gc{CX, CY, CZ, N};
+  hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)in,
+                     (__bf16*)out, gc, C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_pool_bwd(const float* gcoarse, float* gfine, int N, int FX, int FY, int FZ, int C, void* stream) {
+  Geo gf{FX, FY, FZ, N};
+  hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, gcoarse, gfine, gf, C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_up_fwd(const void* in, void* out, int N, int FX, int FY, int FZ, int C, void* stream) {
+  Geo gf{FX, FY, FZ, N};
+  hipLaunchKernelGGL(k_up_fwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)in,
+                     (__bf16*)out, gf, C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_up_bwd(const float* gfine, float* gcoarse, int N, int CX, int CY, int CZ, int C, void* stream) {
+  Geo gc{CX, CY, CZ, N};
+  hipLaunchKernelGGL(k_up_bwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, gfine, gcoarse, gc, C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_mse(const float* pred, const float* target, const float* wmask, float* grad, float* part, int64_t V, int C,
+            int nout, float inv_norm, void* stream) {
+  hipLaunchKernelGGL(k_mse, dim3(nblk(V * C)), dim3(256), 0, (hipStream_t)stream, pred, target, wmask, grad, part,
+                     V, C, nout, inv_norm);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_mse_blocks(int64_t V, int C) { return (int)nblk(V * C); }
+int xnn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, float lr, float b1, float b2,
+             float eps, int step, float wd, void* stream) {
+  const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
+  hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, (__bf16*)wb, n, lr, b1,
+                     b2, eps, c1, c2, wd);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+}  // extern "C"
