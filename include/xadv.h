@@ -161,6 +161,30 @@ enum {
 };
 int xadv_metrics(xadv_batch* b, int slot, int core_x0, int core_nx, int core_ny, double* out);
 
+/* Reductions behind the physics acceptance checks of analyze_dataset.py:184-325,
+ * over one slot's fields of the last readout; out[XADV_NCHECK] doubles, see
+ * XADV_C_*.  k_fluid / k_solid are the thermal conductivities (tau_th - 0.5)/3.
+ * Replaces the numpy checks of validate_sample (analyze_dataset.py:184-325). */
+#define XADV_NCHECK 16
+enum {
+  XADV_C_MDOT_IN = 0,     /* sum rho*u_x over non-solid cells on x=1 */
+  XADV_C_MDOT_OUT,        /* ... on x=nx-2 */
+  XADV_C_VMAX_FLUID,      /* max |u| over non-solid cells */
+  XADV_C_RHODEV_FLUID,    /* max |rho - 1| over non-solid cells */
+  XADV_C_TMIN_FLUID,      /* min T over non-solid cells */
+  XADV_C_RHO_P1_SUM,      /* sum rho over the plane x=1 */
+  XADV_C_RHO_P2_SUM,      /* sum rho over the plane x=nx-2 */
+  XADV_C_DIV_SUM,         /* sum |div u| (central differences) over interior non-solid cells */
+  XADV_C_DIV_N,
+  XADV_C_DIV_MAX,
+  XADV_C_UX_INLET,        /* sum u_x over INLET cells */
+  XADV_C_UXT_OUTLET,      /* sum u_x*T over OUTLET cells */
+  XADV_C_QDIFF_IN,        /* sum -k (T[1] - T[0]) over the plane x=1 */
+  XADV_C_QDIFF_OUT,       /* sum -k (T[nx-1] - T[nx-2]) over the plane x=nx-2 */
+  XADV_C_NFLUID
+};
+int xadv_checks(xadv_batch* b, int slot, double k_fluid, double k_solid, double* out);
+
 /* The batch's HIP stream (hipStream_t), for callers that time or order work
  * against it (e.g. hipEventRecord around xadv_step).  No reference
  * counterpart: Warp's default stream was implicit (src/simulation.py:93-121). */

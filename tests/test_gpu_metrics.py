@@ -37,3 +37,29 @@ def test_device_metrics_match_numpy(gpu, family):
     for k, v in want.items():
         assert got[k] == pytest.approx(v, rel=2e-5, abs=1e-9), k
     assert got["nonfinite_cells"] == 0
+
+
+def test_device_physics_checks_match_numpy(gpu):
+    """analyze_dataset.py:184-325 checks: device reductions vs the numpy restatement on the same fields."""
+    from oracle import checks_ref
+    from xadvcool_amd import geometry as G
+    from xadvcool_amd.checks import device_checks
+    from xadvcool_amd.engine import CoupledSolver, normalized_heat
+    core = (40, 24, 12)
+    d = G.family_design(11, core=core, family="pinfin", heat_name="uniform")
+    mask = d["mask"]
+    s = CoupledSolver(mask, d["heat"], tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
+                      u_inlet_val=d["u_inlet_val"])
+    try:
+        s.init_equilibrium()
+        s.step(2000)
+        s._readout(False)
+        got = device_checks(s.batch, 0, mask, normalized_heat(d["heat"], mask.shape), tau_fluid=d["tau_fluid"],
+                            heat_power=d["heat_power"])
+        rho, u, T = (t.cpu().numpy() for t in s.fields_torch())
+    finally:
+        s.close()
+    want = checks_ref.validate_fields(mask, u, rho, T, normalized_heat(d["heat"], mask.shape),
+                                      {"tau_fluid": d["tau_fluid"], "heat_power": d["heat_power"]})
+    for k, v in want.items():
+        assert got[k] == pytest.approx(v, rel=2e-4, abs=1e-7), k

@@ -1,0 +1,69 @@
+"""Warm start on the GPU: the cold LBM is the fallback and the judge.
+
+* a warm start that blows up (Ma far above the stability limit) is abandoned at
+  its first residual check and the design is re-run cold: the result equals
+  the cold run of the same design bit for bit (same steps, same metrics);
+* the trained surrogate, on designs it never saw, reaches the same stop rule in
+  fewer steps than the cold start, with converged pressure drop and thermal
+  resistance within 0.5 % of the cold result.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+
+def _designs(n, seed_off):
+    from xadvcool_amd import warmstart as W
+    return W.make_designs(W.COMPARE_SEED0 + seed_off, n)
+
+
+def test_blown_up_warm_start_falls_back_to_cold(gpu):
+    from xadvcool_amd import warmstart as W
+
+    designs = _designs(2, 7000)
+
+    def bad(d):
+        m = torch.from_numpy(d["mask"] != 1).cuda()
+        nx, ny, nz = d["mask"].shape
+        u = torch.zeros(nx, ny, nz, 3, device="cuda")
+        u[..., 0] = torch.where(m, 0.4, 0.0)
+        rho = torch.ones(nx, ny, nz, device="cuda")
+        T = torch.full((nx, ny, nz), 25.0, device="cuda")
+        return rho, u, T
+
+    kw = dict(slots=2, tol=1e-5, max_steps=6000)
+    cold = W.solve_designs(designs, None, **kw)
+    warm = W.solve_designs(designs, None, predictor=bad, **kw)
+    assert len(warm["fallbacks"]) == len(designs)
+    for k, status, steps in warm["fallbacks"]:
+        assert status == "diverged" and steps == 100
+    for d in designs:
+        c, w = cold["results"][d["key"]], warm["results"][d["key"]]
+        assert w["abandoned_steps"] == 100
+        assert (w["steps"], w["status"]) == (c["steps"], c["status"])
+        for m in ("pressure_drop", "R_th", "Nu", "Q_vol"):
+            assert w["metrics"][m] == c["metrics"][m], m
+
+
+def test_trained_surrogate_cuts_steps_on_unseen_designs(gpu):
+    from xadvcool_amd import surrogate as S
+    from xadvcool_amd import warmstart as W
+
+    assert os.path.exists(W.DEFAULT_WEIGHTS), "trained weights missing from the package"
+    net = S.UNet3D.load_file(W.DEFAULT_WEIGHTS)
+    n = 8
+    designs = _designs(n, 5000)
+    summ, rows = W.compare(net, designs, slots=n, tol=1e-6, max_steps=30000)
+    print(summ)
+    conv = [r for r in rows if r["cold_status"] == "converged"]
+    assert len(conv) >= n // 2
+    for r in conv:
+        assert r["warm_status"] == "converged", r
+        assert r["d_pressure_drop"] <= 5e-3 and r["d_R_th"] <= 5e-3, r
+    cold_steps = np.mean([r["cold_steps"] for r in conv])
+    warm_steps = np.mean([r["warm_steps"] for r in conv])
+    assert warm_steps < 0.8 * cold_steps, (warm_steps, cold_steps)

@@ -13,12 +13,13 @@ LIB_PATH = os.path.join(HERE, "libxadv.so")
 
 XADV_OK, XADV_E_ARG, XADV_E_HIP, XADV_E_STATE = 0, 1, 2, 3
 NMETRIC = 24
+NCHECK = 16
 
 # Exported symbols, in the order of include/xadv.h.
 EXPORTS = (
     "xadv_batch_create", "xadv_batch_destroy", "xadv_instance_load", "xadv_instance_init", "xadv_set_active",
     "xadv_set_inlet_velocity", "xadv_step", "xadv_readout", "xadv_stats", "xadv_fields", "xadv_metrics",
-    "xadv_get_stream", "xadv_last_error", "xadv_version",
+    "xadv_checks", "xadv_get_stream", "xadv_last_error", "xadv_version",
 )
 
 
@@ -72,6 +73,7 @@ def load():
     lib.xadv_fields.argtypes = [vp, ip, vp, vp, vp]
     lib.xadv_metrics.argtypes = [vp, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_double)]
     lib.xadv_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
+    lib.xadv_checks.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_double)]
     lib.xadv_last_error.restype = ctypes.c_char_p
     lib.xadv_version.restype = ctypes.c_char_p
     for name in EXPORTS:

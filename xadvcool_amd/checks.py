@@ -1,0 +1,84 @@
+"""Physics acceptance checks of a converged solve, reduced on the GPU.
+
+The nine checks of the reference's validate_sample (analyze_dataset.py:184-325):
+mass balance, Mach number, density deviation, fluid temperature lower bound,
+pressure-drop sign, Nusselt range, f*Re range, velocity divergence and energy
+balance, with its thresholds (analyze_dataset.py:111-120).  The field
+reductions run in one device pass (xadv_checks); the closing arithmetic and
+pass/warn flags follow the reference's formulas here.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+THRESHOLDS = {
+    "mass_conservation": {"pass": 0.01, "warn": 0.05},
+    "mach_number": {"pass": 0.15, "warn": 0.30},
+    "density_deviation": {"pass": 0.05, "warn": 0.10},
+    "temp_lower_bound": {"pass": 0.0, "warn": -0.5},
+    "nusselt_low": 1.0,
+    "nusselt_high": 500.0,
+    "f_re_high": 200.0,
+    "divergence_free": {"pass": 0.001, "warn": 0.01},
+}
+
+
+def close_checks(r, *, plane_n, heat_sum, tau_fluid, heat_power, Nu=0.0, f_friction=0.0, Re=1.0, Pr=7.0,
+                 k_ratio=628.0, t_inlet=25.0):
+    """Checks from the XADV_C_* reduction vector r (analyze_dataset.py:198-323)."""
+    th = THRESHOLDS
+    c = {}
+    mdot_in, mdot_out = r[0], r[1]
+    c["mass_err"] = abs(mdot_in - mdot_out) / (abs(mdot_in) + 1e-12)
+    c["mass_pass"] = c["mass_err"] < th["mass_conservation"]["pass"]
+    c["mass_warn"] = c["mass_err"] < th["mass_conservation"]["warn"]
+    has_fluid = r[14] > 0
+    max_v = r[2] if has_fluid else 0.0
+    c["mach"] = max_v * math.sqrt(3.0)
+    c["mach_pass"] = c["mach"] < th["mach_number"]["pass"]
+    c["mach_warn"] = c["mach"] < th["mach_number"]["warn"]
+    c["rho_dev"] = r[3] if has_fluid else 0.0
+    c["rho_dev_pass"] = c["rho_dev"] < max(th["density_deviation"]["pass"], 2.0 * c["mach"] ** 2)
+    c["rho_dev_warn"] = c["rho_dev"] < max(th["density_deviation"]["warn"], 3.0 * c["mach"] ** 2)
+    c["T_lower_bound"] = (r[4] if has_fluid else t_inlet) - t_inlet
+    c["T_lb_pass"] = c["T_lower_bound"] >= th["temp_lower_bound"]["pass"]
+    c["T_lb_warn"] = c["T_lower_bound"] >= th["temp_lower_bound"]["warn"]
+    c["delta_P"] = (r[5] / plane_n - r[6] / plane_n) / 3.0
+    c["delta_P_pass"] = c["delta_P"] > 0
+    c["Nu"] = Nu
+    c["Nu_pass"] = th["nusselt_low"] <= Nu <= th["nusselt_high"]
+    c["f_Re"] = f_friction * Re
+    c["f_Re_pass"] = 0 < c["f_Re"] < th["f_re_high"]
+    c["mean_div"] = r[7] / r[8] if r[8] > 0 else 0.0
+    c["max_div"] = r[9] if r[8] > 0 else 0.0
+    c["div_pass"] = c["mean_div"] < th["divergence_free"]["pass"]
+    c["div_warn"] = c["mean_div"] < th["divergence_free"]["warn"]
+    tau_th_fluid = 0.5 + (tau_fluid - 0.5) / Pr
+    tau_th_solid = 0.5 + k_ratio * (tau_th_fluid - 0.5)
+    Q_source = (1.0 - 0.5 / tau_th_solid) * heat_power * heat_sum
+    Q_removed = (r[11] + r[13]) - (t_inlet * r[10] + r[12])
+    c["Q_source_rate"] = Q_source
+    c["Q_conv"] = Q_removed
+    c["energy_err"] = abs(Q_source - Q_removed) / (abs(Q_source) + 1e-12) if abs(Q_source) > 1e-12 else 0.0
+    c["energy_pass"] = c["energy_err"] < 0.20
+    c["energy_warn"] = c["energy_err"] < 0.40
+    c["all_pass"] = all(v for k, v in c.items() if k.endswith("_pass"))
+    return c
+
+
+def conductivities(tau_fluid, Pr=7.0, k_ratio=628.0):
+    tau_th_fluid = 0.5 + (tau_fluid - 0.5) / Pr
+    tau_th_solid = 0.5 + k_ratio * (tau_th_fluid - 0.5)
+    return (tau_th_fluid - 0.5) / 3.0, (tau_th_solid - 0.5) / 3.0
+
+
+def device_checks(batch, slot, mask, heat, *, tau_fluid, heat_power, Nu=0.0, f_friction=0.0, Re=1.0, Pr=7.0,
+                  k_ratio=628.0):
+    """Checks of the slot's current fields (call after the solve's last readout)."""
+    kf, ks = conductivities(tau_fluid, Pr, k_ratio)
+    r = batch.checks_raw(slot, kf, ks)
+    return close_checks(r, plane_n=mask.shape[1] * mask.shape[2], heat_sum=float(np.sum(heat, dtype=np.float64)),
+                        tau_fluid=tau_fluid, heat_power=heat_power, Nu=Nu, f_friction=f_friction, Re=Re, Pr=Pr,
+                        k_ratio=k_ratio)

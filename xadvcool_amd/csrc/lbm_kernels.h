@@ -878,6 +878,77 @@ __global__ __launch_bounds__(256) void k_metrics(Slot<R>* __restrict__ slots, in
   block_reduce<kNMet>(v, op, partials + (int64_t)blockIdx.x * kNMet);
 }
 
+// Reductions behind the physics acceptance checks of analyze_dataset.py:184-325
+// (mass balance, Mach, density deviation, fluid temperature bound, velocity
+// divergence, energy balance terms).  Index meaning: XADV_C_* in include/xadv.h.
+constexpr int kNChk = 16;
+__constant__ int kChkOp[kNChk] = {0, 0, 1, 1, 2, 0, 0, 0, 0, 1, 0, 0, 0, 0, 0, 0};
+
+template <class R>
+__global__ __launch_bounds__(256) void k_checks(Slot<R>* __restrict__ slots, int slot, double k_fluid,
+                                                double k_solid, double* __restrict__ partials) {
+  const Slot<R>& S = slots[slot];
+  const uint32_t c = (uint32_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double v[kNChk];
+  int op[kNChk];
+#pragma unroll
+  for (int k = 0; k < kNChk; ++k) {
+    op[k] = kChkOp[k];
+    v[k] = op[k] == 1 ? -INFINITY : (op[k] == 2 ? INFINITY : 0.0);
+  }
+  if (c < (uint32_t)S.ncell) {
+    int x, y, z;
+    coords(c, S.P, S.nz, x, y, z);
+    const int nx = S.nx, ny = S.ny, nz = S.nz, P = S.P;
+    const int type = (int)(S.flags[c] & F_TYPE);
+    const bool fluid = type != BC_SOLID;
+    const double r = (double)S.rho[c];
+    const float ux = (float)S.u[3 * c], uy = (float)S.u[3 * c + 1], uz = (float)S.u[3 * c + 2];
+    const double t = (double)S.T[c];
+    if (x == 1 && fluid) v[0] = r * ux;          // mass flow one cell inside the inlet
+    if (x == nx - 2 && fluid) v[1] = r * ux;     // and the outlet
+    if (fluid) {
+      v[2] = sqrtf(ux * ux + uy * uy + uz * uz);
+      v[3] = fabs(r - 1.0);
+      v[4] = t;
+    }
+    if (x == 1) v[5] = r;
+    if (x == nx - 2) v[6] = r;
+    if (fluid && x >= 1 && x <= nx - 2 && y >= 1 && y <= ny - 2 && z >= 1 && z <= nz - 2) {
+      const double d = ((double)S.u[3 * (c + P)] - (double)S.u[3 * (c - P)]) / 2.0 +
+                       ((double)S.u[3 * (c + nz) + 1] - (double)S.u[3 * (c - nz) + 1]) / 2.0 +
+                       ((double)S.u[3 * (c + 1) + 2] - (double)S.u[3 * (c - 1) + 2]) / 2.0;
+      v[7] = fabs(d);
+      v[8] = 1.0;
+      v[9] = fabs(d);
+    }
+    if (x == 0 && type == BC_INLET) v[10] = ux;
+    if (x == nx - 1 && type == BC_OUTLET) v[11] = (double)ux * t;
+    if (x == 1) {
+      const double kk = type == BC_SOLID ? k_solid : k_fluid;
+      v[12] = -kk * (t - (double)S.T[c - P]);
+    }
+    if (x == nx - 2) {
+      const double kk = type == BC_SOLID ? k_solid : k_fluid;
+      v[13] = -kk * ((double)S.T[c + P] - t);
+    }
+    if (fluid) v[14] = 1.0;
+  }
+  block_reduce<kNChk>(v, op, partials + (int64_t)blockIdx.x * kNChk);
+}
+
+__global__ void k_checks_final(int nblocks, const double* __restrict__ partials, double* __restrict__ out) {
+  const int k = threadIdx.x;
+  if (k >= kNChk) return;
+  const int o = kChkOp[k];
+  double a = o == 1 ? -INFINITY : (o == 2 ? INFINITY : 0.0);
+  for (int j = 0; j < nblocks; ++j) {
+    const double w = partials[(int64_t)j * kNChk + k];
+    a = o == 0 ? a + w : (o == 1 ? fmax(a, w) : fmin(a, w));
+  }
+  out[k] = a;
+}
+
 __global__ void k_metrics_final(int nblocks, const double* __restrict__ partials, double* __restrict__ out) {
   const int k = threadIdx.x;
   if (k >= kNMet) return;

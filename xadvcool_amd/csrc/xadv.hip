@@ -581,4 +581,25 @@ int xadv_metrics(xadv_batch* b, int slot, int core_x0, int core_nx, int core_ny,
   return XADV_OK;
 }
 
+int xadv_checks(xadv_batch* b, int slot, double k_fluid, double k_solid, double* out) {
+  if (!b || !out) return fail(XADV_E_ARG, "null argument");
+  if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
+  if (!b->inited[slot]) return fail(XADV_E_STATE, "slot not initialised");
+  int64_t n = b->d.precision == 64 ? host_slots<double>(b)[slot].ncell : host_slots<float>(b)[slot].ncell;
+  const int nb = (int)((n + 255) / 256);
+  static_assert(xadv::kNChk <= xadv::kNMet, "checks reuse the metrics partial buffer");
+  if (b->d.precision == 64)
+    hipLaunchKernelGGL((xadv::k_checks<double>), dim3(nb), dim3(256), 0, b->stream,
+                       reinterpret_cast<Slot<double>*>(b->slots_dev), slot, k_fluid, k_solid, b->met_partials);
+  else
+    hipLaunchKernelGGL((xadv::k_checks<float>), dim3(nb), dim3(256), 0, b->stream,
+                       reinterpret_cast<Slot<float>*>(b->slots_dev), slot, k_fluid, k_solid, b->met_partials);
+  hipLaunchKernelGGL(xadv::k_checks_final, dim3(1), dim3(64), 0, b->stream, nb, b->met_partials, b->met_dev);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->host_small, b->met_dev, xadv::kNChk * sizeof(double), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  memcpy(out, b->host_small, XADV_NCHECK * sizeof(double));
+  return XADV_OK;
+}
+
 }  // extern "C"

@@ -145,6 +145,12 @@ class LatticeBatch:
         _lib.check(self._lib.xadv_fields(self._h, int(slot), p(tr), p(tu), p(tT)), "xadv_fields")
         return tr, tu, tT
 
+    def checks_raw(self, slot, k_fluid, k_solid):
+        out = np.zeros(_lib.NCHECK, np.float64)
+        _lib.check(self._lib.xadv_checks(self._h, int(slot), float(k_fluid), float(k_solid),
+                                         out.ctypes.data_as(ctypes.POINTER(ctypes.c_double))), "xadv_checks")
+        return out
+
     def metrics(self, slot, core_x0=10, core_nx=128, core_ny=128):
         out = np.zeros(_lib.NMETRIC, np.float64)
         _lib.check(self._lib.xadv_metrics(self._h, int(slot), int(core_x0), int(core_nx), int(core_ny),

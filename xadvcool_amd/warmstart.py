@@ -1,0 +1,311 @@
+"""Warm-started batched solving: data generation, surrogate training, and the cold/warm comparison.
+
+* ``generate``: designs are solved cold by the batched solver; their converged
+  fields become training pairs (features, standardised targets, loss weights),
+  kept on the GPU.
+* ``train``: minibatch Adam on the project's HIP kernels (surrogate.py).
+* ``solve_designs``: one batched run over a design list, either cold or warm.
+  Warm designs start from the surrogate's (rho, u, T) (solids at rest);
+  a warm run whose residual grows for ``stall_checks`` consecutive checks
+  after ``stall_after`` steps, or that diverges, is abandoned and re-queued
+  cold (the plain LBM is the fallback and the judge).
+* ``compare``: cold vs warm on the same designs with the same stop rule:
+  steps, converged solves per hour (inference and fallbacks included in the
+  warm arm's wall time), and the relative differences of the converged
+  pressure drop and thermal resistance.
+
+Command line: ``python -m xadvcool_amd.warmstart train|compare ...`` (see main()).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import time
+
+import numpy as np
+import torch
+
+from . import surrogate as S
+from .batch import BatchedSolver
+from .geometry import HEAT_SOURCE_NAMES, family_design
+
+CORE = (128, 128, 32)  # 148 x 128 x 32 lattices
+PAD = (S.round8(CORE[0] + 20), S.round8(CORE[1]), S.round8(CORE[2]))
+COMPARE_SEED0 = 1_000_000  # evaluation designs: seeds far from every training seed
+DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+
+
+def make_designs(seed0, n, heats=HEAT_SOURCE_NAMES):
+    out = []
+    for i in range(n):
+        s = seed0 + i
+        rng = np.random.default_rng(10_000_019 * (s + 1))
+        heat = heats[int(rng.integers(0, len(heats)))]
+        d = family_design(s, core=CORE, heat_name=heat)
+        d["key"] = s
+        out.append(d)
+    return out
+
+
+def _sample(d, fields):
+    rho, u, T = fields
+    f = S.features(d["mask"], d["heat"], d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
+    t, w = S.targets(d["mask"], rho, u, T, d["u_inlet_val"], d["heat_power"], PAD)
+    return (torch.from_numpy(f).to(torch.bfloat16).cuda(), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda())
+
+
+def generate(designs, slots=64, tol=1e-5, max_steps=20000, log=print):
+    """Cold batched solves; returns (samples, results) for the designs that converged."""
+    bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
+                       collect_fields=True, metrics=False)
+    by_key = {d["key"]: d for d in designs}
+    samples, results = [], []
+    t0 = time.perf_counter()
+
+    def on_result(r):
+        if r["status"] == "converged":
+            samples.append(_sample(by_key[r["key"]], r.pop("fields")))
+        else:
+            r.pop("fields", None)
+        results.append(r)
+        if len(results) % 32 == 0:
+            log(f"[datagen] {len(results)}/{len(designs)} solved ({len(samples)} converged), "
+                f"{time.perf_counter() - t0:.0f} s")
+
+    try:
+        bs.run(designs, on_result=on_result)
+    finally:
+        bs.close()
+    return samples, results
+
+
+def fit_norm(samples):
+    """Per-channel mean/std of the targets over their weighted entries."""
+    s1 = torch.zeros(S.NOUT, dtype=torch.float64, device="cuda")
+    s2 = torch.zeros_like(s1)
+    n = torch.zeros_like(s1)
+    for _, t, w in samples:
+        tw = t.double() * w
+        s1 += tw.sum(dim=(0, 1, 2))
+        s2 += (tw * t.double()).sum(dim=(0, 1, 2))
+        n += w.double().sum(dim=(0, 1, 2))
+    mean = s1 / n
+    std = torch.sqrt(torch.clamp(s2 / n - mean * mean, min=1e-12))
+    return mean.float().cpu().numpy(), std.float().cpu().numpy()
+
+
+def train(net, samples, epochs=40, batch=8, lr=5e-4, seed=0, val=None, log=print, ckpt=None, ckpt_every=10,
+          warmup=100):
+    """Adam, linear warm-up over ``warmup`` steps then cosine decay.  (Peak rates of 2e-3 and above blow up the
+    decoder activations of this un-normalised network within ten steps and leave it dead; 5e-4 trains.)"""
+    mean, std = net.norm
+    m_t, s_t = torch.tensor(mean, device="cuda"), torch.tensor(std, device="cuda")
+    rng = np.random.default_rng(seed)
+    nsteps = epochs * max(1, len(samples) // batch)
+    step = 0
+    hist = []
+    t0 = time.perf_counter()
+    for ep in range(epochs):
+        order = rng.permutation(len(samples))
+        tot, cnt = 0.0, 0
+        for b0 in range(0, len(order) - batch + 1, batch):
+            idx = order[b0:b0 + batch]
+            x = torch.stack([samples[i][0] for i in idx])
+            t = (torch.stack([samples[i][1] for i in idx]) - m_t) / s_t
+            w = torch.stack([samples[i][2] for i in idx])
+            out = net.forward(x, keep=True)
+            loss, g = S.mse_loss(out, t, w)
+            net.zero_grad()
+            net.backward(g)
+            cur_lr = lr * min(1.0, (step + 1) / warmup) * 0.5 * (1.0 + np.cos(np.pi * step / max(nsteps, 1)))
+            net.adam_step(float(cur_lr))
+            step += 1
+            tot += loss.item()
+            cnt += 1
+        entry = {"epoch": ep + 1, "train_loss": tot / max(cnt, 1), "s": round(time.perf_counter() - t0, 1)}
+        if val is not None and (ep % 5 == 4 or ep == epochs - 1):
+            entry["val_loss"] = evaluate(net, val)
+        hist.append(entry)
+        log(f"[train] {json.dumps(entry)}")
+        if ckpt is not None and (ep + 1) % ckpt_every == 0:
+            net.save(ckpt)
+    return hist
+
+
+@torch.no_grad()
+def evaluate(net, samples, batch=8):
+    mean, std = net.norm
+    m_t, s_t = torch.tensor(mean, device="cuda"), torch.tensor(std, device="cuda")
+    tot, n = 0.0, 0
+    for b0 in range(0, len(samples), batch):
+        sl = samples[b0:b0 + batch]
+        x = torch.stack([s[0] for s in sl])
+        t = (torch.stack([s[1] for s in sl]) - m_t) / s_t
+        w = torch.stack([s[2] for s in sl])
+        loss, _ = S.mse_loss(net.forward(x), t, w)
+        tot += loss.item() * len(sl)
+        n += len(sl)
+    return tot / max(n, 1)
+
+
+@torch.no_grad()
+def predict_fields(net, d):
+    """Surrogate (rho, u, T) for a design, on the GPU (FP32, lattice shape)."""
+    f = S.features(d["mask"], d["heat"], d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
+    x = torch.from_numpy(f).to(torch.bfloat16).cuda()[None]
+    out = net.forward(x)[0]
+    nx, ny, nz = d["mask"].shape
+    mean, std = net.norm
+    y = out[:nx, :ny, :nz, :S.NOUT] * torch.tensor(std, device="cuda") + torch.tensor(mean, device="cuda")
+    fluid = torch.from_numpy(d["mask"] != S.BC_SOLID).cuda()
+    u = torch.where(fluid[..., None], y[..., 0:3] * d["u_inlet_val"], torch.zeros_like(y[..., 0:3]))
+    rho = torch.where(fluid, 1.0 + y[..., 3] * d["u_inlet_val"], torch.ones_like(y[..., 3]))
+    T = S.T_IN + y[..., 4] * d["heat_power"]
+    return rho.contiguous(), u.contiguous(), T.contiguous()
+
+
+class _Stall:
+    """Residual grows for `checks` consecutive stats checks after `after` steps -> abandon the warm run."""
+
+    def __init__(self, after=1000, checks=5):
+        self.after, self.checks = after, checks
+        self.runs = {}
+
+    def __call__(self, design, step, res):
+        if not design.get("warm"):
+            return False
+        k = design["key"]
+        prev, run = self.runs.get(k, (None, 0))
+        run = run + 1 if (prev is not None and res.d_ke > prev) else 0
+        self.runs[k] = (res.d_ke, run)
+        return step > self.after and run >= self.checks
+
+
+def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None):
+    """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
+
+    Fallback: a warm design that stalls or diverges is re-run cold in the same call."""
+    if predictor is None and net is not None:
+        predictor = lambda d: predict_fields(net, d)  # noqa: E731
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    queue = []
+    t_inf = 0.0
+    for d in designs:
+        dd = dict(d)
+        if predictor is not None:
+            ti = time.perf_counter()
+            dd["init_fields"] = predictor(d)
+            torch.cuda.synchronize()
+            t_inf += time.perf_counter() - ti
+            dd["warm"] = True
+        queue.append(dd)
+    bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
+                       metrics=True, stall_check=_Stall())
+    final = {}
+    fallbacks = []
+    by_key = {q["key"]: q for q in queue}
+    fell_back = set()
+    try:
+        pending = queue
+        while pending:
+            res = bs.run(pending)
+            pending = []
+            for r in res:
+                k = r["key"]
+                if predictor is not None and k not in fell_back and r["status"] in ("stalled", "diverged"):
+                    # the warm start did not lead to convergence: rerun this design cold
+                    cold = {kk: v for kk, v in by_key[k].items() if kk not in ("init_fields", "warm")}
+                    pending.append(cold)
+                    fell_back.add(k)
+                    fallbacks.append((k, r["status"], r["steps"]))
+                    final[k] = {"abandoned_steps": r["steps"]}
+                else:
+                    r["abandoned_steps"] = final.get(k, {}).get("abandoned_steps", 0)
+                    final[k] = r
+    finally:
+        lups = bs.lups
+        bs.close()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    return dict(results=final, wall=wall, inference_s=t_inf, fallbacks=fallbacks, lups=lups)
+
+
+def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print):
+    cold = solve_designs(designs, None, slots, tol, max_steps, log)
+    warm = solve_designs(designs, net, slots, tol, max_steps, log)
+    rows = []
+    for d in designs:
+        c, w = cold["results"][d["key"]], warm["results"][d["key"]]
+        row = dict(key=d["key"], family=d["family"], heat=d["heat_name"], cold_steps=c["steps"],
+                   warm_steps=w["steps"] + w.get("abandoned_steps", 0), cold_status=c["status"],
+                   warm_status=w["status"])
+        if c["converged"] and w["converged"]:
+            for m in ("pressure_drop", "R_th", "Nu", "P_pump"):
+                a, b = w["metrics"][m], c["metrics"][m]
+                row["d_" + m] = abs(a - b) / max(abs(b), 1e-30)
+        rows.append(row)
+    both = [r for r in rows if "d_R_th" in r]
+    summ = {
+        "designs": len(designs),
+        "cold": {"converged": sum(r["converged"] for r in cold["results"].values()), "wall_s": round(cold["wall"], 2),
+                 "solves_per_hr": round(3600 * sum(r["converged"] for r in cold["results"].values()) / cold["wall"], 1),
+                 "mean_steps": float(np.mean([r["cold_steps"] for r in rows]))},
+        "warm": {"converged": sum(r["converged"] for r in warm["results"].values()), "wall_s": round(warm["wall"], 2),
+                 "inference_s": round(warm["inference_s"], 2), "fallbacks": len(warm["fallbacks"]),
+                 "solves_per_hr": round(3600 * sum(r["converged"] for r in warm["results"].values()) / warm["wall"], 1),
+                 "mean_steps": float(np.mean([r["warm_steps"] for r in rows]))},
+        "both_converged": len(both),
+        "max_d_pressure_drop": max((r["d_pressure_drop"] for r in both), default=None),
+        "max_d_R_th": max((r["d_R_th"] for r in both), default=None),
+        "median_d_R_th": float(np.median([r["d_R_th"] for r in both])) if both else None,
+        "within_0.5pct": sum(1 for r in both if r["d_pressure_drop"] <= 5e-3 and r["d_R_th"] <= 5e-3),
+    }
+    summ["speedup_solves_per_hr"] = round(summ["warm"]["solves_per_hr"] / max(summ["cold"]["solves_per_hr"], 1e-9), 3)
+    return summ, rows
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("cmd", choices=["train", "compare"])
+    ap.add_argument("--n-train", type=int, default=512)
+    ap.add_argument("--n-val", type=int, default=32)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--seed0", type=int, default=None,
+                    help="first design seed (default: 0 for train, %d for compare, disjoint from training)"
+                    % COMPARE_SEED0)
+    ap.add_argument("--epochs", type=int, default=40)
+    ap.add_argument("--lr", type=float, default=5e-4)
+    ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--out", default="gpurun_out")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--max-steps", type=int, default=30000)
+    ap.add_argument("--slots", type=int, default=64)
+    args = ap.parse_args()
+    log = lambda s: print(s, flush=True)
+    if args.cmd == "train":
+        designs = make_designs(0 if args.seed0 is None else args.seed0, args.n_train + args.n_val)
+        samples, results = generate(designs, slots=args.slots, log=log)
+        nval = min(args.n_val, len(samples) // 8)
+        val, tr = samples[:nval], samples[nval:]
+        net = S.UNet3D(seed=0)
+        net.norm = fit_norm(tr)
+        log(f"[train] {len(tr)} train / {len(val)} val samples, {net.n_params()} parameters, norm {net.norm}")
+        hist = train(net, tr, epochs=args.epochs, lr=args.lr, val=val, log=log, ckpt=args.weights)
+        net.save(args.weights)
+        with open(f"{args.out}/train_history.json", "w") as f:
+            json.dump({"history": hist, "datagen": [{k: r[k] for k in ("key", "steps", "status")} for r in results]},
+                      f)
+        log(f"[train] saved {args.weights}")
+    else:
+        net = S.UNet3D.load_file(args.weights)
+        designs = make_designs(COMPARE_SEED0 if args.seed0 is None else args.seed0, args.n)
+        summ, rows = compare(net, designs, slots=args.slots, tol=args.tol, max_steps=args.max_steps, log=log)
+        with open(f"{args.out}/warmstart_compare.json", "w") as f:
+            json.dump({"summary": summ, "rows": rows}, f)
+        log(json.dumps(summ))
+
+
+if __name__ == "__main__":
+    main()
