@@ -1,19 +1,24 @@
-"""Benchmark: converged cold-plate solves per hour on one MI355X.
+"""Benchmark: converged cold-plate solves per hour on one MI355X, warm-started vs plain LBM.
 
-Workload (one "step" = one converged solve of one synthetic design):
-full-size single solve on a 148 x 128 x 32 lattice — gyroid TPMS core
+Workload (one "step" = one solve of one synthetic design): full-size single
+solves on a 148 x 128 x 32 lattice — gyroid TPMS core
 (sin x cos y + sin y cos z + sin z cos x < t, cell period in [16, 32], t in
 [-0.4, 0.4]), quad_core heat map on the base, inlet u in [0.01, 0.05], tau in
 [0.52, 0.56], heat power in [0.05, 0.20], all drawn from the design seed;
 FP32 D3Q19 fluid + D3Q19 thermal lattices; run to the residual tolerance
-(the reference's d_ke / dT / dQ residuals, floor off) or 30,000 steps.
+(the reference's d_ke / dT / dQ residuals every 100 steps, floor off) or
+30,000 steps.  The same designs are solved twice, one after the other:
+cold (equilibrium at rest) and warm (surrogate-predicted rho, u, T; a warm
+run that stalls or diverges is re-run cold).  Warm wall time includes the
+surrogate inference and any fallback.
 
-Printed (one JSON line): converged solves/hour, ms per solve, MLUPS (all
-cells x steps / wall time), the fused pass's achieved bandwidth against the
-MI355X HBM roofline (HIP events on the solver's own stream over the timed
-region; algorithmic bytes 152 B per non-solid cell for the fluid lattice plus
-152 B per cell for the thermal lattice, per pass), and the CPU oracle timed on
-this host as the baseline.
+Printed (one JSON line): warm-start converged solves/hour (``value``) with
+the cold-start figure and the relative differences of the converged pressure
+drop and thermal resistance, MLUPS (all cells x steps / wall time), the fused
+pass's achieved bandwidth against the MI355X HBM roofline (HIP events on the
+solver's own stream over the cold arm's passes; algorithmic bytes 152 B per
+non-solid cell for the fluid lattice plus 152 B per cell for the thermal
+lattice, per pass), and the CPU oracle timed on this host as the baseline.
 """
 from __future__ import annotations
 
@@ -56,6 +61,10 @@ class PassTimer:
         self.torch.cuda.synchronize()
         return sum(p[0].elapsed_time(p[1]) for p in self.pairs)
 
+    def totals(self):
+        """(ms, passes, bytes) over every chunk."""
+        return self.first(len(self.pairs))
+
     def first(self, k):
         """(ms, passes, bytes) over the first k chunks."""
         self.torch.cuda.synchronize()
@@ -77,29 +86,6 @@ def load_design(batch, d, Pr=7.0, k_ratio=628.0, T_in=25.0):
                tau_th_fluid=ttf, tau_th_solid=tts, heat_power=d["heat_power"], T_in=T_in, rho_out=1.0,
                u_in=d["u_inlet_val"])
     batch.init(0)
-
-
-def gpu_solves(designs, max_steps, tol, timer_stream=True):
-    import torch
-    from xadvcool_amd.engine import LatticeBatch
-    from xadvcool_amd.simulation import solve_on_batch
-    batch = LatticeBatch(slots=1, cap_shape=LATTICE, graph_chunk=100)
-    timer = PassTimer(batch.stream()) if timer_stream else None
-    results = []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for d in designs:
-        load_design(batch, d)
-        r = solve_on_batch(batch, 0, nx=d["mask"].shape[0], max_steps=max_steps, tau_fluid=d["tau_fluid"],
-                           u_inlet_val=d["u_inlet_val"], stats_every=100, tol_ke=tol, tol_T=tol,
-                           min_steps_rule=False, timer=timer)
-        r["bytes_per_pass"] = algorithmic_bytes_per_pass(d["mask"])
-        r["cells"] = d["mask"].size
-        results.append(r)
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    batch.close()
-    return results, wall, timer
 
 
 def cpu_baseline(design, seconds, threads):
@@ -199,21 +185,51 @@ def batch_bench(n_designs, slots, max_steps, tol, seed0=2000, **shape):
     return out
 
 
+def config1_bench(cpu_steps_1=100, cpu_steps_all=500):
+    """Plain small cold plate (74x64x16, 2,000 steps): the drop-in run_simulation on the GPU and the oracle
+    on this host's cores at 1 thread and at all threads (bounded step counts, MLUPS)."""
+    import torch
+    from oracle import oracle as orc
+    from xadvcool_amd import geometry as G
+    from xadvcool_amd.simulation import run_simulation
+    c = G.CONFIG1
+    m = G.config1_mask()
+    kw = dict(tau_fluid=c["tau_fluid"], heat_power=c["heat_power"], t_inlet=c["t_inlet"], u_inlet_val=c["u_inlet_val"],
+              tol_ke=0.0, tol_T=0.0, Pr=c["Pr"], k_ratio=c["k_ratio"])
+    run_simulation(m, max_steps=200, device="cuda", silent=True, **kw)  # load kernels, build graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = run_simulation(m, max_steps=c["steps"], device="cuda", silent=True, **kw)
+    gpu_s = time.perf_counter() - t0
+    out = {"mode": "config1", "lattice": list(m.shape), "steps": r["steps"],
+           "gpu": {"wall_s": round(gpu_s, 4), "mlups": round(m.size * r["steps"] / gpu_s / 1e6, 1),
+                   "note": "run_simulation end to end: init, graph replays, stats every 100 steps, field copies"}}
+    for threads, n in ((1, cpu_steps_1), (orc.max_threads(), cpu_steps_all)):
+        orc.set_threads(threads)
+        t0 = time.perf_counter()
+        orc.run_simulation(m, max_steps=n, **kw)
+        dt = time.perf_counter() - t0
+        out[f"oracle_{threads}t"] = {"steps": n, "wall_s": round(dt, 3), "mlups": round(m.size * n / dt / 1e6, 2)}
+    out["host_threads"] = orc.max_threads()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=6, help="timed solves (distinct seeded designs)")
+    ap.add_argument("--steps", type=int, default=16, help="timed designs (each solved cold, then warm)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed solves before the timed ones")
     ap.add_argument("--max-steps", type=int, default=30000)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["solves", "passes", "copy", "batch"], default="solves",
+    ap.add_argument("--mode", choices=["solves", "passes", "copy", "batch", "config1"], default="solves",
                     help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy)")
     ap.add_argument("--passes", type=int, default=400)
     ap.add_argument("--copy-mb", type=float, default=512.0)
     ap.add_argument("--designs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
@@ -225,68 +241,104 @@ def main():
         return profile_passes(args.passes)
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
+    if args.mode == "config1":
+        return config1_bench()
     if args.mode == "batch":
         print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots, args.max_steps, args.tol)}),
               flush=True)
         return
 
+    from xadvcool_amd import surrogate as S
+    from xadvcool_amd import warmstart as W
     from xadvcool_amd.geometry import config2_design
 
-    warm = [config2_design(900 + i) for i in range(args.warmup)]
-    timed = [config2_design(1000 + i) for i in range(args.steps)]
-    if warm:
-        gpu_solves(warm, args.max_steps, args.tol, timer_stream=False)
-    results, wall, timer = gpu_solves(timed, args.max_steps, args.tol)
+    def designs(seeds):
+        return [dict(config2_design(sd), key=sd) for sd in seeds]
 
-    n_conv = sum(1 for r in results if r["converged"])
-    total_steps = sum(r["steps"] for r in results)
-    lups = sum(r["steps"] * r["cells"] for r in results)
-    pass_ms_total = timer.total_ms()
-    alg_bytes = sum(r["steps"] * r["bytes_per_pass"] for r in results)
-    avg_pass_ms = pass_ms_total / max(timer.passes, 1)
-    avg_bytes = alg_bytes / max(total_steps, 1)
-    achieved = alg_bytes / (pass_ms_total * 1e-3) / 1e9
-    value = 3600.0 * n_conv / wall
+    seeds = [1000 + i for i in range(args.steps)]
+    timed = designs(seeds)
+    net = S.UNet3D.load_file(args.weights)
+    kw = dict(slots=1, tol=args.tol, max_steps=args.max_steps)
+    if args.warmup:
+        w = designs(range(900, 900 + args.warmup))
+        W.solve_designs(w, None, **kw)
+        W.solve_designs(w, net, **kw)
+    timer = PassTimer(None)
+    cold = W.solve_designs(timed, None, timer=timer, **kw)
+    warm = W.solve_designs(timed, net, **kw)
+
+    def arm(r):
+        res = [r["results"][sd] for sd in seeds]
+        n = sum(1 for x in res if x["converged"])
+        return res, n, 3600.0 * n / r["wall"]
+
+    cres, c_conv, c_rate = arm(cold)
+    wres, w_conv, w_rate = arm(warm)
+    d_dp, d_rth = [], []
+    for c, w in zip(cres, wres):
+        if c["converged"] and w["converged"]:
+            d_dp.append(abs(w["metrics"]["pressure_drop"] / c["metrics"]["pressure_drop"] - 1.0))
+            d_rth.append(abs(w["metrics"]["R_th"] / c["metrics"]["R_th"] - 1.0))
+    cells = timed[0]["mask"].size
+    c_steps = [x["steps"] for x in cres]
+    w_steps = [x["steps"] + x.get("abandoned_steps", 0) for x in wres]
+    pass_ms, npass, alg_bytes = timer.totals()
+    avg_pass_ms = pass_ms / max(npass, 1)
+    achieved = alg_bytes / (pass_ms * 1e-3) / 1e9
 
     line = {
         "metric": "converged solves/hr, warm-start vs plain LBM; MLUPS and % of HBM peak",
-        "value": round(value, 2),
-        "unit": "converged solves/hr (plain cold-start LBM)",
+        "value": round(w_rate, 2),
+        "unit": "converged solves/hr (surrogate warm start, cold fallback, inference included)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * wall / max(args.steps, 1), 2),
+        "ms_per_step": round(1000.0 * warm["wall"] / max(args.steps, 1), 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded gyroid designs, quad_core heat map)",
-        "config": {"workload": "full-size single solve, 148x128x32 gyroid TPMS, to residual tol or 30000 steps",
+        "data": "synthetic (seeded gyroid designs, quad_core heat map; surrogate trained on the solver's own "
+                "solutions of other seeds)",
+        "config": {"workload": "full-size single solves, 148x128x32 gyroid TPMS, to residual tol or 30000 steps, "
+                               "cold then warm-started",
                    "lattice": list(LATTICE), "tol": args.tol, "max_steps": args.max_steps,
                    "stop_rule": "reference d_ke/dT/dQ residuals every 100 steps, flow-through floor off",
-                   "seeds": [1000 + i for i in range(args.steps)]},
-        "mlups": round(lups / wall / 1e6, 1),
-        "converged": n_conv,
-        "steps_to_converge": [r["steps"] for r in results],
+                   "seeds": seeds},
+        "cold_start": {"solves_per_hr": round(c_rate, 2), "converged": c_conv, "wall_s": round(cold["wall"], 3),
+                       "steps": c_steps, "mlups": round(cells * sum(c_steps) / cold["wall"] / 1e6, 1)},
+        "warm_start": {"solves_per_hr": round(w_rate, 2), "converged": w_conv, "wall_s": round(warm["wall"], 3),
+                       "inference_s": round(warm["inference_s"], 3), "fallbacks": len(warm["fallbacks"]),
+                       "steps": w_steps, "both_converged": len(d_dp),
+                       "max_rel_d_pressure_drop": max(d_dp, default=None),
+                       "max_rel_d_R_th": max(d_rth, default=None)},
+        "warm_vs_cold": round(w_rate / c_rate, 3) if c_rate > 0 else None,
+        "mlups": round(cells * (sum(c_steps) + sum(w_steps)) / (cold["wall"] + warm["wall"]) / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "kernel": "k_pass (fused AA fluid+thermal pass)",
                      "avg_launch_us": round(1000.0 * avg_pass_ms, 2),
-                     "alg_bytes_per_launch": int(avg_bytes),
+                     "alg_bytes_per_launch": int(alg_bytes / max(npass, 1)),
                      "note": "one 148x128x32 instance (~95 MB of AA state) fits the 256 MB Infinity Cache: "
                              "LLC-resident, not a pure-HBM figure"},
     }
     if not args.no_cpu:
         mlups1, n1, t1 = cpu_baseline(timed[0], args.cpu_seconds, 1)
-        mean_steps = total_steps / max(len(results), 1)
-        cells = timed[0]["mask"].size
+        cpu_s = sum(c_steps) * cells / (mlups1 * 1e6)  # the cold arm's lattice updates at the oracle's rate
         line["cpu_baseline"] = {
-            "value": round(3600.0 / (mean_steps * cells / (mlups1 * 1e6)), 3),
-            "unit": "converged solves/hr (projected from MLUPS at the GPU's mean steps to converge)",
+            "value": round(3600.0 * c_conv / cpu_s, 4),
+            "unit": "converged solves/hr (the cold arm's designs and steps at the oracle's measured MLUPS)",
             "cores": 1, "kind": "port", "mlups": round(mlups1, 3),
             "sample": f"oracle six-kernel FP32 schedule, 148x128x32 design seed 1000, {n1} steps in {t1:.1f} s, "
                       f"1 thread",
         }
+        from oracle import oracle as orc
+        nt = orc.max_threads()
+        mlups_n, n_n, t_n = cpu_baseline(timed[0], args.cpu_seconds / 2, nt)
+        line["cpu_baseline"]["all_threads"] = {
+            "threads": nt, "mlups": round(mlups_n, 3),
+            "solves_per_hr": round(3600.0 * c_conv / (sum(c_steps) * cells / (mlups_n * 1e6)), 4),
+            "sample": f"{n_n} steps in {t_n:.1f} s"}
     print(json.dumps(line), flush=True)
 
 

@@ -58,6 +58,11 @@ def set_threads(n: int) -> None:
     lib().orc_set_threads(int(n))
 
 
+def max_threads() -> int:
+    """OpenMP's thread count for this process (OMP_NUM_THREADS or the host's cores)."""
+    return int(lib().orc_max_threads())
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
 

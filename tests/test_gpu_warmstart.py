@@ -1,6 +1,6 @@
 """Warm start on the GPU: the cold LBM is the fallback and the judge.
 
-* a warm start that blows up (Ma far above the stability limit) is abandoned at
+* a warm start that blows up (temperatures far above the divergence limit) is abandoned at
   its first residual check and the design is re-run cold: the result equals
   the cold run of the same design bit for bit (same steps, same metrics);
 * the trained surrogate, on designs it never saw, reaches the same stop rule in
@@ -26,13 +26,11 @@ def test_blown_up_warm_start_falls_back_to_cold(gpu):
 
     designs = _designs(2, 7000)
 
-    def bad(d):
-        m = torch.from_numpy(d["mask"] != 1).cuda()
+    def bad(d):  # a thermal explosion: max T far above the divergence limit at the first check
         nx, ny, nz = d["mask"].shape
         u = torch.zeros(nx, ny, nz, 3, device="cuda")
-        u[..., 0] = torch.where(m, 0.4, 0.0)
         rho = torch.ones(nx, ny, nz, device="cuda")
-        T = torch.full((nx, ny, nz), 25.0, device="cuda")
+        T = torch.full((nx, ny, nz), 1.0e6, device="cuda")
         return rho, u, T
 
     kw = dict(slots=2, tol=1e-5, max_steps=6000)

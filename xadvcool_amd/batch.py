@@ -127,7 +127,8 @@ class BatchedSolver:
                 status = None
                 if res.converged(st["steps"]):
                     status = "converged"
-                elif res.diverged():
+                elif res.diverged() or not (np.isfinite(sd["sum_ke"]) and np.isfinite(sd["sum_T"])):
+                    # non-finite sums: the fields hold NaN/inf (the max reductions skip NaN), retire now
                     status = "diverged"
                 elif self.stall_check is not None and self.stall_check(st["design"], st["steps"], res):
                     status = "stalled"

@@ -481,15 +481,50 @@ __device__ __forceinline__ void thermal_scatter(const Slot<R>& S, const Cell& k,
 
 // ------------------------------------------------------------- the pass ----
 
+#ifndef XADV_TH_PREFETCH
+#define XADV_TH_PREFETCH 0
+#endif
+#ifndef XADV_PASS_WAVES
+#define XADV_PASS_WAVES 0
+#endif
+#if XADV_PASS_WAVES > 0
+#define XADV_PASS_ATTR __attribute__((amdgpu_waves_per_eu(XADV_PASS_WAVES)))
+#else
+#define XADV_PASS_ATTR
+#endif
+
+// Thermal gather for a thread's cells: post-collision g of the previous step
+// through the streaming rules, or (first thermal half after an init) the
+// initial populations and T as given (src/simulation.py:73,89).
+template <class R, int GEN, int NODD, int CPT>
+__device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell, int raw, R (*g)[19], R* Tl) {
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    if (!cell[k].valid) continue;
+    if (raw) {
+      const uint32_t N = (uint32_t)S.npad;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) g[k][i] = ld(S.g, (uint32_t)i * N + cell[k].c);
+      Tl[k] = ld(S.T, cell[k].c);
+    } else {
+      thermal_gather<R, GEN, NODD>(S, cell[k], g[k]);
+    }
+  }
+}
+
 // One time step for CPT cells per thread.  The phases (fluid gather, fluid
 // collide + scatter, thermal gather, thermal collide + scatter) run for all
 // of a thread's cells before the next phase, so each thread keeps
-// CPT x 19 loads in flight.  Cell k of a thread is block_base + k*blockDim +
-// threadIdx, so every load instruction of a wavefront covers 64 consecutive
-// cells.
+// CPT x 19 loads in flight.  With XADV_TH_PREFETCH the thermal gather is
+// issued right after the fluid gather (38 loads in flight; the two lattices
+// are separate arrays and each slot read here is written only by this cell,
+// so the order of the fluid stores and thermal loads does not matter).
+// Cell k of a thread is block_base + k*blockDim + threadIdx, so every load
+// instruction of a wavefront covers 64 consecutive cells.
 template <class R, int FL, int TH, int GEN, int NODD, int CPT>
-__global__ __launch_bounds__(256) void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int xcd,
-                                              int want_fields) {
+__global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int xcd,
+                                                             int want_fields) {
+  constexpr bool PRE = XADV_TH_PREFETCH && FL && TH;
   const int b = map_block(G, xcd);
   if (b >= G) return;
   const int si = b / bps;
@@ -524,12 +559,15 @@ __global__ __launch_bounds__(256) void k_pass(Slot<R>* __restrict__ slots, int b
   R ux[CPT], uy[CPT], uz[CPT];
 #pragma unroll
   for (int k = 0; k < CPT; ++k) ux[k] = uy[k] = uz[k] = 0;
+  R g[CPT][19];
+  R Tl[CPT];
 
   if (FL) {
     R f[CPT][19];
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
       if (cell[k].valid && cell[k].type != BC_SOLID) fluid_gather<R, GEN, NODD>(S, cell[k], f[k]);
+    if (PRE) thermal_load<R, GEN, NODD, CPT>(S, cell, raw, g, Tl);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const Cell& q = cell[k];
@@ -581,22 +619,7 @@ __global__ __launch_bounds__(256) void k_pass(Slot<R>* __restrict__ slots, int b
   }
 
   if (TH) {
-    R g[CPT][19];
-    R Tl[CPT];
-#pragma unroll
-    for (int k = 0; k < CPT; ++k) {
-      if (!cell[k].valid) continue;
-      if (raw) {
-        // First thermal half: the initial populations and T as given
-        // (src/simulation.py:73,89), natural layout.
-        const uint32_t N = (uint32_t)S.npad;
-#pragma unroll
-        for (int i = 0; i < 19; ++i) g[k][i] = ld(S.g, (uint32_t)i * N + cell[k].c);
-        Tl[k] = ld(S.T, cell[k].c);
-      } else {
-        thermal_gather<R, GEN, NODD>(S, cell[k], g[k]);
-      }
-    }
+    if (!PRE) thermal_load<R, GEN, NODD, CPT>(S, cell, raw, g, Tl);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       if (!cell[k].valid) continue;

@@ -25,6 +25,9 @@ import numpy as np
 from . import geometry as G
 from .batch import BatchedSolver
 
+# compute_macroscopic_metrics's keys (generate_dataset.py:167-179)
+METRIC_KEYS = ("R_th", "Nu", "f_friction", "P_pump", "COP", "Q_vol", "T_max_surface", "T_avg_surface",
+               "delta_T_max", "sigma_T", "fluid_temp_rise")
 COOLANT_PR = 7.0
 COOLANT_K_RATIO = 628.0
 BUF = 10
@@ -46,15 +49,15 @@ def ordered_samples(samples_per_design, seed=42):
     return out
 
 
-def _metadata_row(sid, p, d, r, met, elapsed, gpu_name, Pr, k_ratio):
+def _metadata_row(sid, p, d, r, met, elapsed, gpu_name, Pr, k_ratio, max_v, attrs):
     n = int(np.prod(d["mask"].shape))
     return {
         "sample_id": sid, "Pr": Pr, "k_ratio": k_ratio, "design_name": p["design_name"],
         "design_idx": p["design_idx"], "heat_source_type": p["heat_source_type"],
         "feature_size": p["feature_size"], "spacing": p["spacing"], "tau_fluid": p["tau_fluid"],
         "u_inlet_val": p["u_inlet_val"], "heat_power": p["heat_power"], "porosity": d["porosity"], "Dh": d["Dh"],
-        "Re": d["Re"], "max_velocity": met["max_velocity"], "max_temperature": met["max_temperature"],
-        "mean_temperature": met["mean_temperature"], "pressure_drop": met["pressure_drop"], "elapsed_s": elapsed,
+        "Re": d["Re"], "max_velocity": max_v, "max_temperature": attrs["max_temperature"],
+        "mean_temperature": attrs["mean_temperature"], "pressure_drop": met["pressure_drop"], "elapsed_s": elapsed,
         "sim_steps": r["steps"], "converged": r["converged"], "grid_nx": d["mask"].shape[0],
         "grid_ny": d["mask"].shape[1], "grid_nz": d["mask"].shape[2], "grid_cells": n,
         "total_lattice_updates": n * r["steps"], "mlups": n * r["steps"] / (elapsed * 1e6) if elapsed > 0 else 0.0,
@@ -127,23 +130,30 @@ def generate_dataset(output_dir, samples_per_design=50, max_steps=30000, seed=42
         rho, u, T = r.pop("fields")
         met = r["metrics"]
         max_v = float(np.max(np.linalg.norm(u, axis=-1)))
+        max_T = float(np.max(T))
         Ma = max_v * 3.0 ** 0.5
-        if np.isnan(max_v) or Ma > 0.3 or np.isnan(met["max_temperature"]) or met["max_temperature"] > 1e4:
-            failed[sid] = f"diverged (max_v={max_v:.2e}, Ma={Ma:.2f}, max_T={met['max_temperature']:.2e})"
+        if np.isnan(max_v) or Ma > 0.3 or np.isnan(max_T) or max_T > 1e4:
+            failed[sid] = f"diverged (max_v={max_v:.2e}, Ma={Ma:.2f}, max_T={max_T:.2e})"
             return
         reasons = plausibility(met, r["converged"])
         if reasons:
             failed[sid] = f"unphysical ({', '.join(reasons)})"
             return
-        attrs = dict(p, coolant="water", Pr=Pr, k_ratio=k_ratio, porosity=d["porosity"], Dh=d["Dh"], Re=d["Re"],
-                     max_velocity=max_v, max_temperature=met["max_temperature"],
-                     mean_temperature=met["mean_temperature"], grid_shape=list(d["mask"].shape),
-                     elapsed_s=r["wall_s"], sim_steps=r["steps"], converged=r["converged"], **met)
+        # attribute order and precedence of generate_dataset.py:495-517 (the metrics are written last)
+        n_cells = int(d["mask"].size)
+        attrs = dict(p)
+        attrs.update(coolant="water", Pr=Pr, k_ratio=k_ratio, porosity=d["porosity"], Dh=d["Dh"], Re=d["Re"],
+                     max_velocity=max_v, max_temperature=max_T, mean_temperature=float(np.mean(T)),
+                     grid_shape=list(d["mask"].shape), elapsed_s=r["wall_s"], sim_steps=r["steps"],
+                     converged=r["converged"], grid_cells=n_cells, total_lattice_updates=n_cells * r["steps"],
+                     mlups=n_cells * r["steps"] / (r["wall_s"] * 1e6) if r["wall_s"] > 0 else 0.0,
+                     gpu_name=gpu_name)
+        attrs.update({k: met[k] for k in METRIC_KEYS})
         np.savez_compressed(os.path.join(output_dir, sid + ".npz"), mask=d["mask"].astype(np.int8),
                             heat_source=d["heat"].astype(np.float32), velocity=u.astype(np.float32),
                             pressure=(rho / 3.0).astype(np.float32), temperature=T.astype(np.float32),
                             attrs=json.dumps(attrs, default=float))
-        rows.append(_metadata_row(sid, p, d, r, met, r["wall_s"], gpu_name, Pr, k_ratio))
+        rows.append(_metadata_row(sid, p, d, r, met, r["wall_s"], gpu_name, Pr, k_ratio, max_v, attrs))
 
     t0 = time.perf_counter()
     try:

@@ -338,6 +338,28 @@ def sample_id(params):
 
 # ------------------------------------------------ benchmark workloads -----
 
+CONFIG1 = dict(shape=(74, 64, 16), tau_fluid=0.6, u_inlet_val=0.02, heat_power=0.1, t_inlet=25.0, Pr=7.0,
+               k_ratio=10.0, steps=2000)
+
+
+def config1_mask():
+    """Plain small cold plate, 74 x 64 x 16: six straight channels of width 4 separated by 2-cell fins
+    (centred in the 62 open cells across y), a 3-cell solid base, a 1-cell lid, side walls, 10-cell open
+    buffers with the inlet at x = 0 and the outlet at x = 73 over the open y, z cells.  The reference's
+    generator cannot produce this layout (its base is 4 cells and its fin gap is derived from the channel
+    count, src/geometry.py:402-403,516), so it is built here and pinned by a committed fixture."""
+    nx, ny, nz = CONFIG1["shape"]
+    buf = BUFFER
+    solid = np.ones((nx - 2 * buf, ny, nz), dtype=bool)
+    n_ch, width, fin = 6, 4, 2
+    span = n_ch * width + (n_ch - 1) * fin
+    y0 = 1 + (ny - 2 - span) // 2
+    for k in range(n_ch):
+        a = y0 + k * (width + fin)
+        solid[:, a:a + width, :] = False
+    return _frame(solid, buf, base=3, top=1)
+
+
 def config2_design(seed, core=(CORE_NX, CORE_NY, CORE_NZ)):
     """Full-size single solve: gyroid with period in [16, 32] and threshold in [-0.4, 0.4], quad_core heat map,
     u_in in [0.01, 0.05]; tau in [0.52, 0.56] and P in [0.05, 0.20] (the generator's ranges), all from ``seed``."""

@@ -182,10 +182,11 @@ class _Stall:
         return step > self.after and run >= self.checks
 
 
-def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None):
+def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None):
     """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
 
-    Fallback: a warm design that stalls or diverges is re-run cold in the same call."""
+    Fallback: a warm design that stalls or diverges is re-run cold in the same call.  ``timer`` (optional):
+    ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes; its ``stream`` is set to the solver's."""
     if predictor is None and net is not None:
         predictor = lambda d: predict_fields(net, d)  # noqa: E731
     torch.cuda.synchronize()
@@ -203,6 +204,8 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
         queue.append(dd)
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
                        metrics=True, stall_check=_Stall())
+    if timer is not None:  # events go on the solver's own stream
+        timer.stream = bs.batch.stream()
     final = {}
     fallbacks = []
     by_key = {q["key"]: q for q in queue}
@@ -210,7 +213,7 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
     try:
         pending = queue
         while pending:
-            res = bs.run(pending)
+            res = bs.run(pending, timer=timer)
             pending = []
             for r in res:
                 k = r["key"]
