@@ -1,36 +1,51 @@
 """Host-side rules of the warm-started batched solver (no GPU needed)."""
 from types import SimpleNamespace
 
+import numpy as np
+
 from xadvcool_amd import warmstart as W
 
-
-def _feed(stall, design, seq, every=100):
-    return [stall(design, (i + 1) * every, SimpleNamespace(d_ke=v)) for i, v in enumerate(seq)]
+TOL = 1e-6
 
 
-def test_stall_fires_after_five_growing_checks_past_step_1000():
-    st = W._Stall(after=1000, checks=5)
-    seq = [1e-3 / (i + 1) for i in range(12)] + [1e-5 * 1.5 ** i for i in range(8)]
-    out = _feed(st, {"key": 1, "warm": True}, seq)
+def _feed(stall, design, dT, every=100):
+    """Residual series with d_ke at FP32-noise level and dQ = dT."""
+    rng = np.random.default_rng(0)
+    out = []
+    for i, v in enumerate(dT):
+        r = SimpleNamespace(d_ke=float(rng.uniform(1e-11, 1e-9)), dT_conv=float(v), dQ_conv=float(v))
+        out.append(stall(design, (i + 1) * every, r))
+    return out
+
+
+def test_decaying_residual_with_dips_is_never_abandoned():
+    st = W._Stall(TOL, TOL)
+    dT = 3e-5 * np.exp(-np.arange(600) / 150.0)
+    dT[[10, 11, 57, 200, 201, 202]] = 1e-9  # rate crossing zero at a few checks
+    assert not any(_feed(st, {"key": 1, "warm": True}, dT))
+
+
+def test_growing_residual_is_abandoned_after_the_start_window():
+    st = W._Stall(TOL, TOL, after=5000)
+    dT = np.concatenate([3e-6 * np.exp(-np.arange(60) / 40.0), 2e-6 * 1.05 ** np.arange(200)])
+    out = _feed(st, {"key": 2, "warm": True}, dT)
     first = out.index(True)
-    assert (first + 1) * 100 == 1800  # drop at 1300, growth 1400..1800 = 5 checks
-    assert not any(out[:first])
+    assert (first + 1) * 100 > 5000
+    assert dT[first] > 4 * dT[:first].min()
 
 
-def test_stall_needs_consecutive_growth_and_ignores_early_steps():
-    st = W._Stall(after=1000, checks=5)
-    # grows for 9 checks but all before step 1000: not a stall at that point
-    early = _feed(st, {"key": 2, "warm": True}, [1e-6 * 2 ** i for i in range(9)])
-    assert not any(early)
-    # alternating growth and decay never reaches 5 in a row
-    st = W._Stall(after=1000, checks=5)
-    zig = _feed(st, {"key": 3, "warm": True}, [1e-5 * (2 if i % 2 else 1) for i in range(40)])
-    assert not any(zig)
+def test_stalled_residual_is_abandoned_after_patience():
+    st = W._Stall(TOL, TOL, after=5000, patience=100)
+    dT = np.full(400, 3e-6)
+    out = _feed(st, {"key": 3, "warm": True}, dT)
+    first = out.index(True)
+    # the median window fills at check 20, the best is set there, then 100 checks without improvement
+    assert first + 1 == 20 + 100
 
 
-def test_stall_never_abandons_cold_runs():
-    st = W._Stall(after=1000, checks=5)
-    out = _feed(st, {"key": 4}, [1e-6 * 2 ** i for i in range(40)])
+def test_cold_runs_are_never_abandoned():
+    st = W._Stall(TOL, TOL)
+    out = _feed(st, {"key": 4}, 1e-6 * 1.1 ** np.arange(300))
     assert not any(out)
 
 

@@ -31,7 +31,7 @@ class BatchedSolver:
     def __init__(self, slots, cap_shape=(148, 128, 32), *, stats_every=100, tol_ke=1e-6, tol_T=1e-6,
                  max_steps=30000, min_steps_rule=False, Pr=7.0, k_ratio=628.0, t_inlet=25.0, precision=32,
                  block=256, cells_per_thread=1, xcd_remap=False, collect_fields=False, metrics=True,
-                 stall_check=None):
+                 stall_check=None, confirm=1):
         self.slots = int(slots)
         self.cap_shape = tuple(cap_shape)
         self.batch = LatticeBatch(slots=self.slots, cap_shape=self.cap_shape, precision=precision, block=block,
@@ -46,6 +46,9 @@ class BatchedSolver:
         # Optional callable(design, step, residuals) -> True to abandon a run
         # (used by the warm-start fallback to detect a stalled or growing residual).
         self.stall_check = stall_check
+        # A slot converges when the reference's residual test passes on `confirm` consecutive checks
+        # (1 = the reference's rule; more keeps a run from stopping where a transient's rate crosses zero).
+        self.confirm = max(1, int(confirm))
         self.state = [None] * self.slots
         self.lups = 0
         self.passes = 0
@@ -68,7 +71,7 @@ class BatchedSolver:
         nx = mask.shape[0]
         min_steps = min_steps_floor(nx, self.max_steps, d["u_inlet_val"]) if self.min_steps_rule else 0
         self.state[s] = dict(design=d, steps=0, res=_Residuals(self.every, self.tol_ke, self.tol_T, min_steps),
-                             cells=int(mask.size), t0=time.perf_counter(), history=[],
+                             cells=int(mask.size), t0=time.perf_counter(), history=[], ok=0,
                              bytes=19 * 4 * 2 * (int(np.count_nonzero(mask != 1)) + int(mask.size)))
 
     def _finish(self, s, status):
@@ -125,7 +128,8 @@ class BatchedSolver:
                 res.update(st["steps"], sd)
                 st["history"].append((st["steps"], res.d_ke, res.dT_conv, res.dQ_conv))
                 status = None
-                if res.converged(st["steps"]):
+                st["ok"] = st["ok"] + 1 if res.converged(st["steps"]) else 0
+                if st["ok"] >= self.confirm:
                     status = "converged"
                 elif res.diverged() or not (np.isfinite(sd["sum_ke"]) and np.isfinite(sd["sum_T"])):
                     # non-finite sums: the fields hold NaN/inf (the max reductions skip NaN), retire now

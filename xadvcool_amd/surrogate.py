@@ -397,17 +397,3 @@ def targets(mask, rho, u, T, u_in, P, pad_to=None):
     w[:nx, :ny, :nz, 0:4] = fluid[..., None]
     w[:nx, :ny, :nz, 4] = 1.0
     return t, w
-
-
-def fields_from_prediction(pred, mask, u_in, P, norm):
-    """Map a (X8,Y8,Z8,>=5) prediction back to (rho, u, T) on the lattice; solids at rest (rho 1, u 0)."""
-    nx, ny, nz = mask.shape
-    mean, std = norm
-    y = pred[:nx, :ny, :nz, :NOUT] * std + mean
-    fluid = (mask != BC_SOLID)
-    u = np.zeros((nx, ny, nz, 3), np.float32)
-    u[fluid] = y[fluid, 0:3] * u_in
-    rho = np.ones((nx, ny, nz), np.float32)
-    rho[fluid] = 1.0 + y[fluid, 3] * u_in
-    T = (T_IN + y[..., 4] * P).astype(np.float32)
-    return rho, u, T
