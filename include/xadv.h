@@ -54,7 +54,15 @@ typedef struct {
   int xcd_remap;        /* 1: map consecutive lattice blocks to one XCD */
   int graph_chunk;      /* steps captured per hipGraph (even, >= 2); 0 = no graphs */
   int cells_per_thread; /* lattice cells per thread in the fused pass: 1 or 2; 0 = default (2) */
+  int fluid_storage;    /* fluid populations: XADV_STORAGE_AUTO (compact where it can be used),
+                           XADV_STORAGE_DENSE (every cell) or XADV_STORAGE_COMPACT (non-solid
+                           cells only: fluid on, general 0, every lattice nz <= 32).  Results are
+                           the same bits either way; compact moves fewer bytes per step. */
 } xadv_batch_desc;
+
+#define XADV_STORAGE_AUTO 0
+#define XADV_STORAGE_DENSE 1
+#define XADV_STORAGE_COMPACT 2
 
 /* One lattice instance.  Mirrors the arguments of src/simulation.py:14-29
  * after its derivations (:70-71 thermal taus, :77-86 heat map normalised,

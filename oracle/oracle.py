@@ -51,7 +51,12 @@ def lib():
         _lib = ctypes.CDLL(_LIB_PATH)
         _lib.orc_set_threads.argtypes = [ctypes.c_int]
         _lib.orc_max_threads.restype = ctypes.c_int
+        global _max0
+        _max0 = int(_lib.orc_max_threads())
     return _lib
+
+
+_max0 = None
 
 
 def set_threads(n: int) -> None:
@@ -59,8 +64,10 @@ def set_threads(n: int) -> None:
 
 
 def max_threads() -> int:
-    """OpenMP's thread count for this process (OMP_NUM_THREADS or the host's cores)."""
-    return int(lib().orc_max_threads())
+    """OpenMP's thread count for this process (OMP_NUM_THREADS or the host's cores), as it was when the library
+    loaded: set_threads() changes what OpenMP reports afterwards, not this."""
+    lib()
+    return _max0
 
 
 def _p(a):

@@ -93,34 +93,39 @@ def hip_coupled(p, steps, precision, chunks=None, graph_chunk=100, **shape):
         s.close()
 
 
+@pytest.mark.parametrize("storage", ["compact", "dense"])
 @pytest.mark.parametrize("steps", [1, 2, 3, 4, 7, 50])
-def test_coupled_fp32_matches_oracle(gpu, steps):
-    print(f"[parity] coupled fp32 steps={steps}", flush=True)
+def test_coupled_fp32_matches_oracle(gpu, steps, storage):
+    """Both fluid storages (non-solid cells only, or every cell) give the oracle's bits."""
+    print(f"[parity] coupled fp32 steps={steps} storage={storage}", flush=True)
     p = coupled_params()
     ref = oracle_coupled(p, steps, np.float32)
-    r, u, T = hip_coupled(p, steps, 32)
+    r, u, T = hip_coupled(p, steps, 32, fluid_storage=storage)
     assert_match("rho", r, ref.rho)
     assert_match("u", u, ref.u)
     assert_match("T", T, ref.T)
 
 
-@pytest.mark.parametrize("block,cpt,xcd", [(64, 1, False), (128, 2, True), (256, 1, True), (64, 2, False)])
-def test_coupled_launch_shapes(gpu, block, cpt, xcd):
+@pytest.mark.parametrize("block,cpt,xcd,storage", [(64, 1, False, "compact"), (128, 2, True, "compact"),
+                                                   (256, 1, True, "compact"), (64, 2, False, "compact"),
+                                                   (128, 2, True, "dense"), (256, 1, False, "dense")])
+def test_coupled_launch_shapes(gpu, block, cpt, xcd, storage):
     """Every launch shape the autotuner may pick computes the same bits."""
-    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd}", flush=True)
+    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd} storage={storage}", flush=True)
     p = coupled_params(seed=2, shape=(23, 13, 11))
     ref = oracle_coupled(p, 13, np.float32)
-    r, u, T = hip_coupled(p, 13, 32, block=block, cells_per_thread=cpt, xcd_remap=xcd)
+    r, u, T = hip_coupled(p, 13, 32, block=block, cells_per_thread=cpt, xcd_remap=xcd, fluid_storage=storage)
     assert_match("rho", r, ref.rho)
     assert_match("u", u, ref.u)
     assert_match("T", T, ref.T)
 
 
-def test_coupled_fp64_matches_oracle(gpu):
-    print("[parity] coupled fp64 steps=20", flush=True)
+@pytest.mark.parametrize("storage", ["compact", "dense"])
+def test_coupled_fp64_matches_oracle(gpu, storage):
+    print(f"[parity] coupled fp64 steps=20 storage={storage}", flush=True)
     p = coupled_params(seed=3)
     ref = oracle_coupled(p, 20, np.float64)
-    r, u, T = hip_coupled(p, 20, 64)
+    r, u, T = hip_coupled(p, 20, 64, fluid_storage=storage)
     assert_match("rho", r, ref.rho, tol=1e-12)
     assert_match("u", u, ref.u, tol=1e-12)
     assert_match("T", T, ref.T, tol=1e-12)
@@ -137,15 +142,16 @@ def test_coupled_graph_and_split_steps(gpu):
     assert_match("T", T, ref.T)
 
 
+@pytest.mark.parametrize("storage", ["compact", "dense"])
 @pytest.mark.parametrize("steps", [1, 2, 5, 30])
-def test_fluid_only_channel(gpu, steps):
-    print(f"[parity] fluid-only channel steps={steps}", flush=True)
+def test_fluid_only_channel(gpu, steps, storage):
+    print(f"[parity] fluid-only channel steps={steps} storage={storage}", flush=True)
     from xadvcool_amd.engine import FluidEngine3D
     m = channel_mask(18, 11, 9, seed=1)
     st = orc.State(m.shape, np.float32)
     st.init_fluid()
     orc.run(st, m, steps, fluid=2, fluid_stream=0, tau=0.6, u_in=0.02, rho_out=1.0)
-    s = FluidEngine3D(m, tau=0.6, u_inlet_val=0.02)
+    s = FluidEngine3D(m, tau=0.6, u_inlet_val=0.02, fluid_storage=storage)
     try:
         s.step(steps)
         r, u, _ = (t.cpu().numpy() for t in s.fields_torch())
@@ -179,6 +185,36 @@ def test_thermal_only_channel(gpu, steps):
         T = s.fields_torch()[2].cpu().numpy()
     finally:
         s.close()
+    assert_match("T", T, st.T)
+
+
+@pytest.mark.parametrize("storage", ["compact", "dense"])
+def test_coupled_from_given_fields(gpu, storage):
+    """Initial rho, u, T given everywhere (solids included, as a warm start may): both storages match."""
+    print(f"[parity] coupled from given fields storage={storage}", flush=True)
+    from xadvcool_amd.engine import LatticeSolver
+    p = coupled_params(seed=4, shape=(21, 14, 12))
+    rng = np.random.default_rng(44)
+    shape = p["mask"].shape
+    rho0 = (1.0 + rng.uniform(-0.01, 0.01, size=shape)).astype(np.float32)
+    u0 = rng.uniform(-0.02, 0.02, size=shape + (3,)).astype(np.float32)
+    T0 = rng.uniform(25.0, 27.0, size=shape).astype(np.float32)
+    st = orc.State(shape, np.float32)
+    st.init_fluid(rho0, u0)
+    st.init_thermal(T0, p["mask"])
+    orc.run(st, p["mask"], 9, fluid=1, thermal=1, tau=p["tau"], u_in=p["u_in"], rho_out=1.0, tau_th_fluid=p["ttf"],
+            tau_th_solid=p["tts"], heat_power=p["P"], T_in=p["T_in"], heat=p["heat"])
+    s = LatticeSolver(p["mask"], fluid=True, thermal=True, heat=p["heat"], tau=p["tau"], tau_th_fluid=p["ttf"],
+                      tau_th_solid=p["tts"], heat_power=p["P"], T_in=p["T_in"], u_in=p["u_in"],
+                      fluid_storage=storage)
+    try:
+        s.init_equilibrium(rho0, u0, T0)
+        s.step(9)
+        r, u, T = (t.cpu().numpy() for t in s.fields_torch())
+    finally:
+        s.close()
+    assert_match("rho", r, st.rho)
+    assert_match("u", u, st.u)
     assert_match("T", T, st.T)
 
 
@@ -281,3 +317,25 @@ def test_stats_match_oracle(gpu):
     finally:
         s.close()
     np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-12)
+
+
+def test_compact_storage_limits(gpu):
+    """Compact fluid storage needs nz <= 32: a taller lattice is refused with a message, and the
+    automatic choice of a batch sized for it is dense storage (which then solves it)."""
+    print("[parity] compact storage limits", flush=True)
+    from xadvcool_amd.engine import LatticeBatch
+    m = np.zeros((6, 4, 40), np.int32)
+    m[:, :, 0] = BC_SOLID
+    b = LatticeBatch(slots=1, cap_shape=(6, 4, 40), fluid_storage="compact")
+    try:
+        with pytest.raises(ValueError, match="nz <= 32"):
+            b.load(0, m, tau=0.6)
+    finally:
+        b.close()
+    b = LatticeBatch(slots=1, cap_shape=(6, 4, 40))
+    try:
+        b.load(0, m, tau=0.6)
+        b.init(0)
+        b.step(4)
+    finally:
+        b.close()

@@ -30,12 +30,13 @@ from .simulation import _Residuals, min_steps_floor
 class BatchedSolver:
     def __init__(self, slots, cap_shape=(148, 128, 32), *, stats_every=100, tol_ke=1e-6, tol_T=1e-6,
                  max_steps=30000, min_steps_rule=False, Pr=7.0, k_ratio=628.0, t_inlet=25.0, precision=32,
-                 block=256, cells_per_thread=1, xcd_remap=False, collect_fields=False, metrics=True,
-                 stall_check=None, confirm=1):
+                 block=256, cells_per_thread=1, xcd_remap=False, fluid_storage="auto", collect_fields=False,
+                 metrics=True, stall_check=None, confirm=1):
         self.slots = int(slots)
         self.cap_shape = tuple(cap_shape)
         self.batch = LatticeBatch(slots=self.slots, cap_shape=self.cap_shape, precision=precision, block=block,
-                                  cells_per_thread=cells_per_thread, xcd_remap=xcd_remap, graph_chunk=stats_every)
+                                  cells_per_thread=cells_per_thread, xcd_remap=xcd_remap, graph_chunk=stats_every,
+                                  fluid_storage=fluid_storage)
         self.every = int(stats_every)
         self.tol_ke, self.tol_T = tol_ke, tol_T
         self.max_steps = int(max_steps)

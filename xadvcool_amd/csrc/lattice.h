@@ -48,4 +48,70 @@ enum : uint32_t {
   F_XH = 1u << 30,          // thermal x-high extrapolation cell (x = nx-1, not INLET/OUTLET)
 };
 
+// Compact storage of the fluid lattice.  SOLID cells hold no fluid
+// populations (theirs never change), so the fluid planes store only the
+// non-solid cells, numbered in cell order (x, y, z).  Column q = x * ny + y
+// records the number of its first non-solid cell and one bit per z that is set
+// for a non-solid cell (nz <= 32); the number of a non-solid (x, y, z) is
+// start + popcount(bits below z).  Neighbour numbers therefore come from the
+// 3 x 3 columns around a cell, without a per-cell neighbour table.
+struct ColInfo {
+  uint32_t start, bits;
+};
+constexpr int kCompactMaxNz = 32;
+
+__host__ __device__ inline uint32_t bits_below(int z) {
+  return z <= 0 ? 0u : (z >= 32 ? 0xffffffffu : ((1u << z) - 1u));
+}
+__host__ __device__ inline uint32_t compact_index(ColInfo c, int z) {
+  return c.start + (uint32_t)__builtin_popcount(c.bits & bits_below(z));
+}
+
+// The 3 x 3 columns around column q (col[dx + 1][dy + 1] is column
+// q + dx * ny + dy); indices off the column range are clamped, since the
+// columns there are never used (those neighbours lie outside the lattice).
+__host__ __device__ inline void around_columns(const ColInfo* cols, int q, int ny, int ncol, ColInfo (&col)[3][3]) {
+#pragma unroll
+  for (int dx = -1; dx <= 1; ++dx)
+#pragma unroll
+    for (int dy = -1; dy <= 1; ++dy) {
+      int t = q + dx * ny + dy;
+      t = t < 0 ? 0 : (t >= ncol ? ncol - 1 : t);
+      col[dx + 1][dy + 1] = cols[t];
+    }
+}
+
+// Number of the non-solid cell at offset (DX, DY, DZ) from non-solid cell
+// number j at height z.  Meaningful only where that neighbour is inside the
+// lattice and non-solid; callers discard it elsewhere.
+template <int DX, int DY, int DZ>
+__host__ __device__ inline uint32_t nb_number(const ColInfo (&col)[3][3], uint32_t j, int z) {
+  if (DX == 0 && DY == 0) return DZ > 0 ? j + 1u : j - 1u;  // same column: next / previous non-solid cell
+  return compact_index(col[DX + 1][DY + 1], z + DZ);
+}
+
+// Column table of a mask (int8 codes, C order (nx, ny, nz), nz <= 32) into
+// cols[nx * ny]; returns the number of non-solid cells.
+inline int64_t build_columns(const int8_t* mask, int nx, int ny, int nz, ColInfo* cols) {
+  int64_t n = 0;
+  const int64_t ncol = (int64_t)nx * ny;
+  for (int64_t q = 0; q < ncol; ++q) {
+    uint32_t bits = 0;
+    for (int z = 0; z < nz; ++z)
+      if (mask[q * nz + z] != BC_SOLID) bits |= 1u << z;
+    cols[q].start = (uint32_t)n;
+    cols[q].bits = bits;
+    n += __builtin_popcount(bits);
+  }
+  return n;
+}
+
+// J[k] = number of the cell at c + e_k, k = 1..18.
+__host__ __device__ inline void neighbor_numbers(const ColInfo (&col)[3][3], uint32_t j, int z, uint32_t* J) {
+#define XADV_NB(k) J[k] = nb_number<ex_of(k), ey_of(k), ez_of(k)>(col, j, z);
+  XADV_NB(1) XADV_NB(2) XADV_NB(3) XADV_NB(4) XADV_NB(5) XADV_NB(6) XADV_NB(7) XADV_NB(8) XADV_NB(9)
+  XADV_NB(10) XADV_NB(11) XADV_NB(12) XADV_NB(13) XADV_NB(14) XADV_NB(15) XADV_NB(16) XADV_NB(17) XADV_NB(18)
+#undef XADV_NB
+}
+
 }  // namespace xadv

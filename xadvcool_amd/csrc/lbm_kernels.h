@@ -54,8 +54,10 @@ struct Slot {
   R omega, rho_out, u_in, force_x, u_wall;
   R omega_tf, omega_ts, sf_tf, sf_ts;  // thermal 1/tau and (1 - 0.5*omega): fluid, solid
   R T_in, dir_T;
+  uint32_t nf;  // plane stride of f: npad (dense) or the padded non-solid count (compact)
   R* f;
   R* g;
+  const ColInfo* cols;  // compact fluid storage: one entry per (x, y) column, else null
   const uint32_t* flags;
   const R* q;
   R* fin;    // [19][P]   INLET own post-collision set
@@ -187,6 +189,7 @@ struct Cell {
   uint32_t fl;  // flag word
   int type;
   int x, y, z;  // coordinates (general mode only)
+  uint32_t j;   // compact fluid storage: the cell's number among non-solid cells
   bool valid;
 };
 
@@ -243,9 +246,79 @@ __device__ __forceinline__ void fluid_store(R* __restrict__ A, uint32_t N, const
   st(A, a, v);
 }
 
+// ------------------------------------------------- compact fluid storage ---
+//
+// Same AA slots as above, addressed by non-solid cell number: slot (i, cell)
+// lives at element i * nf + number(cell).  No non-solid cell ever reads or
+// writes a slot of a SOLID cell (those accesses are the bounce-back redirects
+// to the cell's own slots), so dropping the SOLID cells changes no value.
+
+// The cell's own number and, for odd passes, the numbers of its 18
+// neighbours: J[k] is the number of the cell at c + e_k.
+template <class R, int NODD>
+__device__ __forceinline__ void compact_numbers(const Slot<R>& S, Cell& k, uint32_t* J) {
+  const uint32_t nz = (uint32_t)S.nz;
+  const uint32_t q = k.c / nz;
+  const int z = (int)(k.c - q * nz);
+  if (!NODD) {
+    k.j = compact_index(S.cols[q], z);
+    return;
+  }
+  ColInfo col[3][3];
+  around_columns(S.cols, (int)q, S.ny, S.nx * S.ny, col);
+  k.j = compact_index(col[1][1], z);
+  neighbor_numbers(col, k.j, z, J);
+}
+
+template <class R, int NODD, int i>
+__device__ __forceinline__ R fluid_load_c(const R* __restrict__ A, uint32_t NF, const Cell& k, const uint32_t* J) {
+  constexpr uint32_t rs = sizeof(R);
+  if (NODD == 0 || i == 0) return ldb<R>(A, ((uint32_t)i * NF + k.j) * rs);
+  // Pull from the neighbour's opposite slot, or (bounce-back) the own slot i.
+  const uint32_t own = (uint32_t)i * NF + k.j;
+  const uint32_t nb = (uint32_t)opp_of(i) * NF + J[opp_of(i)];
+  const uint32_t bb = bitmask(k.fl, F_BOUNCE0 + i);
+  return ldb<R>(A, ((bb & own) | (~bb & nb)) * rs);
+}
+
+template <class R, int NODD, int i>
+__device__ __forceinline__ void fluid_store_c(R* __restrict__ A, uint32_t NF, const Cell& k, const uint32_t* J, R v) {
+  constexpr uint32_t rs = sizeof(R);
+  if (NODD == 0 || i == 0) {
+    stb<R>(A, ((uint32_t)opp_of(i) * NF + k.j) * rs, v);
+    return;
+  }
+  // Push into the neighbour's slot i, or (target outside / SOLID) the own slot opp(i).
+  const uint32_t own = (uint32_t)opp_of(i) * NF + k.j;
+  const uint32_t nb = (uint32_t)i * NF + J[i];
+  const uint32_t bb = bitmask(k.fl, F_BOUNCE0 + opp_of(i));
+  stb<R>(A, ((bb & own) | (~bb & nb)) * rs, v);
+}
+
+// Moments of a SOLID cell's frozen populations (src/fluid_engine3d.py:92-113,
+// the full sums over i = 0..18 as the reference forms them).
+template <class R>
+__device__ __forceinline__ void solid_moments(const R* f, R& lr, R& sx, R& sy, R& sz) {
+  R mx = 0, my = 0, mz = 0;
+  lr = 0;
+#pragma unroll
+  for (int i = 0; i < 19; ++i) {
+    lr += f[i];
+    mx += f[i] * (R)ex_of(i);
+    my += f[i] * (R)ey_of(i);
+    mz += f[i] * (R)ez_of(i);
+  }
+  sx = sy = sz = 0;
+  if (lr > (R)1e-10) {
+    sx = mx / lr;
+    sy = my / lr;
+    sz = mz / lr;
+  }
+}
+
 // Streamed fluid populations of a non-solid cell for the pass of parity NODD.
-template <class R, int GEN, int NODD>
-__device__ __forceinline__ void fluid_gather(const Slot<R>& S, const Cell& k, R* f) {
+template <class R, int GEN, int NODD, int CMP>
+__device__ __forceinline__ void fluid_gather(const Slot<R>& S, const Cell& k, const uint32_t* J, R* f) {
   const int P = S.P, nz = S.nz;
   const uint32_t N = (uint32_t)S.npad;
   if (k.type == BC_INLET) {
@@ -279,6 +352,14 @@ __device__ __forceinline__ void fluid_gather(const Slot<R>& S, const Cell& k, R*
 #pragma unroll
       for (int i = 0; i < 19; ++i) f[i] *= sc;
     }
+  } else if (CMP) {
+    const R* __restrict__ A = S.f;
+    const uint32_t NF = S.nf;
+#define XADV_FL(i) f[i] = fluid_load_c<R, NODD, i>(A, NF, k, J);
+    XADV_FL(0) XADV_FL(1) XADV_FL(2) XADV_FL(3) XADV_FL(4) XADV_FL(5) XADV_FL(6) XADV_FL(7) XADV_FL(8)
+    XADV_FL(9) XADV_FL(10) XADV_FL(11) XADV_FL(12) XADV_FL(13) XADV_FL(14) XADV_FL(15) XADV_FL(16)
+    XADV_FL(17) XADV_FL(18)
+#undef XADV_FL
   } else {
     const R* __restrict__ A = S.f;
     const R uw = GEN ? S.u_wall : (R)0;
@@ -323,17 +404,26 @@ __device__ __forceinline__ void fluid_collide(const Slot<R>& S, R* f, R& lr, R& 
   }
 }
 
-template <class R, int GEN, int NODD>
-__device__ __forceinline__ void fluid_scatter(const Slot<R>& S, const Cell& k, const R* f) {
+template <class R, int GEN, int NODD, int CMP>
+__device__ __forceinline__ void fluid_scatter(const Slot<R>& S, const Cell& k, const uint32_t* J, const R* f) {
   R* __restrict__ A = S.f;
   const uint32_t N = (uint32_t)S.npad;
   const int P = S.P, nz = S.nz, nx = S.nx, ny = S.ny;
   const R uw = GEN ? S.u_wall : (R)0;
-#define XADV_FS(i) fluid_store<R, GEN, NODD, i>(A, N, k, P, nz, nx, ny, uw, f[i]);
-  XADV_FS(0) XADV_FS(1) XADV_FS(2) XADV_FS(3) XADV_FS(4) XADV_FS(5) XADV_FS(6) XADV_FS(7) XADV_FS(8)
-  XADV_FS(9) XADV_FS(10) XADV_FS(11) XADV_FS(12) XADV_FS(13) XADV_FS(14) XADV_FS(15) XADV_FS(16)
-  XADV_FS(17) XADV_FS(18)
+  if (CMP) {
+    const uint32_t NF = S.nf;
+#define XADV_FS(i) fluid_store_c<R, NODD, i>(A, NF, k, J, f[i]);
+    XADV_FS(0) XADV_FS(1) XADV_FS(2) XADV_FS(3) XADV_FS(4) XADV_FS(5) XADV_FS(6) XADV_FS(7) XADV_FS(8)
+    XADV_FS(9) XADV_FS(10) XADV_FS(11) XADV_FS(12) XADV_FS(13) XADV_FS(14) XADV_FS(15) XADV_FS(16)
+    XADV_FS(17) XADV_FS(18)
 #undef XADV_FS
+  } else {
+#define XADV_FS(i) fluid_store<R, GEN, NODD, i>(A, N, k, P, nz, nx, ny, uw, f[i]);
+    XADV_FS(0) XADV_FS(1) XADV_FS(2) XADV_FS(3) XADV_FS(4) XADV_FS(5) XADV_FS(6) XADV_FS(7) XADV_FS(8)
+    XADV_FS(9) XADV_FS(10) XADV_FS(11) XADV_FS(12) XADV_FS(13) XADV_FS(14) XADV_FS(15) XADV_FS(16)
+    XADV_FS(17) XADV_FS(18)
+#undef XADV_FS
+  }
   if (k.type == BC_INLET) {
 #pragma unroll
     for (int i = 0; i < 19; ++i) st(S.fin, (uint32_t)(i * P) + k.c, f[i]);
@@ -521,9 +611,10 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // so the order of the fluid stores and thermal loads does not matter).
 // Cell k of a thread is block_base + k*blockDim + threadIdx, so every load
 // instruction of a wavefront covers 64 consecutive cells.
-template <class R, int FL, int TH, int GEN, int NODD, int CPT>
+template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP>
 __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int xcd,
                                                              int want_fields) {
+  static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
   constexpr bool PRE = XADV_TH_PREFETCH && FL && TH;
   const int b = map_block(G, xcd);
   if (b >= G) return;
@@ -553,6 +644,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
     q.fl = q.valid ? ld(S.flags, q.c) : (uint32_t)BC_SOLID;
     q.type = (int)(q.fl & F_TYPE);
     q.x = q.y = q.z = 0;
+    q.j = 0;
     if (GEN && q.valid) coords(q.c, S.P, S.nz, q.x, q.y, q.z);
   }
 
@@ -564,9 +656,13 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
 
   if (FL) {
     R f[CPT][19];
+    uint32_t J[CPT][CMP && NODD ? 19 : 1];
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
-      if (cell[k].valid && cell[k].type != BC_SOLID) fluid_gather<R, GEN, NODD>(S, cell[k], f[k]);
+      if (cell[k].valid && cell[k].type != BC_SOLID) {
+        if (CMP) compact_numbers<R, NODD>(S, cell[k], J[k]);
+        fluid_gather<R, GEN, NODD, CMP>(S, cell[k], J[k], f[k]);
+      }
     if (PRE) thermal_load<R, GEN, NODD, CPT>(S, cell, raw, g, Tl);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -581,26 +677,17 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
           st(S.u, 3 * q.c + 1, uy[k]);
           st(S.u, 3 * q.c + 2, uz[k]);
         }
-        fluid_scatter<R, GEN, NODD>(S, q, f[k]);
-      } else if (want_fields) {
+        fluid_scatter<R, GEN, NODD, CMP>(S, q, J[k], f[k]);
+      } else if (want_fields && !CMP) {
         // Solid cells keep their initial populations, stored in the even
-        // layout (slot opp(i)); the reference's moment kernel still runs on them.
+        // layout (slot opp(i)); the reference's moment kernel still runs on
+        // them.  (With compact storage the init writes these moments once.)
         const uint32_t N = (uint32_t)S.npad;
-        R lr = 0, mx = 0, my = 0, mz = 0;
+        R fs[19];
 #pragma unroll
-        for (int i = 0; i < 19; ++i) {
-          const R v = ld(S.f, (uint32_t)opp_of(i) * N + q.c);
-          lr += v;
-          mx += v * (R)ex_of(i);
-          my += v * (R)ey_of(i);
-          mz += v * (R)ez_of(i);
-        }
-        R sx = 0, sy = 0, sz = 0;
-        if (lr > (R)1e-10) {
-          sx = mx / lr;
-          sy = my / lr;
-          sz = mz / lr;
-        }
+        for (int i = 0; i < 19; ++i) fs[i] = ld(S.f, (uint32_t)opp_of(i) * N + q.c);
+        R lr, sx, sy, sz;
+        solid_moments<R>(fs, lr, sx, sy, sz);
         st(S.rho, q.c, lr);
         st(S.u, 3 * q.c, sx);
         st(S.u, 3 * q.c + 1, sy);
@@ -641,7 +728,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
 // fluid, BGK(f_eq(rho,u), rho, u) returns f_eq exactly, so only the body force
 // term changes the stored values.  The thermal lattice keeps g_eq in the
 // natural layout for the raw first pass.
-template <class R, int FL, int TH, int GEN>
+template <class R, int FL, int TH, int GEN, int CMP>
 __global__ __launch_bounds__(256) void k_init(Slot<R>* __restrict__ slots, int slot, int frozen_u) {
   Slot<R>& S = slots[slot];
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -651,7 +738,51 @@ __global__ __launch_bounds__(256) void k_init(Slot<R>* __restrict__ slots, int s
   const int64_t N = S.npad;
   const int P = S.P;
   const R ux = S.u[3 * c], uy = S.u[3 * c + 1], uz = S.u[3 * c + 2];
-  if (FL) {
+  if (FL && CMP) {
+    // Compact storage: non-solid cells get f_eq in their own slots; SOLID
+    // cells store nothing and their (frozen) moments go to the fields now.
+    const R lr = S.rho[c];
+    const R usq = ux * ux + uy * uy + uz * uz;
+    R f[19];
+#define XADV_FE(i) f[i] = feq_i<R, i>(lr, ux, uy, uz, usq);
+    XADV_FE(0) XADV_FE(1) XADV_FE(2) XADV_FE(3) XADV_FE(4) XADV_FE(5) XADV_FE(6) XADV_FE(7) XADV_FE(8)
+    XADV_FE(9) XADV_FE(10) XADV_FE(11) XADV_FE(12) XADV_FE(13) XADV_FE(14) XADV_FE(15) XADV_FE(16)
+    XADV_FE(17) XADV_FE(18)
+#undef XADV_FE
+    if (type == BC_SOLID) {
+      R sr, sx, sy, sz;
+      solid_moments<R>(f, sr, sx, sy, sz);
+      S.rho[c] = sr;
+      S.u[3 * c] = sx;
+      S.u[3 * c + 1] = sy;
+      S.u[3 * c + 2] = sz;
+    } else {
+      const R om = S.omega;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) f[i] = f[i] - om * (f[i] - f[i]);
+      const uint32_t nz = (uint32_t)S.nz;
+      const uint32_t q = (uint32_t)c / nz;
+      const uint32_t j = compact_index(S.cols[q], (int)((uint32_t)c - q * nz));
+      R* __restrict__ A = S.f;
+      const int64_t NF = S.nf;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) A[(int64_t)opp_of(i) * NF + j] = f[i];
+      if (type == BC_INLET) {
+#pragma unroll
+        for (int i = 0; i < 19; ++i) S.fin[(int64_t)i * P + c] = f[i];
+      }
+    }
+    if (fl & F_SNAP) {
+      // Also for a SOLID cell left of an OUTLET: the outlet copies its frozen
+      // populations every step, and passes never rewrite them.
+      const int64_t p = c - (int64_t)(S.nx - 2) * P;
+#pragma unroll
+      for (int i = 0; i < 19; ++i) {
+        S.fsnap[(int64_t)i * P + p] = f[i];
+        S.fsnap[(int64_t)(19 + i) * P + p] = f[i];
+      }
+    }
+  } else if (FL) {
     const R lr = S.rho[c];
     const R usq = ux * ux + uy * uy + uz * uz;
     R f[19];

@@ -49,9 +49,10 @@ struct xadv_batch {
   void* slots_dev = nullptr;
   void* slots_host = nullptr;  // pinned copy of the slot table
   size_t slot_bytes = 0;
+  int compact = 0;  // fluid lattice stored for non-solid cells only
   // arenas (bytes)
   char *f = nullptr, *g = nullptr, *flags = nullptr, *q = nullptr, *fin = nullptr, *fsnap = nullptr, *gsnap = nullptr,
-       *gxh = nullptr, *rho = nullptr, *u = nullptr, *T = nullptr;
+       *gxh = nullptr, *rho = nullptr, *u = nullptr, *T = nullptr, *cols = nullptr;
   double* partials = nullptr;
   double* stats_dev = nullptr;
   double* met_partials = nullptr;
@@ -99,24 +100,33 @@ int push_field(xadv_batch* b, int s, size_t off, size_t bytes) {
 
 // ----------------------------------------------------------- dispatch -----
 
-template <class R, int FL, int TH, int GEN, int CPT>
+template <class R, int FL, int TH, int GEN, int CPT, int CMP>
 void launch_pass_c(xadv_batch* b, int nodd, int want_fields) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
   dim3 grid(b->G_launch), block(b->d.block);
   if (nodd)
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT>), grid, block, 0, b->stream, sl, b->bps, b->G,
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT, CMP>), grid, block, 0, b->stream, sl, b->bps, b->G,
                        b->d.xcd_remap, want_fields);
   else
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT>), grid, block, 0, b->stream, sl, b->bps, b->G,
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT, CMP>), grid, block, 0, b->stream, sl, b->bps, b->G,
                        b->d.xcd_remap, want_fields);
+}
+
+// Compact storage exists for the channel kernels with the fluid lattice on.
+template <class R, int FL, int TH, int GEN, int CPT>
+void launch_pass_s(xadv_batch* b, int nodd, int want_fields) {
+  if constexpr (FL && !GEN) {
+    if (b->compact) return launch_pass_c<R, FL, TH, GEN, CPT, 1>(b, nodd, want_fields);
+  }
+  launch_pass_c<R, FL, TH, GEN, CPT, 0>(b, nodd, want_fields);
 }
 
 template <class R, int FL, int TH, int GEN>
 void launch_pass_t(xadv_batch* b, int nodd, int want_fields) {
   if (b->d.cells_per_thread == 2)
-    launch_pass_c<R, FL, TH, GEN, 2>(b, nodd, want_fields);
+    launch_pass_s<R, FL, TH, GEN, 2>(b, nodd, want_fields);
   else
-    launch_pass_c<R, FL, TH, GEN, 1>(b, nodd, want_fields);
+    launch_pass_s<R, FL, TH, GEN, 1>(b, nodd, want_fields);
 }
 
 template <class R, int FL, int TH, int GEN>
@@ -136,7 +146,13 @@ void launch_init_t(xadv_batch* b, int slot, int64_t ncell, int frozen) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
   const int bs = 256;
   dim3 grid((unsigned)((ncell + bs - 1) / bs)), block(bs);
-  hipLaunchKernelGGL((xadv::k_init<R, FL, TH, GEN>), grid, block, 0, b->stream, sl, slot, frozen);
+  if constexpr (FL && !GEN) {
+    if (b->compact) {
+      hipLaunchKernelGGL((xadv::k_init<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, slot, frozen);
+      return;
+    }
+  }
+  hipLaunchKernelGGL((xadv::k_init<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, slot, frozen);
 }
 
 #define XADV_DISPATCH(FN, R, ...)                                        \
@@ -225,6 +241,17 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
   int rc = build_flags(d, b->d.fluid, b->d.thermal, b->d.general, fl);
   if (rc) return rc;
   const int64_t n = (int64_t)d->nx * d->ny * d->nz;
+  // Compact fluid storage: per (x, y) column, the number of its first
+  // non-solid cell and a bit per z for its non-solid cells.
+  std::vector<xadv::ColInfo> cols;
+  int64_t nfluid = 0;
+  if (b->compact) {
+    if (d->nz > xadv::kCompactMaxNz)
+      return fail(XADV_E_ARG, "this batch stores the fluid lattice compactly, which needs nz <= 32; create the "
+                              "batch with fluid_storage = XADV_STORAGE_DENSE (or cap_nz > 32) for taller lattices");
+    cols.resize((size_t)d->nx * d->ny);
+    nfluid = xadv::build_columns(d->mask, d->nx, d->ny, d->nz, cols.data());
+  }
   std::vector<R> q(n, (R)0);
   if (d->heat) {
     for (int64_t c = 0; c < n; ++c) q[c] = (R)d->heat[c] * (R)d->heat_power;  // thermal_engine3d.py:116-117
@@ -258,8 +285,11 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
   S.sf_ts = (R)1.0 - (R)0.5 * S.omega_ts;
   S.T_in = (R)d->T_in;
   S.dir_T = (R)d->dirichlet_T;
+  S.nf = (uint32_t)(b->compact ? round_up(nfluid, 64) : S.npad);
   S.f = reinterpret_cast<R*>(b->f + (size_t)s * 19 * capn * rs);
   S.g = reinterpret_cast<R*>(b->g + (size_t)s * 19 * capn * rs);
+  S.cols = b->compact ? reinterpret_cast<const xadv::ColInfo*>(b->cols + (size_t)s * cap * sizeof(xadv::ColInfo))
+                      : nullptr;
   S.flags = reinterpret_cast<const uint32_t*>(b->flags + (size_t)s * cap * 4);
   S.q = reinterpret_cast<const R*>(b->q + (size_t)s * cap * rs);
   S.fin = reinterpret_cast<R*>(b->fin + (size_t)s * 19 * capP * rs);
@@ -271,6 +301,9 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
   S.T = reinterpret_cast<R*>(b->T + (size_t)s * cap * rs);
   HIPCHK(hipMemcpyAsync((void*)S.flags, fl.data(), n * 4, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemcpyAsync((void*)S.q, q.data(), n * rs, hipMemcpyHostToDevice, b->stream));
+  if (b->compact)
+    HIPCHK(hipMemcpyAsync((void*)S.cols, cols.data(), cols.size() * sizeof(xadv::ColInfo), hipMemcpyHostToDevice,
+                          b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->loaded[s] = 1;
   b->inited[s] = 0;
@@ -341,8 +374,15 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   if (d.graph_chunk < 0 || (d.graph_chunk % 2)) return fail(XADV_E_ARG, "graph_chunk must be even and >= 0");
   if (d.cells_per_thread == 0) d.cells_per_thread = 2;
   if (d.cells_per_thread != 1 && d.cells_per_thread != 2) return fail(XADV_E_ARG, "cells_per_thread must be 1 or 2");
+  if (d.fluid_storage < XADV_STORAGE_AUTO || d.fluid_storage > XADV_STORAGE_COMPACT)
+    return fail(XADV_E_ARG, "fluid_storage must be XADV_STORAGE_AUTO, _DENSE or _COMPACT");
+  const bool can_compact = d.fluid && !d.general;
+  if (d.fluid_storage == XADV_STORAGE_COMPACT && !can_compact)
+    return fail(XADV_E_ARG, "compact fluid storage needs the fluid lattice on and general = 0");
   auto* b = new xadv_batch();
   b->d = d;
+  b->compact = can_compact && (d.fluid_storage == XADV_STORAGE_COMPACT ||
+                               (d.fluid_storage == XADV_STORAGE_AUTO && d.cap_nz <= xadv::kCompactMaxNz));
   b->rsize = d.precision == 64 ? 8 : 4;
   b->cap_cells = (int64_t)d.cap_nx * d.cap_ny * d.cap_nz;
   b->cap_npad = round_up(b->cap_cells, 64);
@@ -385,6 +425,7 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
       {&b->rho, S * cap * rs},
       {&b->u, S * 3 * cap * rs},
       {&b->T, S * cap * rs},
+      {&b->cols, b->compact ? S * cap * sizeof(xadv::ColInfo) : 0},  // nx * ny <= cells per slot
   };
   for (auto& a : allocs) {
     if (!a.bytes) continue;
@@ -416,7 +457,8 @@ int xadv_batch_destroy(xadv_batch* b) {
   if (!b) return XADV_OK;
   if (b->stream) hipStreamSynchronize(b->stream);
   for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
-  char** ps[] = {&b->f, &b->g, &b->flags, &b->q, &b->fin, &b->fsnap, &b->gsnap, &b->gxh, &b->rho, &b->u, &b->T};
+  char** ps[] = {&b->f, &b->g, &b->flags, &b->q, &b->fin, &b->fsnap, &b->gsnap, &b->gxh, &b->rho, &b->u, &b->T,
+                 &b->cols};
   for (char** p : ps)
     if (*p) hipFree(*p);
   if (b->partials) hipFree(b->partials);
