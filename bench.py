@@ -73,19 +73,32 @@ class PassTimer:
                 sum(p[2] * (p[3] or 0) for p in sel))
 
 
+COUNTERS_FILE = os.path.join("profiles", "k_pass_counters.json")
+
+
+def pass_counters():
+    """Measured HBM traffic of the fused pass relative to its algorithmic bytes (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE, calibrated on a plain copy), as committed with the profiles; None if absent."""
+    try:
+        with open(os.path.join(ROOT, COUNTERS_FILE)) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
 def algorithmic_bytes_per_pass(mask, rsize=4):
     n_all = mask.size
     n_fluid = int(np.sum(mask != 1))
     return 19 * rsize * 2 * (n_fluid + n_all)
 
 
-def load_design(batch, d, Pr=7.0, k_ratio=628.0, T_in=25.0):
+def load_design(batch, d, Pr=7.0, k_ratio=628.0, T_in=25.0, slot=0):
     from xadvcool_amd.engine import normalized_heat, thermal_taus
     ttf, tts = thermal_taus(d["tau_fluid"], Pr, k_ratio)
-    batch.load(0, d["mask"], heat=normalized_heat(d["heat"], d["mask"].shape), tau=d["tau_fluid"],
+    batch.load(slot, d["mask"], heat=normalized_heat(d["heat"], d["mask"].shape), tau=d["tau_fluid"],
                tau_th_fluid=ttf, tau_th_solid=tts, heat_power=d["heat_power"], T_in=T_in, rho_out=1.0,
                u_in=d["u_inlet_val"])
-    batch.init(0)
+    batch.init(slot)
 
 
 def cpu_baseline(design, seconds, threads):
@@ -111,27 +124,31 @@ def cpu_baseline(design, seconds, threads):
     return mask.size * steps / dt / 1e6, steps, dt
 
 
-def profile_passes(npasses, seed=1000):
-    """Only fused passes on one config-2 design (graph chunks of 100), for kernel traces and counters."""
+def profile_passes(npasses, slots=1, seed=1000):
+    """Only fused passes on ``slots`` config-2 designs (seeds from ``seed``; graph chunks of 100, fewer when
+    npasses is smaller), for kernel traces and counters.  One slot (~140 MB touched per pass) stays in the
+    256 MB Infinity Cache; many slots are HBM-resident."""
     import torch
     from xadvcool_amd.engine import LatticeBatch
     from xadvcool_amd.geometry import config2_design
-    d = config2_design(seed)
-    batch = LatticeBatch(slots=1, cap_shape=LATTICE, graph_chunk=100)
-    load_design(batch, d)
+    designs = [config2_design(seed + i) for i in range(slots)]
+    chunk = 100 if npasses >= 100 else max(2, npasses - npasses % 2)
+    batch = LatticeBatch(slots=slots, cap_shape=LATTICE, graph_chunk=chunk)
+    for s, d in enumerate(designs):
+        load_design(batch, d, slot=s)
     batch.step(200, want_fields=False)  # past the start-up transient
     torch.cuda.synchronize()
     timer = PassTimer(batch.stream())
     done = 0
     while done < npasses:
-        n = min(100, npasses - done)
+        n = min(chunk, npasses - done)
         timer.begin()
         batch.step(n, want_fields=False)
         timer.end(n)
         done += n
     ms = timer.total_ms() / max(done, 1)
-    nb = algorithmic_bytes_per_pass(d["mask"])
-    print(json.dumps({"mode": "passes", "passes": done, "avg_pass_us": round(1000 * ms, 2),
+    nb = sum(algorithmic_bytes_per_pass(d["mask"]) for d in designs)
+    print(json.dumps({"mode": "passes", "slots": slots, "passes": done, "avg_pass_us": round(1000 * ms, 2),
                       "alg_bytes_per_pass": nb, "achieved_GBs": round(nb / (ms * 1e-3) / 1e9, 1)}), flush=True)
     batch.close()
 
@@ -223,12 +240,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["solves", "passes", "copy", "batch", "config1"], default="solves",
-                    help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy)")
+    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1"], default="solves",
+                    help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy); "
+                         "ceilings: bandwidth of plain copies and of the pass's access shapes")
     ap.add_argument("--passes", type=int, default=400)
     ap.add_argument("--copy-mb", type=float, default=512.0)
     ap.add_argument("--designs", type=int, default=64)
-    ap.add_argument("--slots", type=int, default=64)
+    ap.add_argument("--slots", type=int, default=None, help="batch slots (default: 1 for passes, 64 for batch)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
     args = ap.parse_args()
 
@@ -238,13 +256,19 @@ def main():
         sys.exit(2)
 
     if args.mode == "passes":
-        return profile_passes(args.passes)
+        return profile_passes(args.passes, args.slots or 1)
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
+    if args.mode == "ceilings":
+        import subprocess
+        exe = os.path.join(ROOT, "xadvcool_amd", "bw_ceiling")
+        if not os.path.exists(exe):
+            raise SystemExit(f"{exe} is missing: run __graft_entry__.build() first")
+        sys.exit(subprocess.run([exe]).returncode)
     if args.mode == "config1":
         return config1_bench()
     if args.mode == "batch":
-        print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots, args.max_steps, args.tol)}),
+        print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots or 64, args.max_steps, args.tol)}),
               flush=True)
         return
 
@@ -258,7 +282,11 @@ def main():
     seeds = [1000 + i for i in range(args.steps)]
     timed = designs(seeds)
     net = S.UNet3D.load_file(args.weights)
-    kw = dict(slots=1, tol=args.tol, max_steps=args.max_steps)
+    # Designs are solved three at a time (continuous batching): three 148x128x32 lattices per launch is the
+    # measured best per-design pass time (partly Infinity-Cache resident; one slot leaves the chip underfed,
+    # four fall out of the cache).
+    nslots = args.slots or 3
+    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps)
     if args.warmup:
         w = designs(range(900, 900 + args.warmup))
         W.solve_designs(w, None, **kw)
@@ -285,6 +313,7 @@ def main():
     pass_ms, npass, alg_bytes = timer.totals()
     avg_pass_ms = pass_ms / max(npass, 1)
     achieved = alg_bytes / (pass_ms * 1e-3) / 1e9
+    counters = pass_counters()
 
     line = {
         "metric": "converged solves/hr, warm-start vs plain LBM; MLUPS and % of HBM peak",
@@ -315,12 +344,19 @@ def main():
         "warm_vs_cold": round(w_rate / c_rate, 3) if c_rate > 0 else None,
         "mlups": round(cells * (sum(c_steps) + sum(w_steps)) / (cold["wall"] + warm["wall"]) / 1e6, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": (int(counters["traffic_per_alg_byte"] * alg_bytes / max(npass, 1))
+                                 if counters else None),
                      "kernel": "k_pass (fused AA fluid+thermal pass)",
                      "avg_launch_us": round(1000.0 * avg_pass_ms, 2),
                      "alg_bytes_per_launch": int(alg_bytes / max(npass, 1)),
-                     "note": "one 148x128x32 instance (~95 MB of AA state) fits the 256 MB Infinity Cache: "
-                             "LLC-resident, not a pure-HBM figure"},
+                     "traffic_source": (f"rocprofv3 FETCH_SIZE x{counters['fetch_scale']} + WRITE_SIZE "
+                                        f"x{counters['write_scale']} per pass / algorithmic bytes = "
+                                        f"{counters['traffic_per_alg_byte']:.3f} on {counters['workload']} "
+                                        f"({COUNTERS_FILE})" if counters else None),
+                     "note": f"{nslots} lattices per launch (~140 MB touched per lattice per pass): partly "
+                             "resident in the 256 MB Infinity Cache; the all-HBM figure is bench.py --mode "
+                             "passes --slots 64"},
     }
     if not args.no_cpu:
         mlups1, n1, t1 = cpu_baseline(timed[0], args.cpu_seconds, 1)

@@ -14,6 +14,8 @@ SRC = os.path.join(HERE, "csrc", "xadv.hip")
 LIB = os.path.join(HERE, "libxadv.so")
 NN_SRC = os.path.join(HERE, "csrc", "nn_kernels.hip")
 NN_LIB = os.path.join(HERE, "libxadv_nn.so")
+CEIL_SRC = os.path.join(HERE, "csrc", "bw_ceiling.hip")
+CEIL_EXE = os.path.join(HERE, "bw_ceiling")  # bandwidth-ceiling probe run by bench.py --mode ceilings
 ARCH = os.environ.get("XADV_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: every + and * rounds separately, in source order, so the
@@ -27,10 +29,11 @@ HIPCC_FLAGS = [
 ]
 
 
-def _hipcc(src, lib, verbose):
+def _hipcc(src, lib, verbose, shared=True):
     hipcc = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
     tmp = lib + ".tmp"
-    cmd = [hipcc, *HIPCC_FLAGS, "-o", tmp, src]
+    flags = HIPCC_FLAGS if shared else [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")]
+    cmd = [hipcc, *flags, "-o", tmp, src]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -44,6 +47,7 @@ def _hipcc(src, lib, verbose):
 def build(verbose: bool = False) -> str:
     """Build the solver library and the surrogate library; returns the solver library's path."""
     _hipcc(NN_SRC, NN_LIB, verbose)
+    _hipcc(CEIL_SRC, CEIL_EXE, verbose, shared=False)
     return _hipcc(SRC, LIB, verbose)
 
 
