@@ -55,60 +55,96 @@ enum : uint32_t {
 // for a non-solid cell (nz <= 32); the number of a non-solid (x, y, z) is
 // start + popcount(bits below z).  Neighbour numbers therefore come from the
 // 3 x 3 columns around a cell, without a per-cell neighbour table.
+//
+// The column table is padded with ny + 1 empty columns on each side, so the
+// eight neighbour columns of any column are inside the table (those past the
+// lattice's x ends are the padding; their numbers are never used, because the
+// neighbours there lie outside the lattice and bounce back).
 struct ColInfo {
   uint32_t start, bits;
 };
 constexpr int kCompactMaxNz = 32;
 
-__host__ __device__ inline uint32_t bits_below(int z) {
-  return z <= 0 ? 0u : (z >= 32 ? 0xffffffffu : ((1u << z) - 1u));
-}
-__host__ __device__ inline uint32_t compact_index(ColInfo c, int z) {
-  return c.start + (uint32_t)__builtin_popcount(c.bits & bits_below(z));
-}
+// Entries of the padded table of an nx x ny lattice.
+__host__ __device__ inline int64_t padded_columns(int nx, int ny) { return (int64_t)(nx + 2) * ny + 2; }
 
-// The 3 x 3 columns around column q (col[dx + 1][dy + 1] is column
-// q + dx * ny + dy); indices off the column range are clamped, since the
-// columns there are never used (those neighbours lie outside the lattice).
-__host__ __device__ inline void around_columns(const ColInfo* cols, int q, int ny, int ncol, ColInfo (&col)[3][3]) {
-#pragma unroll
-  for (int dx = -1; dx <= 1; ++dx)
-#pragma unroll
-    for (int dy = -1; dy <= 1; ++dy) {
-      int t = q + dx * ny + dy;
-      t = t < 0 ? 0 : (t >= ncol ? ncol - 1 : t);
-      col[dx + 1][dy + 1] = cols[t];
-    }
-}
-
-// Number of the non-solid cell at offset (DX, DY, DZ) from non-solid cell
-// number j at height z.  Meaningful only where that neighbour is inside the
-// lattice and non-solid; callers discard it elsewhere.
-template <int DX, int DY, int DZ>
-__host__ __device__ inline uint32_t nb_number(const ColInfo (&col)[3][3], uint32_t j, int z) {
-  if (DX == 0 && DY == 0) return DZ > 0 ? j + 1u : j - 1u;  // same column: next / previous non-solid cell
-  return compact_index(col[DX + 1][DY + 1], z + DZ);
-}
-
-// Column table of a mask (int8 codes, C order (nx, ny, nz), nz <= 32) into
-// cols[nx * ny]; returns the number of non-solid cells.
+// Padded column table of a mask (int8 codes, C order (nx, ny, nz), nz <= 32)
+// into cols[padded_columns(nx, ny)]; returns the number of non-solid cells.
+// Column q of the lattice is entry ny + 1 + q.
 inline int64_t build_columns(const int8_t* mask, int nx, int ny, int nz, ColInfo* cols) {
   int64_t n = 0;
-  const int64_t ncol = (int64_t)nx * ny;
+  const int64_t ncol = (int64_t)nx * ny, pad = ny + 1;
+  for (int64_t k = 0; k < padded_columns(nx, ny); ++k) cols[k] = ColInfo{0u, 0u};
   for (int64_t q = 0; q < ncol; ++q) {
     uint32_t bits = 0;
     for (int z = 0; z < nz; ++z)
       if (mask[q * nz + z] != BC_SOLID) bits |= 1u << z;
-    cols[q].start = (uint32_t)n;
-    cols[q].bits = bits;
+    cols[pad + q].start = (uint32_t)n;
+    cols[pad + q].bits = bits;
     n += __builtin_popcount(bits);
   }
   return n;
 }
 
-// J[k] = number of the cell at c + e_k, k = 1..18.
-__host__ __device__ inline void neighbor_numbers(const ColInfo (&col)[3][3], uint32_t j, int z, uint32_t* J) {
-#define XADV_NB(k) J[k] = nb_number<ex_of(k), ey_of(k), ez_of(k)>(col, j, z);
+// q = c / nz for c < 2^32 / nz (so for every cell of a slot: fewer than 2^26
+// cells): floor(c * m / 2^32) with m = floor((2^32 - 1) / nz) + 1.  Writing
+// m = 2^32 / nz + e with 0 <= e < 1, c * m / 2^32 = c / nz + c * e / 2^32, and
+// c * e / 2^32 < 1 / nz keeps the floor at c / nz.  nz = 1 is passed through.
+__host__ __device__ inline uint32_t nz_magic(uint32_t nz) {
+  return nz <= 1 ? 0u : (uint32_t)(0xffffffffull / nz + 1ull);
+}
+__host__ __device__ inline uint32_t div_nz(uint32_t c, uint32_t nz, uint32_t magic) {
+  return nz <= 1 ? c : (uint32_t)(((uint64_t)c * magic) >> 32);
+}
+
+// Column entry q + rel of a padded table (cols points at lattice column 0).
+__host__ __device__ inline ColInfo load_col(const ColInfo* cols, int rel, uint32_t q) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  // Uniform base, 32-bit per-lane offset: one global load with an SGPR base.
+  typedef __attribute__((address_space(1))) const char gchar;
+  typedef __attribute__((address_space(1))) const ColInfo gcol;
+  gchar* base = (gchar*)(cols + rel);
+  return *(gcol*)(base + q * (uint32_t)sizeof(ColInfo));
+#else
+  return cols[rel + (int64_t)q];
+#endif
+}
+
+// Byte offsets of one non-solid cell for the compact fluid planes (RS = bytes
+// per value, NFB = plane stride in bytes):
+//   j4      the cell's own number x RS;
+//   JB[k]   the number of the neighbour at c + e_k x RS, plus NFB for even k.
+// With the planes of a direction pair addressed from the lower plane
+// min(i, opp(i)), every odd-pass access of direction i is one of two such
+// offsets (see lbm_kernels.h): the neighbour's slot or the cell's own.
+// JB[k] is meaningful only where that neighbour is inside the lattice and
+// non-solid.
+template <uint32_t RS>
+__host__ __device__ inline uint32_t own_offset(const ColInfo* cols, uint32_t q, int z) {
+  const ColInfo own = load_col(cols, 0, q);
+  return (own.start + (uint32_t)__builtin_popcount(own.bits & ((1u << z) - 1u))) * RS;
+}
+
+template <uint32_t RS>
+__host__ __device__ inline void neighbour_offsets(const ColInfo* cols, uint32_t q, int z, int ny, uint32_t NFB,
+                                                  uint32_t& j4, uint32_t* JB) {
+  const uint32_t m0 = (1u << z) - 1u;  // z <= 31: bits below z
+  const uint32_t mp = (m0 << 1) | 1u;  // bits below z + 1
+  const uint32_t mm = m0 >> 1;         // bits below z - 1
+  const ColInfo own = load_col(cols, 0, q);
+  j4 = (own.start + (uint32_t)__builtin_popcount(own.bits & m0)) * RS;
+#define XADV_NB(k)                                                                                  \
+  {                                                                                                 \
+    constexpr int dx = ex_of(k), dy = ey_of(k), dz = ez_of(k);                                      \
+    constexpr uint32_t even = (k & 1) ? 0u : 1u;                                                    \
+    if (dx == 0 && dy == 0) {                                                                       \
+      JB[k] = (dz > 0 ? j4 + RS : j4 - RS) + (even ? NFB : 0u);                                     \
+    } else {                                                                                        \
+      const ColInfo cn = load_col(cols, dx * ny + dy, q);                                           \
+      const uint32_t m = dz > 0 ? mp : (dz < 0 ? mm : m0);                                          \
+      JB[k] = (cn.start + (uint32_t)__builtin_popcount(cn.bits & m)) * RS + (even ? NFB : 0u);      \
+    }                                                                                               \
+  }
   XADV_NB(1) XADV_NB(2) XADV_NB(3) XADV_NB(4) XADV_NB(5) XADV_NB(6) XADV_NB(7) XADV_NB(8) XADV_NB(9)
   XADV_NB(10) XADV_NB(11) XADV_NB(12) XADV_NB(13) XADV_NB(14) XADV_NB(15) XADV_NB(16) XADV_NB(17) XADV_NB(18)
 #undef XADV_NB

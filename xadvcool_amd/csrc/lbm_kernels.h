@@ -54,10 +54,11 @@ struct Slot {
   R omega, rho_out, u_in, force_x, u_wall;
   R omega_tf, omega_ts, sf_tf, sf_ts;  // thermal 1/tau and (1 - 0.5*omega): fluid, solid
   R T_in, dir_T;
-  uint32_t nf;  // plane stride of f: npad (dense) or the padded non-solid count (compact)
+  uint32_t nf;        // plane stride of f: npad (dense) or the padded non-solid count (compact)
+  uint32_t nz_magic;  // c / nz = div_nz(c, nz, nz_magic) (compact storage)
   R* f;
   R* g;
-  const ColInfo* cols;  // compact fluid storage: one entry per (x, y) column, else null
+  const ColInfo* cols;  // compact fluid storage: padded column table at lattice column 0, else null
   const uint32_t* flags;
   const R* q;
   R* fin;    // [19][P]   INLET own post-collision set
@@ -189,7 +190,8 @@ struct Cell {
   uint32_t fl;  // flag word
   int type;
   int x, y, z;  // coordinates (general mode only)
-  uint32_t j;   // compact fluid storage: the cell's number among non-solid cells
+  uint32_t j4;  // compact fluid storage: the cell's number among non-solid cells, in bytes
+  uint32_t jp;  // j4 + one fluid plane in bytes (odd passes)
   bool valid;
 };
 
@@ -253,46 +255,51 @@ __device__ __forceinline__ void fluid_store(R* __restrict__ A, uint32_t N, const
 // writes a slot of a SOLID cell (those accesses are the bounce-back redirects
 // to the cell's own slots), so dropping the SOLID cells changes no value.
 
-// The cell's own number and, for odd passes, the numbers of its 18
-// neighbours: J[k] is the number of the cell at c + e_k.
+// The cell's own byte offset and, for odd passes, the offsets of its 18
+// neighbours (JB[k]: neighbour c + e_k, plus one plane for even k).
 template <class R, int NODD>
-__device__ __forceinline__ void compact_numbers(const Slot<R>& S, Cell& k, uint32_t* J) {
+__device__ __forceinline__ void compact_offsets(const Slot<R>& S, Cell& k, uint32_t* JB) {
   const uint32_t nz = (uint32_t)S.nz;
-  const uint32_t q = k.c / nz;
+  const uint32_t q = div_nz(k.c, nz, S.nz_magic);
   const int z = (int)(k.c - q * nz);
   if (!NODD) {
-    k.j = compact_index(S.cols[q], z);
+    k.j4 = own_offset<sizeof(R)>(S.cols, q, z);
     return;
   }
-  ColInfo col[3][3];
-  around_columns(S.cols, (int)q, S.ny, S.nx * S.ny, col);
-  k.j = compact_index(col[1][1], z);
-  neighbor_numbers(col, k.j, z, J);
+  const uint32_t NFB = S.nf * (uint32_t)sizeof(R);
+  neighbour_offsets<sizeof(R)>(S.cols, q, z, S.ny, NFB, k.j4, JB);
+  k.jp = k.j4 + NFB;
 }
 
+// Odd passes address the two planes of a direction pair from the lower one,
+// lo = min(i, opp(i)): slot (i, own) is j4 (i odd) or jp (i even) from plane
+// lo, slot (opp(i), neighbour c - e_i) is JB[opp(i)].  One bit-extract and
+// one bitfield insert per access.
 template <class R, int NODD, int i>
-__device__ __forceinline__ R fluid_load_c(const R* __restrict__ A, uint32_t NF, const Cell& k, const uint32_t* J) {
+__device__ __forceinline__ R fluid_load_c(const R* __restrict__ A, uint32_t NF, const Cell& k, const uint32_t* JB) {
   constexpr uint32_t rs = sizeof(R);
-  if (NODD == 0 || i == 0) return ldb<R>(A, ((uint32_t)i * NF + k.j) * rs);
+  const char* Ab = reinterpret_cast<const char*>(A);
+  if (NODD == 0 || i == 0) return ldb<R>(Ab + (size_t)i * NF * rs, k.j4);
   // Pull from the neighbour's opposite slot, or (bounce-back) the own slot i.
-  const uint32_t own = (uint32_t)i * NF + k.j;
-  const uint32_t nb = (uint32_t)opp_of(i) * NF + J[opp_of(i)];
+  constexpr int lo = i < opp_of(i) ? i : opp_of(i);
+  const uint32_t own = (i & 1) ? k.j4 : k.jp;
   const uint32_t bb = bitmask(k.fl, F_BOUNCE0 + i);
-  return ldb<R>(A, ((bb & own) | (~bb & nb)) * rs);
+  return ldb<R>(Ab + (size_t)lo * NF * rs, (bb & own) | (~bb & JB[opp_of(i)]));
 }
 
 template <class R, int NODD, int i>
-__device__ __forceinline__ void fluid_store_c(R* __restrict__ A, uint32_t NF, const Cell& k, const uint32_t* J, R v) {
+__device__ __forceinline__ void fluid_store_c(R* __restrict__ A, uint32_t NF, const Cell& k, const uint32_t* JB, R v) {
   constexpr uint32_t rs = sizeof(R);
+  char* Ab = reinterpret_cast<char*>(A);
   if (NODD == 0 || i == 0) {
-    stb<R>(A, ((uint32_t)opp_of(i) * NF + k.j) * rs, v);
+    stb<R>(Ab + (size_t)opp_of(i) * NF * rs, k.j4, v);
     return;
   }
   // Push into the neighbour's slot i, or (target outside / SOLID) the own slot opp(i).
-  const uint32_t own = (uint32_t)opp_of(i) * NF + k.j;
-  const uint32_t nb = (uint32_t)i * NF + J[i];
+  constexpr int lo = i < opp_of(i) ? i : opp_of(i);
+  const uint32_t own = (i & 1) ? k.jp : k.j4;
   const uint32_t bb = bitmask(k.fl, F_BOUNCE0 + opp_of(i));
-  stb<R>(A, ((bb & own) | (~bb & nb)) * rs, v);
+  stb<R>(Ab + (size_t)lo * NF * rs, (bb & own) | (~bb & JB[i]), v);
 }
 
 // Moments of a SOLID cell's frozen populations (src/fluid_engine3d.py:92-113,
@@ -438,6 +445,34 @@ __device__ __forceinline__ void fluid_scatter(const Slot<R>& S, const Cell& k, c
 
 // ---------------------------------------------------------------- thermal ---
 
+// All-ones words for the cell's face flags, in the order x = 0, x = nx-1,
+// y = 0, y = ny-1, z = 0, z = nz-1 (one bit-extract each).
+struct Faces {
+  uint32_t m[6];
+};
+__device__ __forceinline__ Faces faces_of(uint32_t fl) {
+  Faces f;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) f.m[a] = bitmask(fl, 21 + a);  // F_FX0 .. F_FZN are bits 21..26
+  return f;
+}
+static_assert(F_FX0 == 1u << 21 && F_FXN == 1u << 22 && F_FY0 == 1u << 23 && F_FYN == 1u << 24 &&
+                  F_FZ0 == 1u << 25 && F_FZN == 1u << 26,
+              "face bits");
+// All-ones where the source c - e_i of direction i lies outside a
+// non-periodic axis (the same test as src_face_mask(i), from the face words).
+template <int i>
+__device__ __forceinline__ uint32_t src_outside(const Faces& f) {
+  uint32_t r = 0;
+  if (ex_of(i) == 1) r |= f.m[0];
+  if (ex_of(i) == -1) r |= f.m[1];
+  if (ey_of(i) == 1) r |= f.m[2];
+  if (ey_of(i) == -1) r |= f.m[3];
+  if (ez_of(i) == 1) r |= f.m[4];
+  if (ez_of(i) == -1) r |= f.m[5];
+  return r;
+}
+
 // Streamed thermal populations of cell k for the pass of parity NODD (the
 // thermal half uses the opposite AA parity: TODD = !NODD).
 template <class R, int GEN, int NODD>
@@ -457,13 +492,14 @@ __device__ __forceinline__ void thermal_gather(const Slot<R>& S, const Cell& k, 
   } else {
     const uint32_t fl = k.fl;
     constexpr uint32_t rs = sizeof(R);
+    const Faces fs = faces_of(fl);
 #define XADV_TG(i)                                                                                          \
   {                                                                                                         \
     if (TODD == 0 || i == 0) {                                                                              \
       g[i] = ldb<R>(reinterpret_cast<const char*>(B) + (size_t)i * N * rs, k.c * rs);                        \
     } else if (!GEN) {                                                                                      \
       const int64_t pull = (int64_t)opp_of(i) * N - nb_off<i>(P, nz);                                       \
-      g[i] = ld2<R, (i > opp_of(i))>(B, pull, (int64_t)i * N, k.c, anymask(fl, src_face_mask(i)));          \
+      g[i] = ld2<R, (i > opp_of(i))>(B, pull, (int64_t)i * N, k.c, src_outside<i>(fs));                    \
     } else {                                                                                                \
       const bool oob = (fl & src_face_mask(i)) != 0;                                                        \
       const uint32_t src = nb_wrap<i>(k.x, k.y, k.z, -1, S.nx, S.ny, nz);                                   \
@@ -503,8 +539,8 @@ __device__ __forceinline__ void thermal_gather(const Slot<R>& S, const Cell& k, 
 }
 
 template <class R, int GEN, int NODD, int i>
-__device__ __forceinline__ void thermal_store(R* __restrict__ B, uint32_t N, const Cell& k, int P, int nz, int nx,
-                                              int ny, R v) {
+__device__ __forceinline__ void thermal_store(R* __restrict__ B, uint32_t N, const Cell& k, const Faces& fs, int P,
+                                              int nz, int nx, int ny, R v) {
   constexpr int TODD = 1 - NODD;
   constexpr uint32_t rs = sizeof(R);
   if (TODD == 0 || i == 0) {
@@ -514,7 +550,7 @@ __device__ __forceinline__ void thermal_store(R* __restrict__ B, uint32_t N, con
   // Target c + e_i outside a non-periodic axis = source of opp(i) outside.
   if (!GEN) {
     const int64_t push = (int64_t)i * N + nb_off<i>(P, nz);
-    st2<R, (opp_of(i) > i)>(B, push, (int64_t)opp_of(i) * N, k.c, anymask(k.fl, src_face_mask(opp_of(i))), v);
+    st2<R, (opp_of(i) > i)>(B, push, (int64_t)opp_of(i) * N, k.c, src_outside<opp_of(i)>(fs), v);
     return;
   }
   const bool oob = (k.fl & src_face_mask(opp_of(i))) != 0;
@@ -529,18 +565,21 @@ __device__ __forceinline__ void thermal_collide(const Slot<R>& S, const Cell& k,
   if (solid) ux = uy = uz = 0;  // thermal_engine3d.py:123-125
   const R om = solid ? S.omega_ts : S.omega_tf;
   const R sf = solid ? S.sf_ts : S.sf_tf;
-  const R q = (k.fl & F_HEAT) ? ld(S.q, k.c) : (R)0;
   const R usq = ux * ux + uy * uy + uz * uz;
-#define XADV_TC(i)                                               \
-  {                                                              \
-    R v = g[i] - om * (g[i] - feq_i<R, i>(Tl, ux, uy, uz, usq)); \
-    if (q > (R)0.0) v += (R)w_of(i) * sf * q;                    \
-    g[i] = v;                                                    \
-  }
+#define XADV_TC(i) g[i] = g[i] - om * (g[i] - feq_i<R, i>(Tl, ux, uy, uz, usq));
   XADV_TC(0) XADV_TC(1) XADV_TC(2) XADV_TC(3) XADV_TC(4) XADV_TC(5) XADV_TC(6) XADV_TC(7) XADV_TC(8)
   XADV_TC(9) XADV_TC(10) XADV_TC(11) XADV_TC(12) XADV_TC(13) XADV_TC(14) XADV_TC(15) XADV_TC(16)
   XADV_TC(17) XADV_TC(18)
 #undef XADV_TC
+  // The source term, added to each population after its BGK update
+  // (thermal_engine3d.py:141-143); cells without heat skip the whole block.
+  if (k.fl & F_HEAT) {
+    const R q = ld(S.q, k.c);
+    if (q > (R)0.0) {
+#pragma unroll
+      for (int i = 0; i < 19; ++i) g[i] += (R)w_of(i) * sf * q;
+    }
+  }
 }
 
 template <class R, int GEN, int NODD>
@@ -548,7 +587,8 @@ __device__ __forceinline__ void thermal_scatter(const Slot<R>& S, const Cell& k,
   R* __restrict__ B = S.g;
   const uint32_t N = (uint32_t)S.npad;
   const int P = S.P, nz = S.nz, nx = S.nx, ny = S.ny;
-#define XADV_TS(i) thermal_store<R, GEN, NODD, i>(B, N, k, P, nz, nx, ny, g[i]);
+  const Faces fs = faces_of(k.fl);
+#define XADV_TS(i) thermal_store<R, GEN, NODD, i>(B, N, k, fs, P, nz, nx, ny, g[i]);
   XADV_TS(0) XADV_TS(1) XADV_TS(2) XADV_TS(3) XADV_TS(4) XADV_TS(5) XADV_TS(6) XADV_TS(7) XADV_TS(8)
   XADV_TS(9) XADV_TS(10) XADV_TS(11) XADV_TS(12) XADV_TS(13) XADV_TS(14) XADV_TS(15) XADV_TS(16)
   XADV_TS(17) XADV_TS(18)
@@ -644,7 +684,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
     q.fl = q.valid ? ld(S.flags, q.c) : (uint32_t)BC_SOLID;
     q.type = (int)(q.fl & F_TYPE);
     q.x = q.y = q.z = 0;
-    q.j = 0;
+    q.j4 = q.jp = 0;
     if (GEN && q.valid) coords(q.c, S.P, S.nz, q.x, q.y, q.z);
   }
 
@@ -660,7 +700,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
       if (cell[k].valid && cell[k].type != BC_SOLID) {
-        if (CMP) compact_numbers<R, NODD>(S, cell[k], J[k]);
+        if (CMP) compact_offsets<R, NODD>(S, cell[k], J[k]);
         fluid_gather<R, GEN, NODD, CMP>(S, cell[k], J[k], f[k]);
       }
     if (PRE) thermal_load<R, GEN, NODD, CPT>(S, cell, raw, g, Tl);
@@ -761,12 +801,12 @@ __global__ __launch_bounds__(256) void k_init(Slot<R>* __restrict__ slots, int s
 #pragma unroll
       for (int i = 0; i < 19; ++i) f[i] = f[i] - om * (f[i] - f[i]);
       const uint32_t nz = (uint32_t)S.nz;
-      const uint32_t q = (uint32_t)c / nz;
-      const uint32_t j = compact_index(S.cols[q], (int)((uint32_t)c - q * nz));
-      R* __restrict__ A = S.f;
-      const int64_t NF = S.nf;
+      const uint32_t q = div_nz((uint32_t)c, nz, S.nz_magic);
+      const uint32_t j4 = own_offset<sizeof(R)>(S.cols, q, (int)((uint32_t)c - q * nz));
+      char* __restrict__ A = reinterpret_cast<char*>(S.f);
+      const size_t NFB = (size_t)S.nf * sizeof(R);
 #pragma unroll
-      for (int i = 0; i < 19; ++i) A[(int64_t)opp_of(i) * NF + j] = f[i];
+      for (int i = 0; i < 19; ++i) *reinterpret_cast<R*>(A + (size_t)opp_of(i) * NFB + j4) = f[i];
       if (type == BC_INLET) {
 #pragma unroll
         for (int i = 0; i < 19; ++i) S.fin[(int64_t)i * P + c] = f[i];

@@ -50,6 +50,7 @@ struct xadv_batch {
   void* slots_host = nullptr;  // pinned copy of the slot table
   size_t slot_bytes = 0;
   int compact = 0;  // fluid lattice stored for non-solid cells only
+  int64_t cols_per_slot = 0;  // padded column-table entries per slot
   // arenas (bytes)
   char *f = nullptr, *g = nullptr, *flags = nullptr, *q = nullptr, *fin = nullptr, *fsnap = nullptr, *gsnap = nullptr,
        *gxh = nullptr, *rho = nullptr, *u = nullptr, *T = nullptr, *cols = nullptr;
@@ -249,7 +250,7 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
     if (d->nz > xadv::kCompactMaxNz)
       return fail(XADV_E_ARG, "this batch stores the fluid lattice compactly, which needs nz <= 32; create the "
                               "batch with fluid_storage = XADV_STORAGE_DENSE (or cap_nz > 32) for taller lattices");
-    cols.resize((size_t)d->nx * d->ny);
+    cols.resize((size_t)xadv::padded_columns(d->nx, d->ny));
     nfluid = xadv::build_columns(d->mask, d->nx, d->ny, d->nz, cols.data());
   }
   std::vector<R> q(n, (R)0);
@@ -286,10 +287,13 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
   S.T_in = (R)d->T_in;
   S.dir_T = (R)d->dirichlet_T;
   S.nf = (uint32_t)(b->compact ? round_up(nfluid, 64) : S.npad);
+  S.nz_magic = xadv::nz_magic((uint32_t)d->nz);
   S.f = reinterpret_cast<R*>(b->f + (size_t)s * 19 * capn * rs);
   S.g = reinterpret_cast<R*>(b->g + (size_t)s * 19 * capn * rs);
-  S.cols = b->compact ? reinterpret_cast<const xadv::ColInfo*>(b->cols + (size_t)s * cap * sizeof(xadv::ColInfo))
-                      : nullptr;
+  // The slot's padded column table; S.cols points at lattice column 0 (entry ny + 1).
+  xadv::ColInfo* col_arena =
+      b->compact ? reinterpret_cast<xadv::ColInfo*>(b->cols) + (size_t)s * b->cols_per_slot : nullptr;
+  S.cols = b->compact ? col_arena + d->ny + 1 : nullptr;
   S.flags = reinterpret_cast<const uint32_t*>(b->flags + (size_t)s * cap * 4);
   S.q = reinterpret_cast<const R*>(b->q + (size_t)s * cap * rs);
   S.fin = reinterpret_cast<R*>(b->fin + (size_t)s * 19 * capP * rs);
@@ -302,7 +306,7 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
   HIPCHK(hipMemcpyAsync((void*)S.flags, fl.data(), n * 4, hipMemcpyHostToDevice, b->stream));
   HIPCHK(hipMemcpyAsync((void*)S.q, q.data(), n * rs, hipMemcpyHostToDevice, b->stream));
   if (b->compact)
-    HIPCHK(hipMemcpyAsync((void*)S.cols, cols.data(), cols.size() * sizeof(xadv::ColInfo), hipMemcpyHostToDevice,
+    HIPCHK(hipMemcpyAsync(col_arena, cols.data(), cols.size() * sizeof(xadv::ColInfo), hipMemcpyHostToDevice,
                           b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
   b->loaded[s] = 1;
@@ -387,6 +391,9 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   b->cap_cells = (int64_t)d.cap_nx * d.cap_ny * d.cap_nz;
   b->cap_npad = round_up(b->cap_cells, 64);
   b->cap_P = d.cap_ny * d.cap_nz;
+  // A slot's lattice has nx * ny <= cap_cells columns and ny <= cap_P, so its
+  // padded table, (nx + 2) * ny + 2 entries, fits in cap_cells + 2 * cap_P + 2.
+  b->cols_per_slot = b->cap_cells + 2 * (int64_t)b->cap_P + 2;
   // Every lattice array of a slot is addressed with 32-bit byte offsets.
   if ((double)19 * round_up(b->cap_cells, 64) * b->rsize >= 4294967296.0)
     return cleanup_early(b, fail(XADV_E_ARG, "slot capacity too large for 32-bit addressing (max ~56M cells FP32)"));
@@ -425,7 +432,7 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
       {&b->rho, S * cap * rs},
       {&b->u, S * 3 * cap * rs},
       {&b->T, S * cap * rs},
-      {&b->cols, b->compact ? S * cap * sizeof(xadv::ColInfo) : 0},  // nx * ny <= cells per slot
+      {&b->cols, b->compact ? S * (size_t)b->cols_per_slot * sizeof(xadv::ColInfo) : 0},
   };
   for (auto& a : allocs) {
     if (!a.bytes) continue;
