@@ -286,7 +286,8 @@ def main():
     # measured best per-design pass time (partly Infinity-Cache resident; one slot leaves the chip underfed,
     # four fall out of the cache).
     nslots = args.slots or 3
-    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps)
+    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps,
+              log=lambda m: print(m, file=sys.stderr, flush=True))  # progress to stderr; stdout is the result line
     if args.warmup:
         w = designs(range(900, 900 + args.warmup))
         W.solve_designs(w, None, **kw)

@@ -8,10 +8,10 @@ import pytest
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def declared_symbols():
-    src = open(os.path.join(ROOT, "include", "xadv.h")).read()
+def declared_symbols(header="xadv.h", prefix="xadv_"):
+    src = open(os.path.join(ROOT, "include", header)).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"\b(xadv_[a-z_0-9]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(" + prefix + r"[a-z_0-9]+)\s*\(", src)))
 
 
 def test_header_declares_the_api():
@@ -28,6 +28,18 @@ def test_library_exports_every_declared_symbol():
     missing = [s for s in declared_symbols() if not hasattr(lib, s)]
     assert not missing, f"missing exports: {missing}"
     assert set(_lib.EXPORTS) == set(declared_symbols())
+
+
+def test_surrogate_library_exports_every_declared_symbol():
+    """libxadv_nn.so (the warm-start surrogate's kernels) against include/xadv_nn.h."""
+    from xadvcool_amd import surrogate as S
+    if not os.path.exists(S.NN_LIB):
+        pytest.fail("libxadv_nn.so not built (run __graft_entry__.build())")
+    syms = declared_symbols("xadv_nn.h", "xnn_")
+    assert "xnn_conv_fwd" in syms and "xnn_conv_wgrad" in syms and "xnn_adam" in syms
+    lib = ctypes.CDLL(S.NN_LIB)
+    missing = [s for s in syms if not hasattr(lib, s)]
+    assert not missing, f"missing exports: {missing}"
 
 
 def test_version_and_error_calls_need_no_device():

@@ -69,6 +69,31 @@ def test_conv_fwd_dgrad_wgrad(gpu, cin, cout):
     assert torch.all(c.dw.cpu()[:, 27 * cin:] == 0)
 
 
+@pytest.mark.parametrize("shape,cin,cout", [((2, 5, 7, 32), 16, 16), ((2, 5, 7, 32), 32, 16),
+                                             ((2, 3, 6, 16), 16, 32), ((2, 3, 6, 16), 32, 32),
+                                             ((1, 4, 9, 16), 64, 32), ((2, 3, 5, 8), 32, 64),
+                                             ((1, 4, 4, 8), 64, 64)])
+def test_wgrad_tiled_shapes(gpu, shape, cin, cout):
+    """The z-column-tiled weight gradient (Z in {8, 16, 32}, the network's channel pairs at those levels),
+    with partial tiles at the x and y edges, against torch's conv3d weight and bias gradients."""
+    from xadvcool_amd import surrogate as S
+    print(f"[nn] tiled wgrad {shape} {cin}->{cout}", flush=True)
+    N, X, Y, Z = shape
+    gen = torch.Generator().manual_seed(cin + 3 * cout + Z)
+    c = S.Conv(cin, cout, gen)
+    x = torch.randn(*shape, cin, generator=gen)
+    g = torch.randn(*shape, cout, generator=gen)
+    xr = ncdhw(bf(x)).clone().requires_grad_(True)
+    wr = torch_weight(bf(c.w.cpu()), cin, cout).clone().requires_grad_(True)
+    br = torch.zeros(cout, requires_grad=True)
+    F.conv3d(xr, wr, bias=br, padding=1).backward(ncdhw(bf(g)))
+    for rep in range(2):  # accumulation across calls, partial buffers reused
+        c.wgrad(x.to(torch.bfloat16).cuda(), g.to(torch.bfloat16).cuda(), shape)
+    assert rel(torch_weight(c.dw.cpu(), cin, cout), 2 * wr.grad) < 1e-3
+    assert rel(c.db.cpu(), 2 * br.grad) < 1e-3
+    assert torch.all(c.dw.cpu()[:, 27 * cin:] == 0)
+
+
 def test_pool_up_silu(gpu):
     from xadvcool_amd import surrogate as S
     gen = torch.Generator().manual_seed(3)

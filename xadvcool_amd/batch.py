@@ -31,7 +31,7 @@ class BatchedSolver:
     def __init__(self, slots, cap_shape=(148, 128, 32), *, stats_every=100, tol_ke=1e-6, tol_T=1e-6,
                  max_steps=30000, min_steps_rule=False, Pr=7.0, k_ratio=628.0, t_inlet=25.0, precision=32,
                  block=256, cells_per_thread=1, xcd_remap=False, fluid_storage="auto", collect_fields=False,
-                 metrics=True, stall_check=None, confirm=1):
+                 metrics=True, stall_check=None, confirm=1, stop_rule="reference"):
         self.slots = int(slots)
         self.cap_shape = tuple(cap_shape)
         self.batch = LatticeBatch(slots=self.slots, cap_shape=self.cap_shape, precision=precision, block=block,
@@ -50,6 +50,9 @@ class BatchedSolver:
         # A slot converges when the reference's residual test passes on `confirm` consecutive checks
         # (1 = the reference's rule; more keeps a run from stopping where a transient's rate crosses zero).
         self.confirm = max(1, int(confirm))
+        # "reference": the reference's residuals; "rise": the same on the temperature rise and absorbed heat
+        # (see simulation._Residuals).
+        self.stop_rule = stop_rule
         self.state = [None] * self.slots
         self.lups = 0
         self.passes = 0
@@ -71,7 +74,8 @@ class BatchedSolver:
             self.batch.init(s)
         nx = mask.shape[0]
         min_steps = min_steps_floor(nx, self.max_steps, d["u_inlet_val"]) if self.min_steps_rule else 0
-        self.state[s] = dict(design=d, steps=0, res=_Residuals(self.every, self.tol_ke, self.tol_T, min_steps),
+        self.state[s] = dict(design=d, steps=0, res=_Residuals(self.every, self.tol_ke, self.tol_T, min_steps, self.stop_rule,
+                                                    self.t_inlet),
                              cells=int(mask.size), t0=time.perf_counter(), history=[], ok=0,
                              bytes=19 * 4 * 2 * (int(np.count_nonzero(mask != 1)) + int(mask.size)))
 

@@ -200,19 +200,175 @@ __global__ void k_wgrad_reduce(const float* __restrict__ part, float* __restrict
   dW[i] = s;
 }
 
-// db[co] = sum_v g[v][co]: one block per channel, fixed-order tree.
-__global__ void k_bias_grad(const __bf16* __restrict__ g, float* __restrict__ db, int64_t V, int C) {
-  __shared__ float red[256];
-  const int co = blockIdx.x;
-  float s = 0.0f;
-  for (int64_t v = threadIdx.x; v < V; v += 256) s += bf2f(g[v * C + co]);
-  red[threadIdx.x] = s;
-  __syncthreads();
-  for (int o = 128; o > 0; o >>= 1) {
-    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+// ------------------------------------------- weight grad, z-column tiles ---
+// For Z in {16, 32}: a block owns tiles of TX x TY (x, y) columns over the
+// full z range and the 9 taps that share one dx (grid.y = dx + 1), and loops
+// over tiles (grid.x blocks), accumulating in registers.  Per tile it stages,
+// in LDS and transposed to [channel][column][z]:
+//   sg          the upstream gradient of the tile's columns (A operand, co x voxel);
+//   sx[dz + 1]  the input of the TX x (TY + 2) source columns at x + dx,
+//               shifted by dz (sx[dz + 1][ci][col][z] = in[col][z + dz], zero
+//               outside), so every B operand (voxel x ci) is an aligned
+//               16-byte LDS read whatever the tap.
+// A K-step is 32 consecutive voxels of the tile in (column, z) order; lane
+// group l / 16 takes 8 of them (never straddling a column: Z % 8 == 0).
+// Each wave owns taps p = wave, wave + 4, wave + 8 of the block's nine
+// (p = (dy + 1) * 3 + dz + 1).  Partials: part[block][co][tap * Cin + ci].
+constexpr int kWTX = 2, kWTY = 4;
+
+template <int CIT, int COT, int ZZ>
+__global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restrict__ in,
+                                                         const __bf16* __restrict__ gout, float* __restrict__ part,
+                                                         Geo g, int ntx, int nty) {
+  constexpr int Cin = CIT * 16, Cout = COT * 16;
+  constexpr int OC = kWTX * kWTY;         // output columns per tile
+  constexpr int SY = kWTY + 2;            // source columns per x row (y halo)
+  constexpr int SC = kWTX * SY;           // source columns per tile
+  constexpr int KS = OC * ZZ / 32;        // K-steps per tile
+  // Rows padded by 8 values (16 bytes): consecutive rows start 16 bytes apart
+  // modulo 256, so the 16-byte reads of 8 lanes in different rows hit
+  // different LDS banks.
+  constexpr int RG = OC * ZZ + 8, RX = SC * ZZ + 8;
+  static_assert(ZZ % 8 == 0 && (OC * ZZ) % 32 == 0, "tile shape");
+  __shared__ __attribute__((aligned(16))) __bf16 sg[Cout][RG];
+  __shared__ __attribute__((aligned(16))) __bf16 sx[3][Cin][RX];
+  const int dx = (int)blockIdx.y - 1;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int64_t ntiles = (int64_t)g.N * ntx * nty;
+  floatx4 acc[3][COT][CIT];
+#pragma unroll
+  for (int a = 0; a < 3; ++a)
+#pragma unroll
+    for (int u = 0; u < COT; ++u)
+#pragma unroll
+      for (int w = 0; w < CIT; ++w) acc[a][u][w] = floatx4{0, 0, 0, 0};
+
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int n = (int)(tile / ((int64_t)ntx * nty));
+    const int r = (int)(tile - (int64_t)n * ntx * nty);
+    const int x0 = (r / nty) * kWTX, y0 = (r % nty) * kWTY;
+    // upstream gradient: OC columns x ZZ voxels x Cout / 8 groups of 8 channels
+    for (int e = threadIdx.x; e < OC * ZZ * (Cout / 8); e += 256) {
+      const int cg = e % (Cout / 8), vz = e / (Cout / 8);
+      const int z = vz % ZZ, oc = vz / ZZ;
+      const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
+      u16x8 v = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (x < g.X && y < g.Y)
+        v = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cout + cg * 8);
+#pragma unroll
+      for (int c = 0; c < 8; ++c) reinterpret_cast<unsigned short*>(&sg[cg * 8 + c][0])[oc * ZZ + z] = v[c];
+    }
+    // input of the source columns at x + dx, three z-shifted copies
+    for (int e = threadIdx.x; e < SC * ZZ * (Cin / 8); e += 256) {
+      const int cg = e % (Cin / 8), vz = e / (Cin / 8);
+      const int z = vz % ZZ, sc = vz / ZZ;
+      const int x = x0 + sc / SY + dx, y = y0 + sc % SY - 1;
+      u16x8 v = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (x >= 0 && x < g.X && y >= 0 && y < g.Y)
+        v = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cin + cg * 8);
+      const int o = sc * ZZ + z;
+#pragma unroll
+      for (int c = 0; c < 8; ++c) {
+        if (z + 1 < ZZ) reinterpret_cast<unsigned short*>(&sx[0][cg * 8 + c][0])[o + 1] = v[c];  // dz = -1
+        reinterpret_cast<unsigned short*>(&sx[1][cg * 8 + c][0])[o] = v[c];                      // dz =  0
+        if (z > 0) reinterpret_cast<unsigned short*>(&sx[2][cg * 8 + c][0])[o - 1] = v[c];       // dz = +1
+      }
+    }
+    // positions whose source z lies outside [0, ZZ): zero (never written above)
+    for (int e = threadIdx.x; e < Cin * SC; e += 256) {
+      const int ci = e / SC, sc = e % SC;
+      reinterpret_cast<unsigned short*>(&sx[0][ci][0])[sc * ZZ] = 0;
+      reinterpret_cast<unsigned short*>(&sx[2][ci][0])[sc * ZZ + ZZ - 1] = 0;
+    }
+    __syncthreads();
+    const int a = lane >> 4, m = lane & 15;
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int L = ks * 32 + a * 8;  // the lane group's first voxel, (column, z) order
+      const int oc = L / ZZ, z0 = L % ZZ;
+      const int ox = oc / kWTY, oy = oc % kWTY;
+      bf16x8 A[COT];
+#pragma unroll
+      for (int u = 0; u < COT; ++u) A[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sg[u * 16 + m][L]));
+#pragma unroll
+      for (int pi = 0; pi < 3; ++pi) {
+        const int p = wave + 4 * pi;
+        if (p >= 9) continue;
+        const int dy = p / 3 - 1, dzi = p % 3;
+        const int so = (ox * SY + oy + dy + 1) * ZZ + z0;
+#pragma unroll
+        for (int w = 0; w < CIT; ++w) {
+          const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sx[dzi][w * 16 + m][so]));
+#pragma unroll
+          for (int u = 0; u < COT; ++u)
+            acc[pi][u][w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[u], B, acc[pi][u][w], 0, 0, 0);
+        }
+      }
+    }
     __syncthreads();
   }
-  if (threadIdx.x == 0) db[co] = red[0];
+  // D[m = co][n = ci]: lane holds co = 4 * (l / 16) + j, ci = l % 16 of each 16 x 16 tile.
+  // part[dx + 1][block][co][p * Cin + ci], p = the tap's index among the group's nine.
+  const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+#pragma unroll
+  for (int pi = 0; pi < 3; ++pi) {
+    const int p = wave + 4 * pi;
+    if (p >= 9) continue;
+#pragma unroll
+    for (int u = 0; u < COT; ++u)
+#pragma unroll
+      for (int w = 0; w < CIT; ++w)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int co = u * 16 + (lane >> 4) * 4 + j, ci = w * 16 + (lane & 15);
+          part[(blk * Cout + co) * (9 * Cin) + p * Cin + ci] = acc[pi][u][w][j];
+        }
+  }
+}
+
+// dW[co][tap * Cin + ci] = sum over the blocks of the tap's dx group (fixed
+// order) for tap * Cin + ci < 27 Cin; zero in the K padding.
+__global__ void k_wgrad_tile_reduce(const float* __restrict__ part, float* __restrict__ dW, int nblk, int Cout,
+                                    int Cin, int Kp) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (int64_t)Cout * Kp) return;
+  const int co = (int)(i / Kp), k = (int)(i - (int64_t)co * Kp);
+  float s = 0.0f;
+  if (k < 27 * Cin) {
+    const int grp = k / (9 * Cin), loc = k - grp * 9 * Cin;
+    for (int b = 0; b < nblk; ++b) s += part[(((int64_t)grp * nblk + b) * Cout + co) * (9 * Cin) + loc];
+  }
+  dW[i] = s;
+}
+
+// db[co] = sum_v g[v][co], two passes with fixed order: per-block partials
+// over contiguous voxel ranges (coalesced: thread t reads channel t % C), then
+// one thread per channel sums the blocks.
+__global__ __launch_bounds__(256) void k_bias_part(const __bf16* __restrict__ g, float* __restrict__ part,
+                                                   int64_t V, int C, int64_t vper) {
+  __shared__ float red[256];
+  const int64_t v0 = (int64_t)blockIdx.x * vper;
+  const int64_t v1 = v0 + vper < V ? v0 + vper : V;
+  const int R = 256 / C;  // C divides 256
+  const int c = threadIdx.x % C, r = threadIdx.x / C;
+  float s = 0.0f;
+  if (r < R)
+    for (int64_t v = v0 + r; v < v1; v += R) s += bf2f(g[v * C + c]);
+  red[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float a = 0.0f;
+    for (int k = 0; k < R; ++k) a += red[k * C + threadIdx.x];
+    part[(int64_t)blockIdx.x * C + threadIdx.x] = a;
+  }
+}
+
+__global__ void k_bias_final(const float* __restrict__ part, float* __restrict__ db, int nb, int C) {
+  const int c = threadIdx.x;
+  if (c >= C) return;
+  float a = 0.0f;
+  for (int b = 0; b < nb; ++b) a += part[(int64_t)b * C + c];
+  db[c] = a;
 }
 
 // ------------------------------------------------------------ elementwise ---
@@ -380,17 +536,66 @@ int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void*
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
+}  // extern "C"
+
+namespace {
+
+// Blocks per dx group of the tiled weight gradient (0: the shape takes the chunked kernel).
+int wgrad_tile_blocks(int N, int X, int Y, int Z, int Cin, int Cout) {
+  const int cit = Cin / 16, cot = Cout / 16;
+  const bool ok = (Z == 32 && cot == 1 && (cit == 1 || cit == 2)) ||
+                  (Z == 16 && cot == 2 && (cit == 1 || cit == 2 || cit == 4)) ||
+                  (Z == 8 && cot == 4 && (cit == 2 || cit == 4));
+  if (!ok || Cin % 16 || Cout % 16) return 0;
+  const int64_t ntiles = (int64_t)N * ((X + kWTX - 1) / kWTX) * ((Y + kWTY - 1) / kWTY);
+  int64_t nb = ntiles / 4;
+  nb = nb < 1 ? 1 : (nb > 256 ? 256 : nb);
+  return (int)nb;
+}
+
+template <int CIT, int COT, int ZZ>
+void launch_wgrad_tile(const __bf16* i, const __bf16* go, float* part, Geo g, int nb, hipStream_t s) {
+  const int ntx = (g.X + kWTX - 1) / kWTX, nty = (g.Y + kWTY - 1) / kWTY;
+  hipLaunchKernelGGL((k_conv_wgrad_tile<CIT, COT, ZZ>), dim3(nb, 3), dim3(256), 0, s, i, go, part, g, ntx, nty);
+}
+
+int64_t bias_blocks(int64_t V) {
+  int64_t nb = (V + 4095) / 4096;
+  return nb < 1 ? 1 : (nb > 1024 ? 1024 : nb);
+}
+
+}  // namespace
+
+extern "C" {
+
 int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int N, int X, int Y, int Z, int Cin,
                    int Cout, int Kp, int vchunk, void* stream) {
-  if (Cin % 16 || Cout % 16 || Cin > 128 || vchunk % 32 || vchunk <= 0) return 1;
+  if (Cin % 16 || Cout % 16 || Cin > 128 || vchunk % 32 || vchunk <= 0 || Kp % 32 || Kp < 27 * Cin) return 1;
   Geo g{X, Y, Z, N};
-  const int64_t V = g.V();
-  const int nchunk = (int)((V + vchunk - 1) / vchunk);
   hipStream_t s = (hipStream_t)stream;
-  dim3 grid(27, Cout / 16, nchunk);
   const __bf16* i = (const __bf16*)in;
   const __bf16* go = (const __bf16*)gout;
-  // padded K entries of the partials stay zero: clear once via the reduce input
+  const int64_t n = (int64_t)Cout * Kp;
+  const int nb = wgrad_tile_blocks(N, X, Y, Z, Cin, Cout);
+  if (nb > 0) {
+    const int key = Z * 100 + (Cin / 16) * 10 + Cout / 16;
+    switch (key) {
+      case 3211: launch_wgrad_tile<1, 1, 32>(i, go, part, g, nb, s); break;
+      case 3221: launch_wgrad_tile<2, 1, 32>(i, go, part, g, nb, s); break;
+      case 1612: launch_wgrad_tile<1, 2, 16>(i, go, part, g, nb, s); break;
+      case 1622: launch_wgrad_tile<2, 2, 16>(i, go, part, g, nb, s); break;
+      case 1642: launch_wgrad_tile<4, 2, 16>(i, go, part, g, nb, s); break;
+      case 824: launch_wgrad_tile<2, 4, 8>(i, go, part, g, nb, s); break;
+      case 844: launch_wgrad_tile<4, 4, 8>(i, go, part, g, nb, s); break;
+      default: return 1;
+    }
+    hipLaunchKernelGGL(k_wgrad_tile_reduce, dim3(nblk(n)), dim3(256), 0, s, part, dW, nb, Cout, Cin, Kp);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
+  const int64_t V = g.V();
+  const int nchunk = (int)((V + vchunk - 1) / vchunk);
+  dim3 grid(27, Cout / 16, nchunk);
+  // padded K entries of the partials stay zero: the caller allocates part zeroed
   switch (Cin / 16) {
     case 1: hipLaunchKernelGGL(k_conv_wgrad<1>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
     case 2: hipLaunchKernelGGL(k_conv_wgrad<2>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
@@ -398,20 +603,28 @@ int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int
     case 8: hipLaunchKernelGGL(k_conv_wgrad<8>, grid, dim3(256), 0, s, i, go, part, g, Cout, vchunk, Kp); break;
     default: return 1;
   }
-  const int64_t n = (int64_t)Cout * Kp;
   hipLaunchKernelGGL(k_wgrad_reduce, dim3(nblk(n)), dim3(256), 0, s, part, dW, nchunk, n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 
-int xnn_wgrad_chunks(int N, int X, int Y, int Z, int vchunk) {
+int64_t xnn_wgrad_part_floats(int N, int X, int Y, int Z, int Cin, int Cout, int Kp, int vchunk) {
+  const int nb = wgrad_tile_blocks(N, X, Y, Z, Cin, Cout);
+  if (nb > 0) return (int64_t)3 * nb * Cout * 9 * Cin;
   const int64_t V = (int64_t)N * X * Y * Z;
-  return (int)((V + vchunk - 1) / vchunk);
+  return (V + vchunk - 1) / vchunk * Cout * Kp;
 }
 
-int xnn_bias_grad(const void* g, float* db, int64_t V, int C, void* stream) {
-  hipLaunchKernelGGL(k_bias_grad, dim3(C), dim3(256), 0, (hipStream_t)stream, (const __bf16*)g, db, V, C);
+int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void* stream) {
+  if (C < 1 || C > 256 || 256 % C) return 1;
+  const int64_t nb = bias_blocks(V);
+  const int64_t vper = (V + nb - 1) / nb;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_bias_part, dim3((unsigned)nb), dim3(256), 0, s, (const __bf16*)g, part, V, C, vper);
+  hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(256), 0, s, part, db, (int)nb, C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
+
+int64_t xnn_bias_part_floats(int64_t V, int C) { return bias_blocks(V) * C; }
 
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream) {
   hipLaunchKernelGGL(k_silu_fwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y, n);

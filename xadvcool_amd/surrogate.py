@@ -44,8 +44,11 @@ def lib():
         L.xnn_conv_fwd.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
         L.xnn_wflip.argtypes = [vp, vp, i, i, i, i, vp]
         L.xnn_conv_wgrad.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
-        L.xnn_wgrad_chunks.argtypes = [i, i, i, i, i]
-        L.xnn_bias_grad.argtypes = [vp, vp, i64, i, vp]
+        L.xnn_wgrad_part_floats.argtypes = [i, i, i, i, i, i, i, i]
+        L.xnn_wgrad_part_floats.restype = i64
+        L.xnn_bias_grad.argtypes = [vp, vp, vp, i64, i, vp]
+        L.xnn_bias_part_floats.argtypes = [i64, i]
+        L.xnn_bias_part_floats.restype = i64
         L.xnn_silu_fwd.argtypes = [vp, vp, i64, vp]
         L.xnn_silu_bwd.argtypes = [vp, vp, vp, i64, vp]
         L.xnn_to_bf16.argtypes = [vp, vp, i64, vp]
@@ -182,17 +185,20 @@ class Conv:
         """Accumulate dW, db from the input x (bf16) and upstream gradient g (bf16)."""
         L = lib()
         N, X, Y, Z = shape
-        nchunk = L.xnn_wgrad_chunks(N, X, Y, Z, vchunk)
-        key = (nchunk,)
-        part = self._parts.get(key)
-        if part is None:  # padded K entries must stay zero: allocate zeroed once
-            part = torch.zeros(nchunk * self.cout * self.Kp, dtype=torch.float32, device=x.device)
-            self._parts[key] = part
+        nw = L.xnn_wgrad_part_floats(N, X, Y, Z, self.cin, self.cout, self.Kp, vchunk)
+        nbp = L.xnn_bias_part_floats(N * X * Y * Z, self.cout)
+        key = (nw, nbp)
+        parts = self._parts.get(key)
+        if parts is None:  # zeroed once: the chunked path relies on zero padded K entries
+            parts = (torch.zeros(nw, dtype=torch.float32, device=x.device),
+                     torch.zeros(nbp, dtype=torch.float32, device=x.device))
+            self._parts[key] = parts
+        part, bpart = parts
         dw = torch.empty_like(self.w)
         _ok(L.xnn_conv_wgrad(_p(x), _p(g_bf16), _p(part), _p(dw), N, X, Y, Z, self.cin, self.cout, self.Kp, vchunk,
                              _st()), "xnn_conv_wgrad")
         db = torch.empty_like(self.b)
-        _ok(L.xnn_bias_grad(_p(g_bf16), _p(db), N * X * Y * Z, self.cout, _st()), "xnn_bias_grad")
+        _ok(L.xnn_bias_grad(_p(g_bf16), _p(db), _p(bpart), N * X * Y * Z, self.cout, _st()), "xnn_bias_grad")
         self.dw.add_(dw)
         self.db.add_(db)
 

@@ -233,7 +233,7 @@ class _Stall:
 
 
 def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None,
-                  confirm=CONFIRM):
+                  confirm=CONFIRM, stop_rule="reference"):
     """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
 
     Fallback: a warm design that stalls or diverges is re-run cold in the same call.  Both arms stop when the
@@ -255,17 +255,25 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
             dd["warm"] = True
         queue.append(dd)
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
-                       metrics=True, stall_check=_Stall(tol, tol), confirm=confirm)
+                       metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule)
     if timer is not None:  # events go on the solver's own stream
         timer.stream = bs.batch.stream()
     final = {}
     fallbacks = []
     by_key = {q["key"]: q for q in queue}
     fell_back = set()
+    done = [0]
+
+    def progress(r):
+        done[0] += 1
+        if log is not None and done[0] % 32 == 0:
+            log(f"[solve] {'warm' if predictor is not None else 'cold'}: {done[0]} runs finished, "
+                f"{time.perf_counter() - t0:.0f} s")
+
     try:
         pending = queue
         while pending:
-            res = bs.run(pending, timer=timer)
+            res = bs.run(pending, timer=timer, on_result=progress)
             pending = []
             for r in res:
                 k = r["key"]
@@ -287,9 +295,9 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
     return dict(results=final, wall=wall, inference_s=t_inf, fallbacks=fallbacks, lups=lups)
 
 
-def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print, confirm=CONFIRM):
-    cold = solve_designs(designs, None, slots, tol, max_steps, log, confirm=confirm)
-    warm = solve_designs(designs, net, slots, tol, max_steps, log, confirm=confirm)
+def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print, confirm=CONFIRM, stop_rule="reference"):
+    cold = solve_designs(designs, None, slots, tol, max_steps, log, confirm=confirm, stop_rule=stop_rule)
+    warm = solve_designs(designs, net, slots, tol, max_steps, log, confirm=confirm, stop_rule=stop_rule)
     rows = []
     for d in designs:
         c, w = cold["results"][d["key"]], warm["results"][d["key"]]
@@ -339,6 +347,8 @@ def main():
     ap.add_argument("--slots", type=int, default=64)
     ap.add_argument("--gen-tol", type=float, default=1e-6, help="residual of the cold solves that make the data")
     ap.add_argument("--gen-max-steps", type=int, default=60000)
+    ap.add_argument("--stop-rule", choices=["reference", "rise"], default="reference",
+                    help="residuals of the reference, or the same on the temperature rise and absorbed heat")
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)
     if args.cmd == "train":
@@ -360,7 +370,9 @@ def main():
     else:
         net = S.UNet3D.load_file(args.weights)
         designs = make_designs(COMPARE_SEED0 if args.seed0 is None else args.seed0, args.n)
-        summ, rows = compare(net, designs, slots=args.slots, tol=args.tol, max_steps=args.max_steps, log=log)
+        summ, rows = compare(net, designs, slots=args.slots, tol=args.tol, max_steps=args.max_steps, log=log,
+                             stop_rule=args.stop_rule)
+        summ["stop_rule"] = args.stop_rule
         with open(f"{args.out}/warmstart_compare.json", "w") as f:
             json.dump({"summary": summ, "rows": rows}, f)
         log(json.dumps(summ))
