@@ -173,6 +173,47 @@ def profile_copy(mbytes, reps=20):
                       "copy_GBs": round(8 * n / (ms * 1e-3) / 1e9, 1), "us": round(1000 * ms, 2)}), flush=True)
 
 
+def train_step_bench(steps=4, batch=8, forwards=3):
+    """Surrogate training steps (forward, weighted MSE, backward, Adam) at batch 8 on 152x128x32 with random inputs
+    of the real shape (no data generation): the kernels of the warm-start network, for traces."""
+    import torch
+    from xadvcool_amd import surrogate as S
+    torch.manual_seed(0)
+    net = S.UNet3D(seed=0)
+    net.norm = (np.zeros(S.NOUT, np.float32), np.ones(S.NOUT, np.float32))
+    X, Y, Z = 152, 128, 32
+    x = torch.randn(batch, X, Y, Z, S.CIN, device="cuda").to(torch.bfloat16)
+    t = torch.randn(batch, X, Y, Z, S.NOUT, device="cuda")
+    w = torch.ones(batch, X, Y, Z, S.NOUT, device="cuda")
+    times = []
+    for _ in range(steps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = net.forward(x, keep=True)
+        _, g = S.mse_loss(out, t, w)
+        net.zero_grad()
+        net.backward(g)
+        net.adam_step(1e-4)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(forwards):
+        net.forward(x)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / max(forwards, 1)
+    step = min(times[1:] or times)
+    flop_fwd = 0.0  # 2 * 27 * cin * cout FLOP per output voxel of each layer, at its level's resolution
+    shapes = {"c1a": 0, "c1b": 0, "d1a": 0, "d1b": 0, "head": 0, "c2a": 1, "c2b": 1, "d2a": 1, "d2b": 1,
+              "c3a": 2, "c3b": 2, "d3a": 2, "d3b": 2, "c4a": 3, "c4b": 3}
+    for k, c in net.conv.items():
+        flop_fwd += 2.0 * 27 * c.cin * c.cout * batch * (X * Y * Z) / 8 ** shapes[k]
+    print(json.dumps({"mode": "train-step", "batch": batch, "lattice": [X, Y, Z], "ms_per_step": round(1e3 * step, 2),
+                      "forward_ms": round(1e3 * fwd, 2), "forward_GFLOP": round(flop_fwd / 1e9, 1),
+                      "forward_TFLOPs": round(flop_fwd / fwd / 1e12, 1),
+                      "step_TFLOPs": round(3 * flop_fwd / step / 1e12, 1)}), flush=True)
+
+
 def batch_bench(n_designs, slots, max_steps, tol, seed0=2000, **shape):
     """Config-3 style batch: mixed families and heat maps, uneven lattices, one launch per step."""
     import torch
@@ -240,7 +281,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1"], default="solves",
+    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1", "train-step"],
+                    default="solves",
                     help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy); "
                          "ceilings: bandwidth of plain copies and of the pass's access shapes")
     ap.add_argument("--passes", type=int, default=400)
@@ -267,6 +309,8 @@ def main():
         sys.exit(subprocess.run([exe]).returncode)
     if args.mode == "config1":
         return config1_bench()
+    if args.mode == "train-step":
+        return train_step_bench()
     if args.mode == "batch":
         print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots or 64, args.max_steps, args.tol)}),
               flush=True)

@@ -356,6 +356,12 @@ def mse_loss(pred, target, wmask):
     """Weighted MSE on the first NOUT channels; returns (loss (device scalar), grad (fp32, pred's shape))."""
     V = pred.numel() // pred.shape[-1]
     C = pred.shape[-1]
+    # the kernel reads target and wmask as (V, NOUT): refuse anything else before it runs
+    if C < NOUT or target.numel() != V * NOUT or wmask.numel() != V * NOUT:
+        raise ValueError(f"mse_loss: pred {tuple(pred.shape)}, target {tuple(target.shape)} and wmask "
+                         f"{tuple(wmask.shape)} must share the voxel shape, with {NOUT} target / weight channels")
+    if not (pred.is_contiguous() and target.is_contiguous() and wmask.is_contiguous()):
+        raise ValueError("mse_loss: tensors must be contiguous")
     L = lib()
     nb = L.xnn_mse_blocks(V, C)
     part = torch.empty(nb, dtype=torch.float32, device=pred.device)

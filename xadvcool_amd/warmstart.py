@@ -56,20 +56,22 @@ def _sample(d, fields):
     return (torch.from_numpy(f).to(torch.bfloat16).cuda(), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda())
 
 
-def generate(designs, slots=64, tol=1e-6, max_steps=60000, log=print, confirm=CONFIRM):
-    """Cold batched solves; returns (samples, results) for the designs that converged.
+def generate(designs, slots=64, tol=1e-6, max_steps=60000, log=print, confirm=CONFIRM, stop_rule="reference",
+             keep_unconverged=False):
+    """Cold batched solves; returns (samples, results) for the designs that converged (and, with
+    ``keep_unconverged``, those stopped at ``max_steps``, whose fields are close to the steady state).
 
     The targets must be the converged state the warm runs will be stopped at: the thermal field relaxes
     with a time constant of ~15,000 steps on these lattices, so fields stopped at residual 1e-5 still
     differ from the 1e-6 ones by 10-25 % in R_th."""
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
-                       collect_fields=True, metrics=False, confirm=confirm)
+                       collect_fields=True, metrics=False, confirm=confirm, stop_rule=stop_rule)
     by_key = {d["key"]: d for d in designs}
     samples, results = [], []
     t0 = time.perf_counter()
 
     def on_result(r):
-        if r["status"] == "converged":
+        if r["status"] == "converged" or (keep_unconverged and r["status"] == "max_steps"):
             samples.append(_sample(by_key[r["key"]], r.pop("fields")))
         else:
             r.pop("fields", None)
@@ -101,9 +103,10 @@ def fit_norm(samples):
 
 
 def train(net, samples, epochs=40, batch=8, lr=5e-4, seed=0, val=None, log=print, ckpt=None, ckpt_every=10,
-          warmup=100):
+          warmup=100, deadline=None):
     """Adam, linear warm-up over ``warmup`` steps then cosine decay.  (Peak rates of 2e-3 and above blow up the
-    decoder activations of this un-normalised network within ten steps and leave it dead; 5e-4 trains.)"""
+    decoder activations of this un-normalised network within ten steps and leave it dead; 5e-4 trains.)
+    ``deadline`` (time.perf_counter() value): stop after the epoch that passes it."""
     mean, std = net.norm
     m_t, s_t = torch.tensor(mean, device="cuda"), torch.tensor(std, device="cuda")
     rng = np.random.default_rng(seed)
@@ -135,7 +138,31 @@ def train(net, samples, epochs=40, batch=8, lr=5e-4, seed=0, val=None, log=print
         log(f"[train] {json.dumps(entry)}")
         if ckpt is not None and (ep + 1) % ckpt_every == 0:
             net.save(ckpt)
+        if deadline is not None and time.perf_counter() > deadline:
+            log(f"[train] time budget reached after epoch {ep + 1}")
+            break
     return hist
+
+
+def time_train_step(net, samples, batch=8, reps=3):
+    """Seconds per forward + backward pass of one batch (no weight update; gradients cleared after)."""
+    mean, std = net.norm
+    m_t, s_t = torch.tensor(mean, device="cuda"), torch.tensor(std, device="cuda")
+    idx = list(range(min(batch, len(samples))))
+    x = torch.stack([samples[i][0] for i in idx])
+    t = (torch.stack([samples[i][1] for i in idx]) - m_t) / s_t
+    w = torch.stack([samples[i][2] for i in idx])
+    dt = 0.0
+    for _ in range(reps):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out = net.forward(x, keep=True)
+        _, g = S.mse_loss(out, t, w)
+        net.backward(g)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+    net.zero_grad()
+    return dt
 
 
 @torch.no_grad()
@@ -349,18 +376,38 @@ def main():
     ap.add_argument("--gen-max-steps", type=int, default=60000)
     ap.add_argument("--stop-rule", choices=["reference", "rise"], default="reference",
                     help="residuals of the reference, or the same on the temperature rise and absorbed heat")
+    ap.add_argument("--gen-stop-rule", choices=["reference", "rise"], default="reference",
+                    help="stop rule of the cold solves that make the training data")
+    ap.add_argument("--keep-unconverged", action="store_true",
+                    help="also train on designs stopped at --gen-max-steps (not on diverged ones)")
+    ap.add_argument("--init-weights", default=None, help="continue training from these weights (and their norm)")
+    ap.add_argument("--warmup", type=int, default=100, help="linear learning-rate warm-up steps")
+    ap.add_argument("--deadline-s", type=float, default=None,
+                    help="stop training after the epoch that ends past this many seconds from the start")
     args = ap.parse_args()
     log = lambda s: print(s, flush=True)
     if args.cmd == "train":
+        t_start = time.perf_counter()
         designs = make_designs(0 if args.seed0 is None else args.seed0, args.n_train + args.n_val)
         samples, results = generate(designs, slots=args.slots, tol=args.gen_tol, max_steps=args.gen_max_steps,
-                                    log=log)
+                                    log=log, stop_rule=args.gen_stop_rule, keep_unconverged=args.keep_unconverged)
         nval = min(args.n_val, len(samples) // 8)
         val, tr = samples[:nval], samples[nval:]
-        net = S.UNet3D(seed=0)
-        net.norm = fit_norm(tr)
+        if args.init_weights:
+            net = S.UNet3D.load_file(args.init_weights)
+        else:
+            net = S.UNet3D(seed=0)
+            net.norm = fit_norm(tr)
         log(f"[train] {len(tr)} train / {len(val)} val samples, {net.n_params()} parameters, norm {net.norm}")
-        hist = train(net, tr, epochs=args.epochs, lr=args.lr, val=val, log=log, ckpt=args.weights)
+        deadline = t_start + args.deadline_s if args.deadline_s else None
+        epochs = args.epochs
+        if deadline is not None:  # fit the whole schedule (warm-up + cosine) into the time left
+            step_s = time_train_step(net, tr)
+            per_epoch = step_s * max(1, len(tr) // 8) * 1.1 + 0.5
+            epochs = max(1, min(epochs, int((deadline - time.perf_counter()) / per_epoch)))
+            log(f"[train] {step_s * 1e3:.1f} ms per step: {epochs} epochs fit the time budget")
+        hist = train(net, tr, epochs=epochs, lr=args.lr, val=val, log=log, ckpt=args.weights,
+                     warmup=args.warmup, deadline=deadline)
         net.save(args.weights)
         log(f"[train] validation MSE per channel (u_x, u_y, u_z, rho, T): {evaluate_channels(net, val)}")
         with open(f"{args.out}/train_history.json", "w") as f:
