@@ -20,14 +20,15 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
                  int Cout, int Kp, void* stream);
 /* Weights of the data-gradient convolution: Wt[ci][tap'*Cout+co] = W[co][(26-tap')*Cin+ci]. */
 int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void* stream);
-/* dW[co][tap*Cin+ci] = sum_v gout[v][co] * in[v+tap][ci] (zero in the K padding).  Z in {8, 16, 32} with
+/* dW[co][tap*Cin+ci] = sum_v gout[v][co] * in[v+tap][ci] (zero in the K padding).  Z in {4, 8, 16, 32} with
  * (Cin, Cout) of the network's layers at those levels: z-column tiles staged in LDS, all taps of one dx per
  * block; other shapes: voxel chunks of vchunk (multiple of 32), one tap per block.  part holds
  * xnn_wgrad_part_floats(...) floats, allocated zeroed (the chunked path relies on zero K padding). */
 int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int N, int X, int Y, int Z, int Cin,
                    int Cout, int Kp, int vchunk, void* stream);
 int64_t xnn_wgrad_part_floats(int N, int X, int Y, int Z, int Cin, int Cout, int Kp, int vchunk);
-/* db[c] = sum_v g[v][c] (C divides 256), per-block partials in part (xnn_bias_part_floats floats), fixed order. */
+/* db[c] = sum_v g[v][c] (C a multiple of 8, C/8 dividing 256), per-block partials in part (xnn_bias_part_floats
+ * floats), fixed order. */
 int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void* stream);
 int64_t xnn_bias_part_floats(int64_t V, int C);
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream);

@@ -72,9 +72,10 @@ def test_conv_fwd_dgrad_wgrad(gpu, cin, cout):
 @pytest.mark.parametrize("shape,cin,cout", [((2, 5, 7, 32), 16, 16), ((2, 5, 7, 32), 32, 16),
                                              ((2, 3, 6, 16), 16, 32), ((2, 3, 6, 16), 32, 32),
                                              ((1, 4, 9, 16), 64, 32), ((2, 3, 5, 8), 32, 64),
-                                             ((1, 4, 4, 8), 64, 64)])
+                                             ((1, 4, 4, 8), 64, 64), ((1, 3, 5, 8), 128, 64),
+                                             ((2, 3, 6, 4), 64, 128), ((1, 5, 4, 4), 128, 128)])
 def test_wgrad_tiled_shapes(gpu, shape, cin, cout):
-    """The z-column-tiled weight gradient (Z in {8, 16, 32}, the network's channel pairs at those levels),
+    """The z-column-tiled weight gradient (Z in {4, 8, 16, 32}, the network's channel pairs at those levels),
     with partial tiles at the x and y edges, against torch's conv3d weight and bias gradients."""
     from xadvcool_amd import surrogate as S
     print(f"[nn] tiled wgrad {shape} {cin}->{cout}", flush=True)

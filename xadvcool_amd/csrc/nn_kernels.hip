@@ -201,7 +201,7 @@ __global__ void k_wgrad_reduce(const float* __restrict__ part, float* __restrict
 }
 
 // ------------------------------------------- weight grad, z-column tiles ---
-// For Z in {16, 32}: a block owns tiles of TX x TY (x, y) columns over the
+// For Z in {4, 8, 16, 32}: a block owns tiles of TX x TY (x, y) columns over the
 // full z range and the 9 taps that share one dx (grid.y = dx + 1), and loops
 // over tiles (grid.x blocks), accumulating in registers.  Per tile it stages,
 // in LDS and transposed to [channel][column][z]:
@@ -219,7 +219,8 @@ constexpr int kWTX = 2, kWTY = 4;
 template <int CIT, int COT, int ZZ>
 __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restrict__ in,
                                                          const __bf16* __restrict__ gout, float* __restrict__ part,
-                                                         Geo g, int ntx, int nty) {
+                                                         Geo g, int ntx, int nty, int CoutT) {
+  // This block: COT 16-channel output tiles starting at tile blockIdx.z * COT of the CoutT channels.
   constexpr int Cin = CIT * 16, Cout = COT * 16;
   constexpr int OC = kWTX * kWTY;         // output columns per tile
   constexpr int SY = kWTY + 2;            // source columns per x row (y halo)
@@ -229,7 +230,9 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
   // modulo 256, so the 16-byte reads of 8 lanes in different rows hit
   // different LDS banks.
   constexpr int RG = OC * ZZ + 8, RX = SC * ZZ + 8;
-  static_assert(ZZ % 8 == 0 && (OC * ZZ) % 32 == 0, "tile shape");
+  // A lane group's 8 voxels are 8 / ZZ consecutive columns of one x row when ZZ < 8.
+  static_assert((ZZ % 8 == 0 || (ZZ == 4 && kWTY % 2 == 0)) && (OC * ZZ) % 32 == 0, "tile shape");
+  const int co0 = (int)blockIdx.z * Cout;
   __shared__ __attribute__((aligned(16))) __bf16 sg[Cout][RG];
   __shared__ __attribute__((aligned(16))) __bf16 sx[3][Cin][RX];
   const int dx = (int)blockIdx.y - 1;
@@ -254,7 +257,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
       const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
       u16x8 v = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
       if (x < g.X && y < g.Y)
-        v = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cout + cg * 8);
+        v = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * CoutT + co0 +
+                                             cg * 8);
 #pragma unroll
       for (int c = 0; c < 8; ++c) reinterpret_cast<unsigned short*>(&sg[cg * 8 + c][0])[oc * ZZ + z] = v[c];
     }
@@ -321,7 +325,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
           const int co = u * 16 + (lane >> 4) * 4 + j, ci = w * 16 + (lane & 15);
-          part[(blk * Cout + co) * (9 * Cin) + p * Cin + ci] = acc[pi][u][w][j];
+          part[(blk * CoutT + co0 + co) * (9 * Cin) + p * Cin + ci] = acc[pi][u][w][j];
         }
   }
 }
@@ -342,24 +346,32 @@ __global__ void k_wgrad_tile_reduce(const float* __restrict__ part, float* __res
 }
 
 // db[co] = sum_v g[v][co], two passes with fixed order: per-block partials
-// over contiguous voxel ranges (coalesced: thread t reads channel t % C), then
-// one thread per channel sums the blocks.
+// over contiguous voxel ranges, each thread summing 8 channels loaded as one
+// 16-byte vector (thread t: channel group t % (C/8), voxels t / (C/8) + k R),
+// then one thread per channel sums the blocks.
 __global__ __launch_bounds__(256) void k_bias_part(const __bf16* __restrict__ g, float* __restrict__ part,
                                                    int64_t V, int C, int64_t vper) {
-  __shared__ float red[256];
+  __shared__ float red[256][9];  // 8 channel sums per thread, padded row
   const int64_t v0 = (int64_t)blockIdx.x * vper;
   const int64_t v1 = v0 + vper < V ? v0 + vper : V;
-  const int R = 256 / C;  // C divides 256
-  const int c = threadIdx.x % C, r = threadIdx.x / C;
-  float s = 0.0f;
-  if (r < R)
-    for (int64_t v = v0 + r; v < v1; v += R) s += bf2f(g[v * C + c]);
-  red[threadIdx.x] = s;
+  const int G = C / 8;    // channel groups (G divides 256, checked by the host)
+  const int R = 256 / G;  // voxels per sweep
+  const int cg = threadIdx.x % G, r = threadIdx.x / G;
+  float s[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int64_t v = v0 + r; v < v1; v += R) {
+    const u16x8 raw = *reinterpret_cast<const u16x8*>(g + v * C + cg * 8);
+    const bf16x8 h = __builtin_bit_cast(bf16x8, raw);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) s[j] += bf2f(h[j]);
+  }
+#pragma unroll
+  for (int j = 0; j < 8; ++j) red[threadIdx.x][j] = s[j];
   __syncthreads();
   if (threadIdx.x < C) {
+    const int c = threadIdx.x, cgc = c / 8, j = c % 8;
     float a = 0.0f;
-    for (int k = 0; k < R; ++k) a += red[k * C + threadIdx.x];
-    part[(int64_t)blockIdx.x * C + threadIdx.x] = a;
+    for (int k = 0; k < R; ++k) a += red[k * G + cgc][j];
+    part[(int64_t)blockIdx.x * C + c] = a;
   }
 }
 
@@ -540,13 +552,26 @@ int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void*
 
 namespace {
 
+// Tiled weight gradient for a layer shape: output tiles per block (cotb) and blocks along z (split); false if the
+// shape takes the chunked kernel.  Accumulators per wave: 3 taps x cotb x Cin/16 tiles.
+bool wgrad_tile_plan(int Z, int Cin, int Cout, int& cotb, int& split) {
+  if (Cin % 16 || Cout % 16) return false;
+  const int cit = Cin / 16, cot = Cout / 16;
+  cotb = cot;
+  split = 1;
+  if (Z == 32 && cot == 1 && (cit == 1 || cit == 2)) return true;
+  if (Z == 16 && cot == 2 && (cit == 1 || cit == 2 || cit == 4)) return true;
+  if (Z == 8 && cot == 4 && (cit == 2 || cit == 4)) return true;
+  if (Z == 8 && cot == 4 && cit == 8) return cotb = 1, split = 4, true;
+  if (Z == 4 && cot == 8 && cit == 4) return cotb = 2, split = 4, true;
+  if (Z == 4 && cot == 8 && cit == 8) return cotb = 1, split = 8, true;
+  return false;
+}
+
 // Blocks per dx group of the tiled weight gradient (0: the shape takes the chunked kernel).
 int wgrad_tile_blocks(int N, int X, int Y, int Z, int Cin, int Cout) {
-  const int cit = Cin / 16, cot = Cout / 16;
-  const bool ok = (Z == 32 && cot == 1 && (cit == 1 || cit == 2)) ||
-                  (Z == 16 && cot == 2 && (cit == 1 || cit == 2 || cit == 4)) ||
-                  (Z == 8 && cot == 4 && (cit == 2 || cit == 4));
-  if (!ok || Cin % 16 || Cout % 16) return 0;
+  int cotb, split;
+  if (!wgrad_tile_plan(Z, Cin, Cout, cotb, split)) return 0;
   const int64_t ntiles = (int64_t)N * ((X + kWTX - 1) / kWTX) * ((Y + kWTY - 1) / kWTY);
   int64_t nb = ntiles / 4;
   nb = nb < 1 ? 1 : (nb > 256 ? 256 : nb);
@@ -554,9 +579,11 @@ int wgrad_tile_blocks(int N, int X, int Y, int Z, int Cin, int Cout) {
 }
 
 template <int CIT, int COT, int ZZ>
-void launch_wgrad_tile(const __bf16* i, const __bf16* go, float* part, Geo g, int nb, hipStream_t s) {
+void launch_wgrad_tile(const __bf16* i, const __bf16* go, float* part, Geo g, int nb, int split, int CoutT,
+                       hipStream_t s) {
   const int ntx = (g.X + kWTX - 1) / kWTX, nty = (g.Y + kWTY - 1) / kWTY;
-  hipLaunchKernelGGL((k_conv_wgrad_tile<CIT, COT, ZZ>), dim3(nb, 3), dim3(256), 0, s, i, go, part, g, ntx, nty);
+  hipLaunchKernelGGL((k_conv_wgrad_tile<CIT, COT, ZZ>), dim3(nb, 3, split), dim3(256), 0, s, i, go, part, g, ntx,
+                     nty, CoutT);
 }
 
 int64_t bias_blocks(int64_t V) {
@@ -578,15 +605,20 @@ int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int
   const int64_t n = (int64_t)Cout * Kp;
   const int nb = wgrad_tile_blocks(N, X, Y, Z, Cin, Cout);
   if (nb > 0) {
-    const int key = Z * 100 + (Cin / 16) * 10 + Cout / 16;
+    int cotb = 0, split = 0;
+    wgrad_tile_plan(Z, Cin, Cout, cotb, split);
+    const int key = Z * 100 + (Cin / 16) * 10 + cotb;
     switch (key) {
-      case 3211: launch_wgrad_tile<1, 1, 32>(i, go, part, g, nb, s); break;
-      case 3221: launch_wgrad_tile<2, 1, 32>(i, go, part, g, nb, s); break;
-      case 1612: launch_wgrad_tile<1, 2, 16>(i, go, part, g, nb, s); break;
-      case 1622: launch_wgrad_tile<2, 2, 16>(i, go, part, g, nb, s); break;
-      case 1642: launch_wgrad_tile<4, 2, 16>(i, go, part, g, nb, s); break;
-      case 824: launch_wgrad_tile<2, 4, 8>(i, go, part, g, nb, s); break;
-      case 844: launch_wgrad_tile<4, 4, 8>(i, go, part, g, nb, s); break;
+      case 3211: launch_wgrad_tile<1, 1, 32>(i, go, part, g, nb, split, Cout, s); break;
+      case 3221: launch_wgrad_tile<2, 1, 32>(i, go, part, g, nb, split, Cout, s); break;
+      case 1612: launch_wgrad_tile<1, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
+      case 1622: launch_wgrad_tile<2, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
+      case 1642: launch_wgrad_tile<4, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
+      case 824: launch_wgrad_tile<2, 4, 8>(i, go, part, g, nb, split, Cout, s); break;
+      case 844: launch_wgrad_tile<4, 4, 8>(i, go, part, g, nb, split, Cout, s); break;
+      case 881: launch_wgrad_tile<8, 1, 8>(i, go, part, g, nb, split, Cout, s); break;
+      case 442: launch_wgrad_tile<4, 2, 4>(i, go, part, g, nb, split, Cout, s); break;
+      case 481: launch_wgrad_tile<8, 1, 4>(i, go, part, g, nb, split, Cout, s); break;
       default: return 1;
     }
     hipLaunchKernelGGL(k_wgrad_tile_reduce, dim3(nblk(n)), dim3(256), 0, s, part, dW, nb, Cout, Cin, Kp);
@@ -615,7 +647,7 @@ int64_t xnn_wgrad_part_floats(int N, int X, int Y, int Z, int Cin, int Cout, int
 }
 
 int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void* stream) {
-  if (C < 1 || C > 256 || 256 % C) return 1;
+  if (C < 8 || C > 256 || C % 8 || 256 % (C / 8)) return 1;
   const int64_t nb = bias_blocks(V);
   const int64_t vper = (V + nb - 1) / nb;
   hipStream_t s = (hipStream_t)stream;
