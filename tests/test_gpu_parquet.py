@@ -31,7 +31,7 @@ LOOSE = ("P_pump", "Q_vol", "f_friction", "COP", "T_max_surface", "T_avg_surface
          "fluid_temp_rise", "max_velocity", "max_temperature", "mean_temperature")
 
 
-def solve_row(row):
+def solve_row(row, precision=32, at_steps=None):
     from xadvcool_amd import geometry as G
     from xadvcool_amd.engine import CoupledSolver
     from xadvcool_amd.metrics import device_metrics
@@ -40,9 +40,14 @@ def solve_row(row):
                             "spacing": row["spacing"], "buffer_size": 10})
     heat = G.heat_source_3d(mask.shape, row["heat_source_type"])
     s = CoupledSolver(mask, heat, tau_fluid=row["tau_fluid"], heat_power=row["heat_power"], t_inlet=25.0,
-                      u_inlet_val=row["u_inlet_val"], Pr=7.0, k_ratio=628.0)
+                      u_inlet_val=row["u_inlet_val"], Pr=7.0, k_ratio=628.0, precision=precision)
     try:
         s.init_equilibrium()
+        if at_steps is not None:  # fields after exactly at_steps steps
+            s.step(int(at_steps))
+            s._readout(False)
+            return {"steps": int(at_steps)}, device_metrics(s.batch, 0, mask, tau_fluid=row["tau_fluid"],
+                                                            u_inlet_val=row["u_inlet_val"])
         out = solve_on_batch(s.batch, 0, nx=mask.shape[0], max_steps=30000, tau_fluid=row["tau_fluid"],
                              u_inlet_val=row["u_inlet_val"], stats_every=100, tol_ke=1e-5, tol_T=1e-5,
                              min_steps_rule=True)
@@ -71,3 +76,17 @@ def test_parquet_row(gpu, row):
         assert errs[k] <= 1e-3, (k, met[k], row[k])
     for k in LOOSE:
         assert errs[k] <= 5e-3, (k, met[k], row[k])
+
+
+@pytest.mark.parametrize("row", ROWS[::5], ids=[r["sample_id"] for r in ROWS[::5]])
+def test_parquet_row_fp32_error_against_fp64(gpu, row):
+    """FP32 converged scalars against the FP64 build of the same kernels, at the reference's stop step."""
+    print(f"[parquet fp64] {row['sample_id']}", flush=True)
+    n = int(row["sim_steps"])
+    _, m32 = solve_row(row, 32, at_steps=n)
+    _, m64 = solve_row(row, 64, at_steps=n)
+    errs = {k: abs(m32[k] - m64[k]) / max(abs(m64[k]), 1e-12) for k in TIGHT + LOOSE}
+    print(f"[parquet fp64] {row['sample_id']} at {n} steps: " +
+          ", ".join(f"{k} {v:.2e}" for k, v in errs.items()), flush=True)
+    for k in TIGHT:
+        assert errs[k] <= 1e-3, (k, m32[k], m64[k])
