@@ -37,15 +37,18 @@ COMPARE_SEED0 = 1_000_000  # evaluation designs: seeds far from every training s
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
 
 
-def make_designs(seed0, n, heats=HEAT_SOURCE_NAMES):
+def make_designs(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
+    """n seeded 148x128x32 designs from seed0 on; with ``families``, the first n of those families."""
     out = []
-    for i in range(n):
-        s = seed0 + i
+    s = seed0
+    while len(out) < n:
         rng = np.random.default_rng(10_000_019 * (s + 1))
         heat = heats[int(rng.integers(0, len(heats)))]
         d = family_design(s, core=CORE, heat_name=heat)
         d["key"] = s
-        out.append(d)
+        s += 1
+        if families is None or d["family"] in families:
+            out.append(d)
     return out
 
 
@@ -382,6 +385,8 @@ def main():
                     help="also train on designs stopped at --gen-max-steps (not on diverged ones)")
     ap.add_argument("--init-weights", default=None, help="continue training from these weights (and their norm)")
     ap.add_argument("--warmup", type=int, default=100, help="linear learning-rate warm-up steps")
+    ap.add_argument("--families", default=None,
+                    help="compare: comma-separated families (straight, pinfin, gyroid, schwarz_p, schwarz_d) to keep")
     ap.add_argument("--deadline-s", type=float, default=None,
                     help="stop training after the epoch that ends past this many seconds from the start")
     args = ap.parse_args()
@@ -416,10 +421,22 @@ def main():
         log(f"[train] saved {args.weights}")
     else:
         net = S.UNet3D.load_file(args.weights)
-        designs = make_designs(COMPARE_SEED0 if args.seed0 is None else args.seed0, args.n)
+        fams = set(args.families.split(",")) if args.families else None
+        designs = make_designs(COMPARE_SEED0 if args.seed0 is None else args.seed0, args.n, families=fams)
         summ, rows = compare(net, designs, slots=args.slots, tol=args.tol, max_steps=args.max_steps, log=log,
                              stop_rule=args.stop_rule)
         summ["stop_rule"] = args.stop_rule
+        by_family = {}
+        for r in rows:
+            fam = by_family.setdefault(r["family"], {"designs": 0, "cold_converged": 0, "warm_converged": 0,
+                                                     "both": 0, "within_0.5pct": 0})
+            fam["designs"] += 1
+            fam["cold_converged"] += r["cold_status"] == "converged"
+            fam["warm_converged"] += r["warm_status"] == "converged"
+            if "d_R_th" in r:
+                fam["both"] += 1
+                fam["within_0.5pct"] += r["d_R_th"] <= 5e-3 and r["d_pressure_drop"] <= 5e-3
+        summ["by_family"] = by_family
         with open(f"{args.out}/warmstart_compare.json", "w") as f:
             json.dump({"summary": summ, "rows": rows}, f)
         log(json.dumps(summ))
