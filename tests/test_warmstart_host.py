@@ -54,3 +54,22 @@ def test_compare_designs_are_unseen():
     test = {d["key"] for d in W.make_designs(W.COMPARE_SEED0, 8)}
     assert not train & test
     assert W.COMPARE_SEED0 > 100_000  # far above any training set size used
+
+
+def test_warm_start_dip_then_slow_decay_is_not_abandoned():
+    """A warm start's residual often dips right after the start, rises while the slow modes develop, then
+    decays with the thermal time constant (~15,000 steps); the early dip is not the level to beat."""
+    st = W._Stall(TOL, TOL)
+    i = np.arange(400)
+    dip = 1e-5 * np.exp(-i / 5.0) + 1e-8
+    rise = 3e-6 * np.clip((i - 30) / 30.0, 0.0, 1.0) * np.exp(-np.clip(i - 60, 0, None) / 150.0)
+    dT = np.maximum(dip, rise)
+    assert not any(_feed(st, {"key": 5, "warm": True}, dT))
+
+
+def test_decay_slower_than_the_stall_bound_is_abandoned():
+    """Decaying by less than 10 % over 100 checks counts as stalled."""
+    st = W._Stall(TOL, TOL, after=5000, patience=100)
+    dT = 3e-6 * np.exp(-np.arange(400) / 2000.0)
+    out = _feed(st, {"key": 6, "warm": True}, dT)
+    assert True in out

@@ -231,35 +231,46 @@ class _Stall:
     The run's residual is the worst of its three stop-rule ratios
     (d_ke / tol_ke, dT / tol_T, dQ / tol_T), smoothed by a running median over
     ``window`` checks so that a transient's rate crossing zero or FP32 noise in
-    a converged quantity does not count.  After ``after`` steps the run is
-    abandoned when the smoothed residual exceeds ``growth`` times its best value
-    so far (it grows) or has not improved for ``patience`` checks (it stalls).
+    a converged quantity does not count.  After ``after`` steps (about one
+    flow-through time: a warm start's residual typically dips right after the
+    start, rises while the slow modes develop, then decays), the run is
+    abandoned when the smoothed residual
+      * grows: it is more than ``growth`` times its value ``grow_span`` checks
+        earlier, or
+      * stalls: over the last ``patience`` checks it stayed within a factor
+        ``plateau`` (neither decaying nor growing; a decay with the thermal time
+        constant, ~15,000 steps, spans a factor ~2 over 100 checks).
     Cold runs are never abandoned.
     """
 
-    def __init__(self, tol_ke, tol_T, after=5000, window=20, growth=4.0, patience=100):
+    def __init__(self, tol_ke, tol_T, after=10000, window=20, growth=2.0, grow_span=30, patience=100, plateau=1.25):
         self.tol_ke, self.tol_T = tol_ke, tol_T
-        self.after, self.window, self.growth, self.patience = after, window, growth, patience
+        self.after, self.window, self.patience = after, window, patience
+        self.growth, self.grow_span, self.plateau = growth, grow_span, plateau
         self.runs = {}
 
     def __call__(self, design, step, res):
         if not design.get("warm"):
             return False
         worst = max(res.d_ke / self.tol_ke, res.dT_conv / self.tol_T, res.dQ_conv / self.tol_T)
-        st = self.runs.setdefault(design["key"], {"hist": [], "best": np.inf, "since": 0})
+        st = self.runs.setdefault(design["key"], {"hist": [], "smooth": []})
         st["hist"].append(worst)
         if len(st["hist"]) > self.window:
             st["hist"].pop(0)
         if len(st["hist"]) < self.window:
             return False
         smooth = float(np.median(st["hist"]))
-        if smooth < st["best"]:
-            st["best"], st["since"] = smooth, 0
-        else:
-            st["since"] += 1
+        sm = st["smooth"]
+        sm.append(smooth)
+        keep = max(self.patience, self.grow_span) + 1
+        if len(sm) > keep:
+            sm.pop(0)
         if step <= self.after:
             return False
-        return smooth > self.growth * st["best"] or st["since"] >= self.patience
+        grows = len(sm) > self.grow_span and smooth > self.growth * sm[-self.grow_span - 1]
+        recent = sm[-self.patience - 1:]
+        stalls = len(sm) > self.patience and max(recent) < self.plateau * min(recent)
+        return grows or stalls
 
 
 def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None,
