@@ -1,6 +1,6 @@
 """Benchmark: converged cold-plate solves per hour on one MI355X, warm-started vs plain LBM.
 
-Workload (one "step" = one solve of one synthetic design): full-size single
+Workload (one "step" = one solve of one synthetic design; 24 designs by default): full-size single
 solves on a 148 x 128 x 32 lattice — gyroid TPMS core
 (sin x cos y + sin y cos z + sin z cos x < t, cell period in [16, 32], t in
 [-0.4, 0.4]), quad_core heat map on the base, inlet u in [0.01, 0.05], tau in
@@ -275,7 +275,7 @@ def config1_bench(cpu_steps_1=100, cpu_steps_all=500):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=16, help="timed designs (each solved cold, then warm)")
+    ap.add_argument("--steps", type=int, default=24, help="timed designs (each solved cold, then warm)")
     ap.add_argument("--warmup", type=int, default=1, help="untimed solves before the timed ones")
     ap.add_argument("--max-steps", type=int, default=30000)
     ap.add_argument("--tol", type=float, default=1e-6)

@@ -107,6 +107,53 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const __bf16* __restrict__ in,
   }
 }
 
+// The same convolution for Cout = 16 and Cin in {16, 32} (the full-resolution
+// layers and their data gradients): a wave keeps all of its B fragments (the
+// whole weight matrix, KS x 8 values per lane) in registers and sweeps
+// `tiles` 16-voxel tiles, so per MFMA only the A operand is loaded.  Same
+// MFMA sequence per tile as k_conv_fwd<1>, hence the same results.
+template <int CIN>
+__global__ __launch_bounds__(256) void k_conv_fwd16(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
+                                                    const float* __restrict__ bias, float* __restrict__ out, Geo g,
+                                                    int Kp, int tiles) {
+  constexpr int K = 27 * CIN, KS = (K + 31) / 32;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kg = (lane >> 4) * 8;
+  const int64_t V = g.V();
+  const int64_t t0 = ((int64_t)blockIdx.x * 4 + wave) * tiles;
+  if (t0 * 16 >= V) return;
+  bf16x8 B[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+    B[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)m * Kp + ks * 32 + kg));
+  const float b = bias ? bias[m] : 0.0f;
+  for (int t = 0; t < tiles; ++t) {
+    const int64_t v0 = (t0 + t) * 16;
+    if (v0 >= V) break;
+    const int64_t va = v0 + m;
+    int n = 0, x = 0, y = 0, z = 0;
+    const bool va_ok = va < V;
+    if (va_ok) decompose(g, va, n, x, y, z);
+    floatx4 acc = floatx4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k0 = ks * 32 + kg;
+      u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (va_ok && k0 < K) {
+        const int tap = k0 / CIN, ci = k0 - tap * CIN;
+        const int64_t nb = tap_voxel(g, n, x, y, z, tap);
+        if (nb >= 0) raw = *reinterpret_cast<const u16x8*>(in + nb * CIN + ci);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), B[ks], acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t v = v0 + (lane >> 4) * 4 + j;
+      if (v < V) out[v * 16 + m] = acc[j] + b;
+    }
+  }
+}
+
 // W'[ci][tap' * Cout + co] = W[co][(26 - tap') * Cin + ci], K' padded with zeros.
 __global__ void k_wflip(const __bf16* __restrict__ W, __bf16* __restrict__ Wt, int Cin, int Cout, int Kp,
                         int Kpt) {
@@ -531,6 +578,15 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
   hipStream_t s = (hipStream_t)stream;
   const __bf16* i = (const __bf16*)in;
   const __bf16* w = (const __bf16*)W;
+  if (Cout == 16 && (Cin == 16 || Cin == 32) && Kp == (27 * Cin + 31) / 32 * 32) {
+    const int tiles = 8;
+    const unsigned gw = (unsigned)(((V + 15) / 16 + 4 * tiles - 1) / (4 * tiles));
+    if (Cin == 16)
+      hipLaunchKernelGGL(k_conv_fwd16<16>, dim3(gw), dim3(256), 0, s, i, w, bias, out, g, Kp, tiles);
+    else
+      hipLaunchKernelGGL(k_conv_fwd16<32>, dim3(gw), dim3(256), 0, s, i, w, bias, out, g, Kp, tiles);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
   switch (Cout / 16) {
     case 1: hipLaunchKernelGGL(k_conv_fwd<1>, dim3(grid), dim3(256), 0, s, i, w, bias, out, g, Cin, Kp); break;
     case 2: hipLaunchKernelGGL(k_conv_fwd<2>, dim3(grid), dim3(256), 0, s, i, w, bias, out, g, Cin, Kp); break;

@@ -350,6 +350,33 @@ int cleanup_early(xadv_batch* b, int rc) {
   delete b;
   return rc;
 }
+
+// Frees everything a batch holds, including partially created batches. Every
+// release is attempted; the first HIP error is returned and the last-error
+// text is left untouched, so a failed create keeps its own message.
+hipError_t release_batch(xadv_batch* b) {
+  if (!b) return hipSuccess;
+  hipError_t first = hipSuccess;
+  auto keep = [&](hipError_t e) {
+    if (first == hipSuccess && e != hipSuccess) first = e;
+  };
+  if (b->stream) keep(hipStreamSynchronize(b->stream));
+  for (auto& kv : b->graphs) keep(hipGraphExecDestroy(kv.second));
+  char** ps[] = {&b->f, &b->g, &b->flags, &b->q, &b->fin, &b->fsnap, &b->gsnap, &b->gxh, &b->rho, &b->u, &b->T,
+                 &b->cols};
+  for (char** p : ps)
+    if (*p) keep(hipFree(*p));
+  if (b->partials) keep(hipFree(b->partials));
+  if (b->stats_dev) keep(hipFree(b->stats_dev));
+  if (b->met_partials) keep(hipFree(b->met_partials));
+  if (b->met_dev) keep(hipFree(b->met_dev));
+  if (b->slots_dev) keep(hipFree(b->slots_dev));
+  if (b->slots_host) keep(hipHostFree(b->slots_host));
+  if (b->host_small) keep(hipHostFree(b->host_small));
+  if (b->stream) keep(hipStreamDestroy(b->stream));
+  delete b;
+  return first;
+}
 }  // namespace
 
 // ================================================================ C ABI ====
@@ -411,7 +438,7 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   const size_t rs = b->rsize, S = d.slots;
   const size_t capn = b->cap_npad, cap = b->cap_cells, capP = b->cap_P;
   auto cleanup = [&](int rc) {
-    xadv_batch_destroy(b);
+    (void)release_batch(b);
     return rc;
   };
   hipError_t e;
@@ -461,22 +488,8 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
 }
 
 int xadv_batch_destroy(xadv_batch* b) {
-  if (!b) return XADV_OK;
-  if (b->stream) hipStreamSynchronize(b->stream);
-  for (auto& kv : b->graphs) hipGraphExecDestroy(kv.second);
-  char** ps[] = {&b->f, &b->g, &b->flags, &b->q, &b->fin, &b->fsnap, &b->gsnap, &b->gxh, &b->rho, &b->u, &b->T,
-                 &b->cols};
-  for (char** p : ps)
-    if (*p) hipFree(*p);
-  if (b->partials) hipFree(b->partials);
-  if (b->stats_dev) hipFree(b->stats_dev);
-  if (b->met_partials) hipFree(b->met_partials);
-  if (b->met_dev) hipFree(b->met_dev);
-  if (b->slots_dev) hipFree(b->slots_dev);
-  if (b->slots_host) hipHostFree(b->slots_host);
-  if (b->host_small) hipHostFree(b->host_small);
-  if (b->stream) hipStreamDestroy(b->stream);
-  delete b;
+  hipError_t e = release_batch(b);
+  if (e != hipSuccess) return fail(XADV_E_HIP, std::string("xadv_batch_destroy: ") + hipGetErrorString(e));
   return XADV_OK;
 }
 

@@ -66,8 +66,8 @@ class LatticeBatch:
     # -- lifetime -------------------------------------------------------
     def close(self):
         if getattr(self, "_h", None):
-            self._lib.xadv_batch_destroy(self._h)
-            self._h = None
+            h, self._h = self._h, None
+            _lib.check(self._lib.xadv_batch_destroy(h), "xadv_batch_destroy")
 
     def __del__(self):
         try:

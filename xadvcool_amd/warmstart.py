@@ -80,7 +80,7 @@ def generate(designs, slots=64, tol=1e-6, max_steps=60000, log=print, confirm=CO
             r.pop("fields", None)
         results.append(r)
         if len(results) % 32 == 0:
-            log(f"[datagen] {len(results)}/{len(designs)} solved ({len(samples)} converged), "
+            log(f"[datagen] {len(results)}/{len(designs)} solved ({len(samples)} kept as samples), "
                 f"{time.perf_counter() - t0:.0f} s")
 
     try:
