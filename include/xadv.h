@@ -198,6 +198,84 @@ int xadv_checks(xadv_batch* b, int slot, double k_fluid, double k_solid, double*
  * counterpart: Warp's default stream was implicit (src/simulation.py:93-121). */
 int xadv_get_stream(xadv_batch* b, void** stream);
 
+/* ---- Designs built on the device ---------------------------------------
+ * Masks and heat maps generated on the GPU from design parameters, and a
+ * slot loaded from device-resident inputs, so that a design goes from its
+ * parameters to a loaded slot without a host pass over its cells. */
+
+/* Core families of src/geometry.py:365-503 (design_idx). */
+#define XADV_CORE_EMPTY 0
+#define XADV_CORE_STRAIGHT 1
+#define XADV_CORE_PINFIN 2
+#define XADV_CORE_GYROID 3
+#define XADV_CORE_SCHWARZ_P 4
+#define XADV_CORE_SCHWARZ_D 5
+
+typedef struct {
+  int nx, ny, nz;       /* full lattice; the core spans x in [buffer, nx - buffer), all y and z */
+  int buffer;           /* open cells before and after the core along x (>= 0, 2 buffer < nx) */
+  int base, top;        /* solid floor and lid thickness (base >= 0, top >= 1) */
+  int kind;             /* XADV_CORE_* */
+  double feature_size;  /* straight / pin-fin feature size (and the TPMS level when period <= 0) */
+  double spacing;       /* straight / pin-fin spacing (and the TPMS cell size when period <= 0) */
+  double period;        /* TPMS: > 0 gives omega = 2 pi / period and solid where f < threshold */
+  double threshold;
+  int manifold;         /* TPMS: open core cells at each x end (the reference's 5) */
+} xadv_geometry_desc;
+
+/* Write the (nx, ny, nz) int8 mask of a design into a caller device buffer:
+ * the core of src/geometry.py:365-503 (mapped from feature_size/spacing as
+ * there, or an explicit TPMS period and threshold), then the frame of
+ * src/geometry.py:506-524 (buffers, side walls y = 0 and ny-1, base, lid,
+ * INLET on x = 0 and OUTLET on x = nx-1 over the open y, z cells).  Runs on
+ * the batch's stream and synchronises it. */
+int xadv_geometry_mask(xadv_batch* b, const xadv_geometry_desc* g, int8_t* mask);
+
+#define XADV_HEAT_RECT 0
+#define XADV_HEAT_DISK 1
+#define XADV_HEAT_GAUSS 2
+#define XADV_HEAT_MAX_PRIMS 16
+#define XADV_HEAT_MAX_RADIUS 63
+
+typedef struct {
+  int kind;              /* XADV_HEAT_* */
+  int x0, x1, y0, y1;    /* RECT: cells x0 <= x < x1, y0 <= y < y1 take value */
+  int cx, cy, r2;        /* DISK: cells with (x-cx)^2 + (y-cy)^2 < r2 take value */
+  float value;
+  double gx, gy;         /* GAUSS: adds amplitude * exp(-((x-gx)^2 + (y-gy)^2) / den), */
+  double den, amplitude; /*        in FP64, rounded to FP32 after each addition */
+} xadv_heat_prim;
+
+typedef struct {
+  int mx, my;                   /* map size (the core's x and y extent) */
+  float base;                   /* initial value of every cell */
+  int nprims;                   /* <= XADV_HEAT_MAX_PRIMS */
+  const xadv_heat_prim* prims;  /* host, applied in order */
+  int radius;                   /* Gaussian blur half-width (<= XADV_HEAT_MAX_RADIUS), 0 = none */
+  const double* weights;        /* host, weights[j] for offsets +-j, j = 0..radius */
+  int normalise;                /* 1: divide by the maximum when it is positive */
+} xadv_heat_desc;
+
+/* Write a heat field (nx, ny, nz) FP32 into a caller device buffer: zero
+ * everywhere except the z = 0 plane at x in [x_offset, x_offset + mx),
+ * y in [0, my), which holds the map painted from its primitives, blurred
+ * along x then y with reflecting edges (FP64 sums, FP32 after each axis) and
+ * divided by its maximum.  Restates the ten generators of
+ * src/geometry.py:69-319 (their scipy.ndimage.gaussian_filter and
+ * division by the maximum) and the placement of generate_dataset.py:402-410.
+ * Runs on the batch's stream and synchronises it. */
+int xadv_heat_map(xadv_batch* b, const xadv_heat_desc* h, int nx, int ny, int nz, int x_offset, float* heat);
+
+/* xadv_instance_load with d->mask (int8) and d->heat (FP32, or NULL) in
+ * device memory: the mask checks, cell flags, heat source and compact column
+ * table are built on the device; the results are the same as
+ * xadv_instance_load's for the same mask and heat map.  When the mask breaks
+ * more than one rule, the message names the first of: a code outside 0..3,
+ * an INLET off x = 0, an OUTLET off x = nx-1.  On error the slot is left as it
+ * was.  No reference counterpart (the reference copies host arrays,
+ * src/simulation.py:62-86). */
+int xadv_instance_load_device(xadv_batch* b, int slot, const xadv_instance_desc* d);
+
 /* Message of the last error on this thread ("" if none). */
 const char* xadv_last_error(void);
 

@@ -72,3 +72,43 @@ def test_workload_generators():
         nx, ny, nz = d["mask"].shape
         assert 96 <= nx <= 148 and 64 <= ny <= 128 and 16 <= nz <= 32
         assert d["heat"].shape == d["mask"].shape
+
+
+@pytest.mark.parametrize("mx,my", [(128, 128), (76, 64), (100, 97), (128, 80), (64, 128)])
+def test_heat_programs_reproduce_the_generators(mx, my):
+    """The paint programs the device heat-map builder takes, rendered on the host in the device's order of
+    operations (FP64 blur sums: centre, then the symmetric pairs from the outermost in), give the generators'
+    exact FP32 values, scipy.ndimage.gaussian_filter included."""
+    for name in G.HEAT_SOURCE_NAMES:
+        for seed in ([0, 7, 12345] if name == "gaussian_hotspots" else [0]):
+            prog = G.heat_program(name, mx, my, seed)
+            assert len(prog["prims"]) <= 16
+            if prog["sigma"] is not None:
+                assert G.gauss_weights(prog["sigma"])[0] <= 63
+            ref = G.HEAT_SOURCE_REGISTRY[name](mx, my, seed=seed)
+            got = G.render_heat_program(prog)
+            assert got.dtype == np.float32
+            assert np.array_equal(got, ref), f"{name} {mx}x{my} seed {seed}: {int((got != ref).sum())} differ"
+
+
+def test_gauss_weights_match_scipy():
+    from scipy.ndimage import gaussian_filter1d
+    for sigma in (1.28, 1.6, 128 / 70.0, 128 / 60.0, 0.7, 3.3):
+        r, w = G.gauss_weights(sigma)
+        impulse = np.zeros(4 * r + 3)
+        impulse[2 * r + 1] = 1.0
+        resp = gaussian_filter1d(impulse, sigma)
+        assert np.array_equal(resp[2 * r + 1:3 * r + 2], w)
+        assert np.array_equal(resp[2 * r + 1::-1][:r + 1], w)
+        assert abs(w[0] + 2 * w[1:].sum() - 1.0) < 1e-15
+
+
+def test_family_params_drive_family_design():
+    for seed in range(12):
+        d = G.family_design(seed)
+        p = G.family_params(seed)
+        mask, heat = G.design_arrays(p)
+        assert np.array_equal(mask, d["mask"]) and np.array_equal(heat, d["heat"])
+        assert (p["family"], p["heat_name"], p["tau_fluid"], p["u_inlet_val"]) == \
+            (d["family"], d["heat_name"], d["tau_fluid"], d["u_inlet_val"])
+        assert mask.shape == (p["core"][0] + 2 * G.BUFFER, p["core"][1], p["core"][2])

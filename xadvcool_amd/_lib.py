@@ -19,7 +19,8 @@ NCHECK = 16
 EXPORTS = (
     "xadv_batch_create", "xadv_batch_destroy", "xadv_instance_load", "xadv_instance_init", "xadv_set_active",
     "xadv_set_inlet_velocity", "xadv_step", "xadv_readout", "xadv_stats", "xadv_fields", "xadv_metrics",
-    "xadv_checks", "xadv_get_stream", "xadv_last_error", "xadv_version",
+    "xadv_checks", "xadv_get_stream", "xadv_geometry_mask", "xadv_heat_map", "xadv_instance_load_device",
+    "xadv_last_error", "xadv_version",
 )
 
 
@@ -33,6 +34,34 @@ class BatchDesc(ctypes.Structure):
 
 
 STORAGE_AUTO, STORAGE_DENSE, STORAGE_COMPACT = 0, 1, 2
+
+HEAT_MAX_PRIMS, HEAT_MAX_RADIUS = 16, 63
+
+
+class GeometryDesc(ctypes.Structure):
+    _fields_ = [
+        ("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int), ("buffer", ctypes.c_int),
+        ("base", ctypes.c_int), ("top", ctypes.c_int), ("kind", ctypes.c_int),
+        ("feature_size", ctypes.c_double), ("spacing", ctypes.c_double), ("period", ctypes.c_double),
+        ("threshold", ctypes.c_double), ("manifold", ctypes.c_int),
+    ]
+
+
+class HeatPrim(ctypes.Structure):
+    _fields_ = [
+        ("kind", ctypes.c_int), ("x0", ctypes.c_int), ("x1", ctypes.c_int), ("y0", ctypes.c_int),
+        ("y1", ctypes.c_int), ("cx", ctypes.c_int), ("cy", ctypes.c_int), ("r2", ctypes.c_int),
+        ("value", ctypes.c_float), ("gx", ctypes.c_double), ("gy", ctypes.c_double), ("den", ctypes.c_double),
+        ("amplitude", ctypes.c_double),
+    ]
+
+
+class HeatDesc(ctypes.Structure):
+    _fields_ = [
+        ("mx", ctypes.c_int), ("my", ctypes.c_int), ("base", ctypes.c_float), ("nprims", ctypes.c_int),
+        ("prims", ctypes.POINTER(HeatPrim)), ("radius", ctypes.c_int), ("weights", ctypes.POINTER(ctypes.c_double)),
+        ("normalise", ctypes.c_int),
+    ]
 
 
 class InstanceDesc(ctypes.Structure):
@@ -77,6 +106,9 @@ def load():
     lib.xadv_metrics.argtypes = [vp, ip, ip, ip, ip, ctypes.POINTER(ctypes.c_double)]
     lib.xadv_get_stream.argtypes = [vp, ctypes.POINTER(vp)]
     lib.xadv_checks.argtypes = [vp, ip, dp, dp, ctypes.POINTER(ctypes.c_double)]
+    lib.xadv_geometry_mask.argtypes = [vp, ctypes.POINTER(GeometryDesc), vp]
+    lib.xadv_heat_map.argtypes = [vp, ctypes.POINTER(HeatDesc), ip, ip, ip, ip, vp]
+    lib.xadv_instance_load_device.argtypes = [vp, ip, ctypes.POINTER(InstanceDesc)]
     lib.xadv_last_error.restype = ctypes.c_char_p
     lib.xadv_version.restype = ctypes.c_char_p
     for name in EXPORTS:

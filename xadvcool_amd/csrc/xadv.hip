@@ -6,6 +6,8 @@
 // pass are captured once into a hipGraph and replayed.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
@@ -16,6 +18,7 @@
 #include "../../include/xadv.h"
 #include "lattice.h"
 #include "lbm_kernels.h"
+#include "geometry_kernels.h"
 
 namespace {
 
@@ -59,6 +62,8 @@ struct xadv_batch {
   double* met_partials = nullptr;
   double* met_dev = nullptr;
   double* host_small = nullptr;  // pinned, for stats/metrics read-back
+  char* gen = nullptr;           // device scratch of the design builders (grown on demand)
+  size_t gen_bytes = 0;
   int64_t npass = 0;             // passes since the batch's last init of all slots
   bool fields_fresh = false;
   std::vector<int> loaded, inited;
@@ -236,26 +241,117 @@ int build_flags(const xadv_instance_desc* d, int fluid, int thermal, int general
   return XADV_OK;
 }
 
-template <class R>
-int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
-  std::vector<uint32_t> fl;
-  int rc = build_flags(d, b->d.fluid, b->d.thermal, b->d.general, fl);
-  if (rc) return rc;
-  const int64_t n = (int64_t)d->nx * d->ny * d->nz;
-  // Compact fluid storage: per (x, y) column, the number of its first
-  // non-solid cell and a bit per z for its non-solid cells.
-  std::vector<xadv::ColInfo> cols;
-  int64_t nfluid = 0;
-  if (b->compact) {
-    if (d->nz > xadv::kCompactMaxNz)
-      return fail(XADV_E_ARG, "this batch stores the fluid lattice compactly, which needs nz <= 32; create the "
-                              "batch with fluid_storage = XADV_STORAGE_DENSE (or cap_nz > 32) for taller lattices");
-    cols.resize((size_t)xadv::padded_columns(d->nx, d->ny));
-    nfluid = xadv::build_columns(d->mask, d->nx, d->ny, d->nz, cols.data());
+// Device scratch of at least `bytes` for the design builders (kept, grown on demand).
+int ensure_gen(xadv_batch* b, size_t bytes) {
+  if (b->gen_bytes >= bytes) return XADV_OK;
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->gen) {
+    HIPCHK(hipFree(b->gen));
+    b->gen = nullptr;
+    b->gen_bytes = 0;
   }
-  std::vector<R> q(n, (R)0);
-  if (d->heat) {
-    for (int64_t c = 0; c < n; ++c) q[c] = (R)d->heat[c] * (R)d->heat_power;  // thermal_engine3d.py:116-117
+  HIPCHK(hipMalloc((void**)&b->gen, bytes));
+  b->gen_bytes = bytes;
+  return XADV_OK;
+}
+
+constexpr int kCheckBlocks = 512;                 // blocks of the mask check
+constexpr size_t kGenHead = 4096;                 // check partials, error word, cell count
+const char* compact_nz_message() {
+  return "this batch stores the fluid lattice compactly, which needs nz <= 32; create the batch with "
+         "fluid_storage = XADV_STORAGE_DENSE (or cap_nz > 32) for taller lattices";
+}
+
+// Slot inputs from device-resident mask and heat map: checks first (nothing
+// of the slot is touched when they fail), then flags, q and the column table
+// written straight into the slot's arenas.  Returns the non-solid cell count
+// through *nfluid.
+template <class R>
+int build_inputs_device(xadv_batch* b, int s, const xadv_instance_desc* d, int64_t* nfluid) {
+  using namespace xadv;
+  const int nx = d->nx, ny = d->ny, nz = d->nz;
+  const int64_t n = (int64_t)nx * ny * nz;
+  const int px = b->d.general ? d->periodic[0] : 0;
+  int rc = ensure_gen(b, kGenHead);
+  if (rc) return rc;
+  uint32_t* part = reinterpret_cast<uint32_t*>(b->gen);
+  uint32_t* err = part + kCheckBlocks;
+  uint64_t* cnt = reinterpret_cast<uint64_t*>(b->gen + 2048 + 64);
+  const int nb = (int)std::min<int64_t>(kCheckBlocks, (n + 255) / 256);
+  hipLaunchKernelGGL(k_mask_check, dim3(nb), dim3(256), 0, b->stream, d->mask, nx, ny, nz, b->d.fluid,
+                     b->d.thermal, part);
+  hipLaunchKernelGGL(k_or_reduce, dim3(1), dim3(64), 0, b->stream, part, nb, err);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(b->host_small, err, sizeof(uint32_t), hipMemcpyDeviceToHost, b->stream));
+  HIPCHK(hipStreamSynchronize(b->stream));
+  uint32_t bits;
+  memcpy(&bits, b->host_small, sizeof(bits));
+  if (bits & ERR_CODE) return fail(XADV_E_ARG, "mask holds a code other than 0..3");
+  if (bits & ERR_INLET) return fail(XADV_E_ARG, "INLET cells must lie on x = 0 when the fluid lattice runs");
+  if (bits & ERR_OUTLET) return fail(XADV_E_ARG, "OUTLET cells must lie on x = nx-1");
+  if ((bits & ERR_IO) && px) return fail(XADV_E_ARG, "INLET/OUTLET cells need a non-periodic x axis");
+  if ((b->d.fluid || b->d.thermal) && nx < 3) return fail(XADV_E_ARG, "nx must be at least 3");
+  if (b->compact && nz > kCompactMaxNz) return fail(XADV_E_ARG, compact_nz_message());
+
+  FlagParams fp{};
+  fp.nx = nx;
+  fp.ny = ny;
+  fp.nz = nz;
+  fp.px = px;
+  fp.py = b->d.general ? d->periodic[1] : 0;
+  fp.pz = b->d.general ? d->periodic[2] : 0;
+  fp.general = b->d.general;
+  fp.dirichlet_x = d->dirichlet_x;
+  fp.heat_power = d->heat_power;
+  uint32_t* flags = reinterpret_cast<uint32_t*>(b->flags + (size_t)s * b->cap_cells * 4);
+  R* q = reinterpret_cast<R*>(b->q + (size_t)s * b->cap_cells * sizeof(R));
+  hipLaunchKernelGGL((k_slot_flags<R>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, d->mask,
+                     d->heat, fp, flags, q);
+  *nfluid = 0;
+  if (b->compact) {
+    ColInfo* arena = reinterpret_cast<ColInfo*>(b->cols) + (size_t)s * b->cols_per_slot;
+    const int64_t ncol = (int64_t)nx * ny, pad = ny + 1;
+    hipLaunchKernelGGL(k_col_bits, dim3((unsigned)((ncol + 255) / 256)), dim3(256), 0, b->stream, d->mask, ncol,
+                       nz, arena + pad);
+    hipLaunchKernelGGL(k_col_scan, dim3(1), dim3(1024), 0, b->stream, arena, ncol, pad, padded_columns(nx, ny),
+                       cnt);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(b->host_small, cnt, sizeof(uint64_t), hipMemcpyDeviceToHost, b->stream));
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  if (b->compact) {
+    uint64_t v;
+    memcpy(&v, b->host_small, sizeof(v));
+    *nfluid = (int64_t)v;
+  }
+  return XADV_OK;
+}
+
+template <class R>
+int load_t(xadv_batch* b, int s, const xadv_instance_desc* d, bool device_inputs) {
+  const int64_t n = (int64_t)d->nx * d->ny * d->nz;
+  std::vector<uint32_t> fl;
+  std::vector<xadv::ColInfo> cols;
+  std::vector<R> q;
+  int64_t nfluid = 0;
+  if (device_inputs) {
+    int rc = build_inputs_device<R>(b, s, d, &nfluid);
+    if (rc) return rc;
+  } else {
+    int rc = build_flags(d, b->d.fluid, b->d.thermal, b->d.general, fl);
+    if (rc) return rc;
+    // Compact fluid storage: per (x, y) column, the number of its first
+    // non-solid cell and a bit per z for its non-solid cells.
+    if (b->compact) {
+      if (d->nz > xadv::kCompactMaxNz) return fail(XADV_E_ARG, compact_nz_message());
+      cols.resize((size_t)xadv::padded_columns(d->nx, d->ny));
+      nfluid = xadv::build_columns(d->mask, d->nx, d->ny, d->nz, cols.data());
+    }
+    q.assign(n, (R)0);
+    if (d->heat) {
+      for (int64_t c = 0; c < n; ++c) q[c] = (R)d->heat[c] * (R)d->heat_power;  // thermal_engine3d.py:116-117
+    }
   }
   Slot<R>& S = host_slots<R>(b)[s];
   const int64_t cap = b->cap_cells, capn = b->cap_npad, capP = b->cap_P;
@@ -303,12 +399,14 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d) {
   S.rho = reinterpret_cast<R*>(b->rho + (size_t)s * cap * rs);
   S.u = reinterpret_cast<R*>(b->u + (size_t)s * 3 * cap * rs);
   S.T = reinterpret_cast<R*>(b->T + (size_t)s * cap * rs);
-  HIPCHK(hipMemcpyAsync((void*)S.flags, fl.data(), n * 4, hipMemcpyHostToDevice, b->stream));
-  HIPCHK(hipMemcpyAsync((void*)S.q, q.data(), n * rs, hipMemcpyHostToDevice, b->stream));
-  if (b->compact)
-    HIPCHK(hipMemcpyAsync(col_arena, cols.data(), cols.size() * sizeof(xadv::ColInfo), hipMemcpyHostToDevice,
-                          b->stream));
-  HIPCHK(hipStreamSynchronize(b->stream));
+  if (!device_inputs) {
+    HIPCHK(hipMemcpyAsync((void*)S.flags, fl.data(), n * 4, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipMemcpyAsync((void*)S.q, q.data(), n * rs, hipMemcpyHostToDevice, b->stream));
+    if (b->compact)
+      HIPCHK(hipMemcpyAsync(col_arena, cols.data(), cols.size() * sizeof(xadv::ColInfo), hipMemcpyHostToDevice,
+                            b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+  }
   b->loaded[s] = 1;
   b->inited[s] = 0;
   b->frozen[s] = d->frozen_velocity;
@@ -373,6 +471,7 @@ hipError_t release_batch(xadv_batch* b) {
   if (b->slots_dev) keep(hipFree(b->slots_dev));
   if (b->slots_host) keep(hipHostFree(b->slots_host));
   if (b->host_small) keep(hipHostFree(b->host_small));
+  if (b->gen) keep(hipFree(b->gen));
   if (b->stream) keep(hipStreamDestroy(b->stream));
   delete b;
   return first;
@@ -493,7 +592,10 @@ int xadv_batch_destroy(xadv_batch* b) {
   return XADV_OK;
 }
 
-int xadv_instance_load(xadv_batch* b, int slot, const xadv_instance_desc* d) {
+}  // extern "C"
+
+namespace {
+int instance_load(xadv_batch* b, int slot, const xadv_instance_desc* d, bool device_inputs) {
   if (!b || !d || !d->mask) return fail(XADV_E_ARG, "null argument");
   if (slot < 0 || slot >= b->d.slots) return fail(XADV_E_ARG, "slot out of range");
   if (d->nx < 1 || d->ny < 1 || d->nz < 1) return fail(XADV_E_ARG, "dims must be >= 1");
@@ -502,7 +604,133 @@ int xadv_instance_load(xadv_batch* b, int slot, const xadv_instance_desc* d) {
   if (!(d->tau > 0.5) || !(d->tau_th_fluid > 0.5) || !(d->tau_th_solid > 0.5))
     return fail(XADV_E_ARG, "relaxation times must exceed 0.5");
   if (d->frozen_velocity && b->d.fluid) return fail(XADV_E_ARG, "frozen_velocity needs a thermal-only batch");
-  return b->d.precision == 64 ? load_t<double>(b, slot, d) : load_t<float>(b, slot, d);
+  return b->d.precision == 64 ? load_t<double>(b, slot, d, device_inputs) : load_t<float>(b, slot, d, device_inputs);
+}
+}  // namespace
+
+extern "C" {
+
+int xadv_instance_load(xadv_batch* b, int slot, const xadv_instance_desc* d) {
+  return instance_load(b, slot, d, false);
+}
+
+int xadv_instance_load_device(xadv_batch* b, int slot, const xadv_instance_desc* d) {
+  return instance_load(b, slot, d, true);
+}
+
+int xadv_geometry_mask(xadv_batch* b, const xadv_geometry_desc* g, int8_t* mask) {
+  using namespace xadv;
+  if (!b || !g || !mask) return fail(XADV_E_ARG, "null argument");
+  if (g->nx < 1 || g->ny < 1 || g->nz < 1) return fail(XADV_E_ARG, "dims must be >= 1");
+  if (g->buffer < 0 || 2 * (int64_t)g->buffer >= g->nx) return fail(XADV_E_ARG, "need 0 <= buffer and 2 buffer < nx");
+  if (g->base < 0 || g->top < 1) return fail(XADV_E_ARG, "need base >= 0 and top >= 1");
+  if (g->kind < XADV_CORE_EMPTY || g->kind > XADV_CORE_SCHWARZ_D) return fail(XADV_E_ARG, "unknown core kind");
+  GeomParams p{};
+  p.nx = g->nx;
+  p.ny = g->ny;
+  p.nz = g->nz;
+  p.buf = g->buffer;
+  p.base = g->base;
+  p.top = g->top;
+  p.kind = g->kind;
+  p.cnx = g->nx - 2 * g->buffer;
+  p.manifold = g->manifold;
+  // Parameter mapping of src/geometry.py:394-398 (raw_width = feature_size,
+  // raw_count = spacing), evaluated as Python does (truncating int()).
+  const double raw_width = g->feature_size, raw_count = g->spacing;
+  if (g->kind == XADV_CORE_STRAIGHT || g->kind == XADV_CORE_PINFIN) {
+    if (!(raw_width > 0.0) || !(raw_count > 0.0) || raw_width > 1e6 || raw_count > 1e6)
+      return fail(XADV_E_ARG, "feature_size and spacing must be positive");
+  }
+  if (g->kind == XADV_CORE_STRAIGHT) {
+    const int cw = std::max(2, (int)(raw_width * 2.0));
+    const int target = std::max(4, (int)(raw_count * 4.0));
+    const int max_n = (int)((double)(g->ny - 1) / (double)(cw + 1));
+    const int nch = std::max(2, std::min(target, max_n));
+    p.channel_width = cw;
+    p.n_channels = nch;
+    p.straight_carves = (int64_t)nch * cw < g->ny;
+    p.gap = (double)(g->ny - nch * cw) / (double)(nch + 1);
+  } else if (g->kind == XADV_CORE_PINFIN) {
+    const int d_pin = std::max(2, (int)(raw_width * 2.0));
+    p.pitch = std::max(d_pin + 2, (int)(raw_count * 3.0));
+    p.half_pitch = (double)p.pitch / 2.0;
+    const double radius = (double)d_pin / 2.0;
+    p.radius2 = radius * radius;
+  } else if (g->kind >= XADV_CORE_GYROID) {
+    if (g->manifold < 0) return fail(XADV_E_ARG, "manifold must be >= 0");
+    if (g->period > 0.0) {
+      p.omega = 2.0 * M_PI / g->period;
+      p.level = g->threshold;
+    } else {
+      const double cell = 20.0 + ((raw_count - 1.0) / 9.0) * 80.0;
+      if (!(cell > 0.0)) return fail(XADV_E_ARG, "TPMS cell size must be positive");
+      p.level = 0.8 - ((raw_width - 1.0) / 9.0) * 1.6;
+      p.omega = 2.0 * M_PI / cell;
+    }
+  }
+  const int64_t n = (int64_t)g->nx * g->ny * g->nz;
+  hipLaunchKernelGGL(k_geometry_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, mask, p);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XADV_OK;
+}
+
+int xadv_heat_map(xadv_batch* b, const xadv_heat_desc* h, int nx, int ny, int nz, int x_offset, float* heat) {
+  using namespace xadv;
+  if (!b || !h || !heat) return fail(XADV_E_ARG, "null argument");
+  if (nx < 1 || ny < 1 || nz < 1) return fail(XADV_E_ARG, "dims must be >= 1");
+  if (h->mx < 1 || h->my < 1 || x_offset < 0 || x_offset + h->mx > nx || h->my > ny)
+    return fail(XADV_E_ARG, "the map must fit the lattice's z = 0 plane at x_offset");
+  if (h->nprims < 0 || h->nprims > XADV_HEAT_MAX_PRIMS || (h->nprims && !h->prims))
+    return fail(XADV_E_ARG, "nprims must be 0..XADV_HEAT_MAX_PRIMS with a prims array");
+  if (h->radius < 0 || h->radius > XADV_HEAT_MAX_RADIUS || (h->radius && !h->weights))
+    return fail(XADV_E_ARG, "radius must be 0..XADV_HEAT_MAX_RADIUS with a weights array");
+  if (h->radius >= h->mx || h->radius >= h->my) return fail(XADV_E_ARG, "blur radius must be below the map size");
+  HeatParams hp{};
+  hp.mx = h->mx;
+  hp.my = h->my;
+  hp.base = h->base;
+  hp.nprims = h->nprims;
+  hp.radius = h->radius;
+  hp.normalise = h->normalise ? 1 : 0;
+  for (int k = 0; k < h->nprims; ++k) {
+    const xadv_heat_prim& a = h->prims[k];
+    if (a.kind < XADV_HEAT_RECT || a.kind > XADV_HEAT_GAUSS) return fail(XADV_E_ARG, "unknown heat primitive");
+    if (a.kind == XADV_HEAT_GAUSS && !(a.den > 0.0)) return fail(XADV_E_ARG, "Gaussian primitive needs den > 0");
+    HeatPrim& o = hp.prims[k];
+    o.kind = a.kind;
+    o.x0 = a.x0;
+    o.x1 = a.x1;
+    o.y0 = a.y0;
+    o.y1 = a.y1;
+    o.cx = a.cx;
+    o.cy = a.cy;
+    o.r2 = a.r2;
+    o.value = a.value;
+    o.gx = a.gx;
+    o.gy = a.gy;
+    o.den = a.den;
+    o.amplitude = a.amplitude;
+  }
+  for (int j = 0; j <= h->radius && h->radius; ++j) hp.w[j] = h->weights[j];
+  const int64_t m = (int64_t)h->mx * h->my, n = (int64_t)nx * ny * nz;
+  int rc = ensure_gen(b, kGenHead + 2 * (size_t)m * sizeof(float));
+  if (rc) return rc;
+  float* A = reinterpret_cast<float*>(b->gen + kGenHead);
+  float* B = A + m;
+  const unsigned gm = (unsigned)((m + 255) / 256);
+  hipLaunchKernelGGL((k_fill<float>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, b->stream, heat, n, 0.0f);
+  hipLaunchKernelGGL(k_heat_paint, dim3(gm), dim3(256), 0, b->stream, A, hp);
+  const float* src = A;
+  if (hp.radius) {
+    hipLaunchKernelGGL(k_heat_blur<0>, dim3(gm), dim3(256), 0, b->stream, A, B, hp);
+    hipLaunchKernelGGL(k_heat_blur<1>, dim3(gm), dim3(256), 0, b->stream, B, A, hp);
+  }
+  hipLaunchKernelGGL(k_heat_place, dim3(1), dim3(1024), 0, b->stream, src, heat, hp, ny, nz, x_offset);
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipStreamSynchronize(b->stream));
+  return XADV_OK;
 }
 
 int xadv_instance_init(xadv_batch* b, int slot, const void* rho, const void* u, const void* T) {

@@ -96,6 +96,69 @@ class LatticeBatch:
         _lib.check(self._lib.xadv_instance_load(self._h, int(slot), ctypes.byref(d)), "xadv_instance_load")
         self.shapes[slot] = (nx, ny, nz)
 
+    # -- designs built on the device -------------------------------------
+    def geometry_mask(self, shape, kind, *, buffer=10, base=4, top=1, feature_size=3.0, spacing=3.0, period=0.0,
+                      threshold=0.0, manifold=5):
+        """Device int8 mask (nx, ny, nz) of a core family (``kind``: 0 empty .. 5 Schwarz D) inside the frame."""
+        nx, ny, nz = (int(v) for v in shape)
+        out = torch.empty((nx, ny, nz), dtype=torch.int8, device="cuda")
+        g = _lib.GeometryDesc(nx, ny, nz, int(buffer), int(base), int(top), int(kind), float(feature_size),
+                              float(spacing), float(period), float(threshold), int(manifold))
+        torch.cuda.synchronize()
+        _lib.check(self._lib.xadv_geometry_mask(self._h, ctypes.byref(g), ctypes.c_void_p(out.data_ptr())),
+                   "xadv_geometry_mask")
+        return out
+
+    def heat_map(self, program, shape, x_offset):
+        """Device FP32 heat field (nx, ny, nz) from a paint program (``geometry.heat_program``)."""
+        from .geometry import gauss_weights
+        nx, ny, nz = (int(v) for v in shape)
+        prims = program["prims"]
+        if len(prims) > _lib.HEAT_MAX_PRIMS:
+            raise ValueError(f"a heat program holds at most {_lib.HEAT_MAX_PRIMS} primitives")
+        arr = (_lib.HeatPrim * max(1, len(prims)))()
+        for k, q in enumerate(prims):
+            a = arr[k]
+            a.kind = int(q["kind"])
+            for f in ("x0", "x1", "y0", "y1", "cx", "cy", "r2"):
+                setattr(a, f, int(q.get(f, 0)))
+            a.value = float(q.get("value", 0.0))
+            for f in ("gx", "gy", "den", "amplitude"):
+                setattr(a, f, float(q.get(f, 0.0)))
+        radius, w = (0, None) if program["sigma"] is None else gauss_weights(program["sigma"])
+        wp = w.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if w is not None else None
+        h = _lib.HeatDesc(int(program["mx"]), int(program["my"]), float(program["base"]), len(prims), arr,
+                          int(radius), wp, int(bool(program["normalise"])))
+        out = torch.empty((nx, ny, nz), dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        _lib.check(self._lib.xadv_heat_map(self._h, ctypes.byref(h), nx, ny, nz, int(x_offset),
+                                           ctypes.c_void_p(out.data_ptr())), "xadv_heat_map")
+        return out
+
+    def load_device(self, slot, mask, *, heat=None, tau=0.6, tau_th_fluid=0.6, tau_th_solid=0.6, heat_power=0.0,
+                    T_in=0.0, rho_out=1.0, u_in=0.0, periodic=(0, 0, 0), force_x=0.0, u_wall=0.0, dirichlet=None,
+                    frozen_velocity=False):
+        """``load`` from a device int8 mask and an optional device FP32 heat field of the same shape."""
+        if not (torch.is_tensor(mask) and mask.is_cuda and mask.dtype == torch.int8 and mask.dim() == 3):
+            raise ValueError("mask must be a 3-D int8 CUDA tensor")
+        mask = mask.contiguous()
+        nx, ny, nz = mask.shape
+        if heat is not None:
+            if not (torch.is_tensor(heat) and heat.is_cuda and heat.dtype == torch.float32):
+                raise ValueError("heat must be a float32 CUDA tensor")
+            if tuple(heat.shape) != (nx, ny, nz):
+                raise ValueError(f"heat_source shape {tuple(heat.shape)} != domain shape ({nx}, {ny}, {nz})")
+            heat = heat.contiguous()
+        dx, dT = dirichlet if dirichlet is not None else (-1, 0.0)
+        d = _lib.InstanceDesc(nx, ny, nz, mask.data_ptr(), heat.data_ptr() if heat is not None else None,
+                              float(tau), float(tau_th_fluid), float(tau_th_solid), float(heat_power), float(T_in),
+                              float(rho_out), float(u_in), (ctypes.c_int * 3)(*[int(bool(p)) for p in periodic]),
+                              float(force_x), float(u_wall), int(dx), float(dT), int(bool(frozen_velocity)))
+        torch.cuda.synchronize()
+        _lib.check(self._lib.xadv_instance_load_device(self._h, int(slot), ctypes.byref(d)),
+                   "xadv_instance_load_device")
+        self.shapes[slot] = (nx, ny, nz)
+
     def _dev(self, a, shape):
         if a is None:
             return None

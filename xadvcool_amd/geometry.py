@@ -254,6 +254,153 @@ def heat_source_3d(mask_shape, name, core=(CORE_NX, CORE_NY), buf=BUFFER, seed=0
     return hs
 
 
+
+# ------------------------------------------- heat maps as paint programs ---
+#
+# Each generator above, restated as the data the device heat-map builder
+# (xadv_heat_map) takes: a base value, primitives painted in order (RECT /
+# DISK assign a value, GAUSS adds a Gaussian in FP64), an optional Gaussian
+# blur (scipy.ndimage.gaussian_filter's weights, truncate 4.0, reflecting
+# edges) and the division by the maximum.  Integer bounds are computed here
+# exactly as the generators compute their slices.
+
+HEAT_RECT, HEAT_DISK, HEAT_GAUSS = 0, 1, 2
+
+
+def _rect(x0, x1, y0, y1, value):
+    return dict(kind=HEAT_RECT, x0=int(x0), x1=int(x1), y0=int(y0), y1=int(y1), value=float(value))
+
+
+def _disk(cx, cy, r, value):
+    return dict(kind=HEAT_DISK, cx=int(cx), cy=int(cy), r2=int(r) ** 2, value=float(value))
+
+
+def gauss_weights(sigma, truncate=4.0):
+    """(radius, w[0..radius]) of scipy.ndimage.gaussian_filter1d for ``sigma`` (weights of offsets 0..radius)."""
+    sd = float(sigma)
+    lw = int(truncate * sd + 0.5)
+    x = np.arange(-lw, lw + 1)
+    phi = np.exp(-0.5 / (sigma * sigma) * x ** 2)
+    phi = phi / phi.sum()
+    return lw, np.ascontiguousarray(phi[lw:], dtype=np.float64)
+
+
+def heat_program(name, nx, ny, seed=0):
+    """The paint program of heat map ``name`` on an (nx, ny) map (see the generators above)."""
+    P = []
+    sigma = None
+    normalise = True
+    if name == "uniform":
+        return dict(mx=nx, my=ny, base=1.0, prims=[], sigma=None, normalise=False)
+    if name == "dual_core":
+        base = 0.1
+        P += [_rect(nx * 0.25, nx * 0.75, ny * 0.55, ny * 0.85, 1.0),
+              _rect(nx * 0.25, nx * 0.75, ny * 0.15, ny * 0.45, 1.0),
+              _rect(nx * 0.30, nx * 0.70, ny * 0.45, ny * 0.55, 0.5)]
+        sigma = nx / 80.0
+    elif name == "quad_core":
+        base = 0.08
+        for x0f, y0f, x1f, y1f in [(0.15, 0.55, 0.45, 0.85), (0.15, 0.15, 0.45, 0.45), (0.55, 0.55, 0.85, 0.85),
+                                   (0.55, 0.15, 0.85, 0.45)]:
+            P.append(_rect(nx * x0f, nx * x1f, ny * y0f, ny * y1f, 1.0))
+        P += [_rect(nx * 0.45, nx * 0.55, ny * 0.10, ny * 0.90, 0.45),
+              _rect(nx * 0.10, nx * 0.90, ny * 0.45, ny * 0.55, 0.45)]
+        sigma = nx / 80.0
+    elif name == "gpu_die":
+        base = 0.05
+        P += [_rect(nx * 0.15, nx * 0.85, ny * 0.15, ny * 0.85, 0.75),
+              _rect(nx * 0.10, nx * 0.90, ny * 0.02, ny * 0.12, 1.0),
+              _rect(nx * 0.10, nx * 0.90, ny * 0.88, ny * 0.98, 1.0)]
+        for cx_f in [0.30, 0.50, 0.70]:
+            P.append(_disk(int(nx * cx_f), int(ny * 0.50), int(min(nx, ny) * 0.04), 1.0))
+        sigma = nx / 60.0
+    elif name == "chiplet":
+        base = 0.03
+        cw, ch = 0.18, 0.22
+        for x0f, y0f in [(0.12, 0.08), (0.12, 0.40), (0.12, 0.72), (0.58, 0.08), (0.58, 0.40), (0.58, 0.72)]:
+            P.append(_rect(nx * x0f, nx * (x0f + cw), ny * y0f, ny * (y0f + ch), 1.0))
+        P.append(_rect(nx * 0.38, nx * 0.52, ny * 0.10, ny * 0.90, 0.35))
+        sigma = nx / 70.0
+    elif name == "igbt_half_bridge":
+        base = 0.02
+        die_h, igbt_w, diode_w = 0.10, 0.50, 0.25
+        for i, y0f in enumerate([0.05, 0.20, 0.38, 0.55, 0.72, 0.88]):
+            if i % 2 == 0:
+                P.append(_rect(nx * 0.20, nx * (0.20 + igbt_w), ny * y0f, ny * (y0f + die_h), 1.0))
+            else:
+                P.append(_rect(nx * 0.30, nx * (0.30 + diode_w), ny * y0f, ny * (y0f + die_h), 0.55))
+        sigma = nx / 60.0
+    elif name == "soc_heterogeneous":
+        base = 0.06
+        P.append(_rect(nx * 0.10, nx * 0.35, ny * 0.60, ny * 0.90, 1.0))
+        for dy in [0.08, 0.22]:
+            for dx in [0.10, 0.28]:
+                P.append(_rect(nx * dx, nx * (dx + 0.12), ny * (0.35 + dy), ny * (0.35 + dy + 0.10), 0.30))
+        P += [_rect(nx * 0.50, nx * 0.90, ny * 0.35, ny * 0.90, 0.65),
+              _rect(nx * 0.05, nx * 0.95, ny * 0.02, ny * 0.12, 0.40),
+              _rect(nx * 0.45, nx * 0.60, ny * 0.15, ny * 0.30, 0.80)]
+        sigma = nx / 70.0
+    elif name == "gaussian_hotspots":
+        base = 0.05
+        rng = np.random.RandomState(seed)
+        for _ in range(rng.randint(1, 4)):
+            cx = rng.uniform(0.2, 0.8) * nx
+            cy = rng.uniform(0.2, 0.8) * ny
+            sg = rng.uniform(0.06, 0.15) * min(nx, ny)
+            amp = rng.uniform(0.5, 1.0)
+            P.append(dict(kind=HEAT_GAUSS, gx=float(cx), gy=float(cy), den=float(2 * sg ** 2), amplitude=float(amp)))
+    elif name == "hotspot_on_background":
+        base = 0.20
+        P += [_disk(int(nx * 0.40), int(ny * 0.60), int(min(nx, ny) * 0.07), 1.0),
+              _disk(int(nx * 0.65), int(ny * 0.35), int(min(nx, ny) * 0.05), 0.85)]
+        sigma = nx / 100.0
+    elif name == "peripheral_ring":
+        base = 0.05
+        rw = 0.10
+        P += [_rect(0, nx * rw, 0, ny, 0.85), _rect(nx * (1 - rw), nx, 0, ny, 0.85),
+              _rect(0, nx, 0, ny * rw, 0.85), _rect(0, nx, ny * (1 - rw), ny, 0.85)]
+        corner_r = int(min(nx, ny) * 0.12)
+        for cx_f, cy_f in [(0.0, 0.0), (0.0, 1.0), (1.0, 0.0), (1.0, 1.0)]:
+            P.append(_disk(int(nx * cx_f), int(ny * cy_f), corner_r, 1.0))
+        P.append(_rect(nx * 0.20, nx * 0.80, ny * 0.20, ny * 0.80, 0.15))
+        sigma = nx / 60.0
+    else:
+        raise KeyError(f"unknown heat map {name!r}")
+    return dict(mx=nx, my=ny, base=base, prims=P, sigma=sigma, normalise=normalise)
+
+
+def _reflect(i, n):
+    return np.where(i < 0, -i - 1, np.where(i >= n, 2 * n - i - 1, i))
+
+
+def render_heat_program(prog):
+    """Host rendering of a paint program, in the device builder's order of operations (test restatement)."""
+    nx, ny = prog["mx"], prog["my"]
+    hm = np.full((nx, ny), prog["base"], dtype=np.float32)
+    xx, yy = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    for q in prog["prims"]:
+        if q["kind"] == HEAT_RECT:
+            hm[max(q["x0"], 0):max(q["x1"], 0), max(q["y0"], 0):max(q["y1"], 0)] = np.float32(q["value"])
+        elif q["kind"] == HEAT_DISK:
+            hm[(xx - q["cx"]) ** 2 + (yy - q["cy"]) ** 2 < q["r2"]] = np.float32(q["value"])
+        else:
+            dx, dy = xx - q["gx"], yy - q["gy"]
+            hm = (hm.astype(np.float64) + q["amplitude"] * np.exp(-(dx * dx + dy * dy) / q["den"])).astype(np.float32)
+    if prog["sigma"] is not None:
+        r, w = gauss_weights(prog["sigma"])
+        for axis in (0, 1):
+            n = hm.shape[axis]
+            idx = np.arange(n)
+            src = hm.astype(np.float64)
+            take = (lambda j: src[_reflect(idx + j, n), :]) if axis == 0 else (lambda j: src[:, _reflect(idx + j, n)])
+            acc = take(0) * w[0]
+            for j in range(r, 0, -1):
+                acc = acc + (take(-j) + take(j)) * w[j]
+            hm = acc.astype(np.float32)
+    if prog["normalise"] and hm.max() > 0:
+        hm = hm / hm.max()
+    return hm.astype(np.float32)
+
 # ------------------------------------------------- design-level helpers ---
 
 def compute_porosity(mask, buf=BUFFER):
@@ -379,12 +526,8 @@ FAMILIES = ("straight", "pinfin", "gyroid", "schwarz_p", "schwarz_d")
 BATCH_HEATS = ("uniform", "dual_core", "quad_core", "gaussian_hotspots", "peripheral_ring")
 
 
-def family_design(seed, core=None, family=None, heat_name=None):
-    """A seeded design of any family on an (optionally random) core: the batched workloads.
-
-    Lattice: nx in [96, 148] (core = nx - 20), ny in [64, 128], nz in [16, 32] unless ``core`` is given.
-    TPMS u_in is capped at 0.025 as in the reference's generator (tortuous channels amplify the inlet velocity).
-    """
+def family_params(seed, core=None, family=None, heat_name=None):
+    """The seeded parameters of ``family_design`` (every random draw, no arrays)."""
     rng = np.random.default_rng(seed)
     if core is None:
         core = (int(rng.integers(96, 149)) - 2 * BUFFER, int(rng.integers(64, 129)), int(rng.integers(16, 33)))
@@ -392,21 +535,60 @@ def family_design(seed, core=None, family=None, heat_name=None):
     heat_name = heat_name or BATCH_HEATS[int(rng.integers(0, len(BATCH_HEATS)))]
     tau = float(rng.uniform(0.52, 0.56))
     P = float(rng.uniform(0.05, 0.20))
+    out = dict(seed=seed, core=tuple(int(c) for c in core), family=family, heat_name=heat_name, tau_fluid=tau,
+               heat_power=P)
     if family in ("straight", "pinfin"):
-        idx = 1 if family == "straight" else 2
-        r = DESIGN_CONFIGS["Straight Channels" if idx == 1 else "Pin-Fin Staggered"]["ranges"]
-        fs = float(rng.uniform(*r["feature_size"]))
-        sp = float(rng.uniform(*r["spacing"]))
-        mask = _frame(core_solid(idx, fs, sp, core), BUFFER)
-        u_in = float(rng.uniform(0.02, 0.05))
-        extra = dict(feature_size=fs, spacing=sp)
+        r = DESIGN_CONFIGS["Straight Channels" if family == "straight" else "Pin-Fin Staggered"]["ranges"]
+        out["feature_size"] = float(rng.uniform(*r["feature_size"]))
+        out["spacing"] = float(rng.uniform(*r["spacing"]))
+        out["u_inlet_val"] = float(rng.uniform(0.02, 0.05))
     else:
-        kind = {"gyroid": 3, "schwarz_p": 4, "schwarz_d": 5}[family]
-        period = float(rng.uniform(16.0, 32.0))
-        thr = float(rng.uniform(-0.4, 0.4))
-        mask = tpms_mask(kind, period, thr, core)
-        u_in = float(rng.uniform(0.01, 0.025))
-        extra = dict(period=period, threshold=thr)
-    heat = heat_source_3d(mask.shape, heat_name, core=core[:2], seed=seed)
-    return dict(mask=mask, heat=heat, tau_fluid=tau, u_inlet_val=u_in, heat_power=P, seed=seed, family=family,
-                heat_name=heat_name, **extra)
+        out["period"] = float(rng.uniform(16.0, 32.0))
+        out["threshold"] = float(rng.uniform(-0.4, 0.4))
+        out["u_inlet_val"] = float(rng.uniform(0.01, 0.025))
+    return out
+
+
+FAMILY_CORE_KIND = {"empty": 0, "straight": 1, "pinfin": 2, "gyroid": 3, "schwarz_p": 4, "schwarz_d": 5}
+
+
+def design_arrays(params):
+    """Host (mask, heat) of a parameter set from ``family_params``."""
+    core, family = params["core"], params["family"]
+    if family in ("straight", "pinfin"):
+        mask = _frame(core_solid(FAMILY_CORE_KIND[family], params["feature_size"], params["spacing"], core), BUFFER)
+    else:
+        mask = tpms_mask(FAMILY_CORE_KIND[family], params["period"], params["threshold"], core)
+    heat = heat_source_3d(mask.shape, params["heat_name"], core=core[:2], seed=params["seed"])
+    return mask, heat
+
+
+def device_design(batch, params):
+    """(mask, heat) of a ``family_params`` parameter set built on the GPU by ``batch`` (a LatticeBatch).
+
+    Same arrays as ``design_arrays(params)``: the mask bit for bit, the heat map to the last bit of the device's
+    exp against the host's (the blur and the division are the same operations in the same order)."""
+    core, family = params["core"], params["family"]
+    shape = (core[0] + 2 * BUFFER, core[1], core[2])
+    kind = FAMILY_CORE_KIND[family]
+    if family in ("straight", "pinfin"):
+        mask = batch.geometry_mask(shape, kind, buffer=BUFFER, base=BASE_THICKNESS, top=TOP_THICKNESS,
+                                   feature_size=params["feature_size"], spacing=params["spacing"])
+    else:
+        mask = batch.geometry_mask(shape, kind, buffer=BUFFER, base=BASE_THICKNESS, top=TOP_THICKNESS,
+                                   period=params["period"], threshold=params["threshold"], manifold=5)
+    heat = batch.heat_map(heat_program(params["heat_name"], core[0], core[1], params["seed"]), shape, BUFFER)
+    return mask, heat
+
+
+def family_design(seed, core=None, family=None, heat_name=None):
+    """A seeded design of any family on an (optionally random) core: the batched workloads.
+
+    Lattice: nx in [96, 148] (core = nx - 20), ny in [64, 128], nz in [16, 32] unless ``core`` is given.
+    TPMS u_in is capped at 0.025 as in the reference's generator (tortuous channels amplify the inlet velocity).
+    """
+    p = family_params(seed, core, family, heat_name)
+    mask, heat = design_arrays(p)
+    extra = {k: p[k] for k in ("feature_size", "spacing", "period", "threshold") if k in p}
+    return dict(mask=mask, heat=heat, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"],
+                heat_power=p["heat_power"], seed=seed, family=p["family"], heat_name=p["heat_name"], **extra)
