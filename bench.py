@@ -9,8 +9,10 @@ FP32 D3Q19 fluid + D3Q19 thermal lattices; run to the residual tolerance
 (the reference's d_ke / dT / dQ residuals every 100 steps, floor off) or
 30,000 steps.  The same designs are solved twice, one after the other:
 cold (equilibrium at rest) and warm (surrogate-predicted rho, u, T; a warm
-run that stalls or diverges is re-run cold).  Warm wall time includes the
-surrogate inference and any fallback.
+run that stalls or diverges is re-run cold).  Each arm's wall time covers
+building its designs on the GPU from their parameters (mask, heat map, slot
+inputs), the solves, metrics and, for the warm arm, the surrogate inference
+and any fallback.
 
 Printed (one JSON line): warm-start converged solves/hour (``value``) with
 the cold-start figure and the relative differences of the converged pressure
@@ -173,6 +175,54 @@ def profile_copy(mbytes, reps=20):
                       "copy_GBs": round(8 * n / (ms * 1e-3) / 1e9, 1), "us": round(1000 * ms, 2)}), flush=True)
 
 
+def designs_bench(n=64, seed0=2_000_000):
+    """Design parameters to a loaded slot: host path (numpy generators + host flag build) against the device path
+    (xadv_geometry_mask, xadv_heat_map, xadv_instance_load_device), over the same n full-size designs of every
+    family and batch heat map (148x128x32)."""
+    import torch
+    from xadvcool_amd import geometry as G
+    from xadvcool_amd.engine import LatticeBatch
+    core = (128, 128, 32)
+    params = [G.family_params(seed0 + i, core=core) for i in range(n)]
+    b = LatticeBatch(slots=1, cap_shape=(core[0] + 2 * G.BUFFER, core[1], core[2]))
+
+    def kw(p):
+        tau = p["tau_fluid"]
+        ttf = 0.5 + (tau - 0.5) / 7.0
+        return dict(tau=tau, tau_th_fluid=ttf, tau_th_solid=0.5 + 628.0 * (ttf - 0.5), heat_power=p["heat_power"],
+                    T_in=25.0, u_in=p["u_inlet_val"])
+
+    def host(p):
+        t0 = time.perf_counter()
+        mask, heat = G.design_arrays(p)
+        t1 = time.perf_counter()
+        b.load(0, mask, heat=heat, **kw(p))
+        return t1 - t0, time.perf_counter() - t1
+
+    def dev(p):
+        t0 = time.perf_counter()
+        mask, heat = G.device_design(b, p)
+        t1 = time.perf_counter()
+        b.load_device(0, mask, heat=heat, **kw(p))
+        return t1 - t0, time.perf_counter() - t1
+
+    out = {"mode": "designs", "designs": n, "lattice": [core[0] + 2 * G.BUFFER, core[1], core[2]],
+           "families": {f: sum(p["family"] == f for p in params) for f in G.FAMILIES}}
+    for name, fn in (("host", host), ("device", dev)):
+        fn(params[0])  # first call: allocations, module loads
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        parts = np.array([fn(p) for p in params])
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t
+        out[name] = {"designs_per_s": round(n / wall, 1), "ms_per_design": round(1e3 * wall / n, 3),
+                     "build_ms": round(1e3 * parts[:, 0].mean(), 3), "load_ms": round(1e3 * parts[:, 1].mean(), 3)}
+        print(f"[designs] {name}: {out[name]}", file=sys.stderr, flush=True)
+    out["speedup"] = round(out["host"]["ms_per_design"] / out["device"]["ms_per_design"], 1)
+    b.close()
+    print(json.dumps(out), flush=True)
+
+
 def train_step_bench(steps=4, batch=8, forwards=3):
     """Surrogate training steps (forward, weighted MSE, backward, Adam) at batch 8 on 152x128x32 with random inputs
     of the real shape (no data generation): the kernels of the warm-start network, for traces."""
@@ -281,7 +331,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1", "train-step"],
+    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1", "train-step",
+                                       "designs"],
                     default="solves",
                     help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy); "
                          "ceilings: bandwidth of plain copies and of the pass's access shapes")
@@ -311,6 +362,8 @@ def main():
         return config1_bench()
     if args.mode == "train-step":
         return train_step_bench()
+    if args.mode == "designs":
+        return designs_bench(args.designs)
     if args.mode == "batch":
         print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots or 64, args.max_steps, args.tol)}),
               flush=True)
@@ -318,10 +371,17 @@ def main():
 
     from xadvcool_amd import surrogate as S
     from xadvcool_amd import warmstart as W
-    from xadvcool_amd.geometry import config2_design
+    from xadvcool_amd.geometry import config2_design, config2_params
 
     def designs(seeds):
-        return [dict(config2_design(sd), key=sd) for sd in seeds]
+        # Parameters only: each arm builds its designs on the GPU (xadv_geometry_mask, xadv_heat_map) and loads
+        # them from device memory (xadv_instance_load_device), inside its timed wall.
+        out = []
+        for sd in seeds:
+            p = config2_params(sd)
+            out.append(dict(params=p, key=sd, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"],
+                            heat_power=p["heat_power"], family=p["family"], heat_name=p["heat_name"]))
+        return out
 
     seeds = [1000 + i for i in range(args.steps)]
     timed = designs(seeds)
@@ -352,7 +412,7 @@ def main():
         if c["converged"] and w["converged"]:
             d_dp.append(abs(w["metrics"]["pressure_drop"] / c["metrics"]["pressure_drop"] - 1.0))
             d_rth.append(abs(w["metrics"]["R_th"] / c["metrics"]["R_th"] - 1.0))
-    cells = timed[0]["mask"].size
+    cells = int(np.prod(LATTICE))
     c_steps = [x["steps"] for x in cres]
     w_steps = [x["steps"] + x.get("abandoned_steps", 0) for x in wres]
     pass_ms, npass, alg_bytes = timer.totals()
@@ -404,7 +464,8 @@ def main():
                              "passes --slots 64"},
     }
     if not args.no_cpu:
-        mlups1, n1, t1 = cpu_baseline(timed[0], args.cpu_seconds, 1)
+        d0 = dict(config2_design(seeds[0]), key=seeds[0])
+        mlups1, n1, t1 = cpu_baseline(d0, args.cpu_seconds, 1)
         cpu_s = sum(c_steps) * cells / (mlups1 * 1e6)  # the cold arm's lattice updates at the oracle's rate
         line["cpu_baseline"] = {
             "value": round(3600.0 * c_conv / cpu_s, 4),
@@ -415,7 +476,7 @@ def main():
         }
         from oracle import oracle as orc
         nt = orc.max_threads()
-        mlups_n, n_n, t_n = cpu_baseline(timed[0], args.cpu_seconds / 2, nt)
+        mlups_n, n_n, t_n = cpu_baseline(d0, args.cpu_seconds / 2, nt)
         line["cpu_baseline"]["all_threads"] = {
             "threads": nt, "mlups": round(mlups_n, 3),
             "solves_per_hr": round(3600.0 * c_conv / (sum(c_steps) * cells / (mlups_n * 1e6)), 4),

@@ -202,3 +202,47 @@ def test_device_load_rejects_bad_masks_and_keeps_the_slot():
             g.load_device(0, torch.from_numpy(m).cuda(), periodic=(1, 0, 0))
         finally:
             g.close()
+
+
+def test_device_features_equal_host_features():
+    from xadvcool_amd import surrogate as S
+    from xadvcool_amd.batch import device_inputs
+    for seed, family, heat in [(31, "gyroid", "quad_core"), (32, "straight", "gaussian_hotspots"),
+                               (33, "schwarz_p", "uniform")]:
+        p = G.family_params(seed, family=family, heat_name=heat)
+        mask, hs = G.design_arrays(p)
+        print(f"features {family} {heat} core {p['core']}", flush=True)
+        ref = S.features(mask, hs, p["u_inlet_val"], p["tau_fluid"], p["heat_power"], (160, 128, 32))
+        m, h = device_inputs(dict(mask=mask, heat=hs))
+        got = _np(S.features_device(m, h, p["u_inlet_val"], p["tau_fluid"], p["heat_power"], (160, 128, 32)))
+        bad = np.argwhere(got != ref)
+        assert bad.size == 0, f"{family}: {len(bad)} feature values differ, first {tuple(bad[0])}"
+
+
+def test_solver_on_device_built_designs_matches_host_designs():
+    """BatchedSolver on parameter-only designs (built on the GPU when a slot starts) gives the same steps,
+    metrics and fields as on the same designs' host arrays."""
+    from xadvcool_amd.batch import BatchedSolver
+    params = [G.config2_params(1000 + i) for i in range(3)] + [G.family_params(41, core=(128, 128, 32),
+                                                                               family="pinfin", heat_name="dual_core")]
+    host, dev = [], []
+    for i, p in enumerate(params):
+        mask, hs = G.design_arrays(p)
+        common = dict(key=i, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"], heat_power=p["heat_power"])
+        host.append(dict(mask=mask, heat=hs, **common))
+        dev.append(dict(params=p, **common))
+    out = []
+    for designs in (host, dev):
+        bs = BatchedSolver(2, CAP, max_steps=600, tol_ke=1e-6, tol_T=1e-6, collect_fields=True)
+        try:
+            out.append({r["key"]: r for r in bs.run(designs)})
+        finally:
+            bs.close()
+    for k in range(len(params)):
+        a, b = out[0][k], out[1][k]
+        print(f"solver design {k}: {a['status']} {a['steps']}", flush=True)
+        assert (a["status"], a["steps"]) == (b["status"], b["steps"])
+        assert a["metrics"] == b["metrics"]
+        for fa, fb in zip(a["fields"], b["fields"]):
+            assert np.array_equal(fa, fb)
+    assert "mask" not in dev[0] and isinstance(host[0]["mask"], np.ndarray)

@@ -14,7 +14,12 @@ the reference's residual rule (src/simulation.py:130-156) per slot:
 
 The queue holds design dicts (``mask``, ``heat``, ``tau_fluid``,
 ``u_inlet_val``, ``heat_power``; optional ``init_fields`` = (rho, u, T) for a
-warm start, and ``key``).
+warm start, and ``key``).  ``mask`` and ``heat`` are host arrays or device
+tensors (int8 / FP32 CUDA); a design with ``params`` (``geometry.family_params``
+or ``config2_params``) and no ``mask`` is built on the GPU when it starts.
+Every slot is loaded from device-resident inputs (xadv_instance_load_device):
+host arrays are uploaded once and their flags, heat source and column table
+are built on the GPU.
 """
 from __future__ import annotations
 
@@ -22,9 +27,35 @@ import time
 from collections import deque
 
 import numpy as np
+import torch
 
 from .engine import STAT_NAMES, LatticeBatch, normalized_heat, thermal_taus
 from .simulation import _Residuals, min_steps_floor
+
+
+def device_inputs(d, batch=None):
+    """(mask, heat) of a design as device tensors (int8, FP32 divided by its maximum), uploading host arrays and
+    building ``params``-only designs on ``batch``; the tensors are stored back into ``d``."""
+    if d.get("mask") is None:
+        from .geometry import device_design
+        if batch is None:
+            raise ValueError("a design without a mask needs a batch to be built on")
+        d["mask"], d["heat"] = device_design(batch, d["params"])
+    mask = d["mask"]
+    if not torch.is_tensor(mask):
+        mask = torch.from_numpy(np.ascontiguousarray(mask, dtype=np.int8)).cuda()
+    heat = d.get("heat")
+    if heat is None:
+        heat = torch.from_numpy(normalized_heat(None, tuple(mask.shape))).cuda()
+    else:
+        if not torch.is_tensor(heat):
+            heat = torch.from_numpy(np.ascontiguousarray(heat, dtype=np.float32)).cuda()
+        # Division by the maximum as src/simulation.py:84-85 does it; by a device tensor, so each cell is an
+        # IEEE division (as numpy's), not a multiplication by a reciprocal.
+        mx = heat.max()
+        heat = torch.where(mx > 0, heat / mx, heat).contiguous()
+    d["mask"], d["heat"] = mask, heat
+    return mask, heat
 
 
 class BatchedSolver:
@@ -62,11 +93,11 @@ class BatchedSolver:
 
     # -- slots ----------------------------------------------------------------
     def _start(self, s, d):
-        mask = d["mask"]
+        d = dict(d)  # the slot's device inputs stay with the slot, not in the caller's design dict
+        mask, heat = device_inputs(d, self.batch)
         ttf, tts = thermal_taus(d["tau_fluid"], self.Pr, self.k_ratio)
-        self.batch.load(s, mask, heat=normalized_heat(d.get("heat"), mask.shape), tau=d["tau_fluid"],
-                        tau_th_fluid=ttf, tau_th_solid=tts, heat_power=d["heat_power"], T_in=self.t_inlet,
-                        rho_out=1.0, u_in=d["u_inlet_val"])
+        self.batch.load_device(s, mask, heat=heat, tau=d["tau_fluid"], tau_th_fluid=ttf, tau_th_solid=tts,
+                               heat_power=d["heat_power"], T_in=self.t_inlet, rho_out=1.0, u_in=d["u_inlet_val"])
         init = d.get("init_fields")
         if init is not None:
             self.batch.init(s, *init)
@@ -76,8 +107,8 @@ class BatchedSolver:
         min_steps = min_steps_floor(nx, self.max_steps, d["u_inlet_val"]) if self.min_steps_rule else 0
         self.state[s] = dict(design=d, steps=0, res=_Residuals(self.every, self.tol_ke, self.tol_T, min_steps, self.stop_rule,
                                                     self.t_inlet),
-                             cells=int(mask.size), t0=time.perf_counter(), history=[], ok=0,
-                             bytes=19 * 4 * 2 * (int(np.count_nonzero(mask != 1)) + int(mask.size)))
+                             cells=int(mask.numel()), t0=time.perf_counter(), history=[], ok=0,
+                             bytes=19 * 4 * 2 * (int((mask != 1).sum()) + int(mask.numel())))
 
     def _finish(self, s, status):
         st = self.state[s]

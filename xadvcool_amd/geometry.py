@@ -404,20 +404,24 @@ def render_heat_program(prog):
 # ------------------------------------------------- design-level helpers ---
 
 def compute_porosity(mask, buf=BUFFER):
-    """generate_dataset.py:71-74."""
+    """generate_dataset.py:71-74 (numpy array or torch tensor)."""
     core = mask[buf:-buf, :, :]
-    return 1.0 - np.sum(core == BC_SOLID) / core.size
+    return 1.0 - int((core == BC_SOLID).sum()) / _numel(core)
+
+
+def _numel(a):
+    return a.numel() if hasattr(a, "numel") else a.size
 
 
 def compute_hydraulic_diameter(mask, buf=BUFFER):
-    """generate_dataset.py:57-68: (D_h, V_fluid, A_wetted)."""
+    """generate_dataset.py:57-68: (D_h, V_fluid, A_wetted); ``mask`` a numpy array or a (device) torch tensor."""
     core = mask[buf:-buf, 1:-1, 4:-2]
     fluid = core != BC_SOLID
     solid = core == BC_SOLID
-    V = int(np.sum(fluid))
-    A = int(np.sum(fluid[1:] & solid[:-1]) + np.sum(solid[1:] & fluid[:-1]) +
-            np.sum(fluid[:, 1:] & solid[:, :-1]) + np.sum(solid[:, 1:] & fluid[:, :-1]) +
-            np.sum(fluid[:, :, 1:] & solid[:, :, :-1]) + np.sum(solid[:, :, 1:] & fluid[:, :, :-1]))
+    V = int(fluid.sum())
+    A = int((fluid[1:] & solid[:-1]).sum()) + int((solid[1:] & fluid[:-1]).sum()) + \
+        int((fluid[:, 1:] & solid[:, :-1]).sum()) + int((solid[:, 1:] & fluid[:, :-1]).sum()) + \
+        int((fluid[:, :, 1:] & solid[:, :, :-1]).sum()) + int((solid[:, :, 1:] & fluid[:, :, :-1]).sum())
     Dh = 4.0 * V / A if A > 0 else float(mask.shape[1])
     return Dh, V, A
 
@@ -507,19 +511,27 @@ def config1_mask():
     return _frame(solid, buf, base=3, top=1)
 
 
-def config2_design(seed, core=(CORE_NX, CORE_NY, CORE_NZ)):
+def config2_params(seed, core=(CORE_NX, CORE_NY, CORE_NZ)):
     """Full-size single solve: gyroid with period in [16, 32] and threshold in [-0.4, 0.4], quad_core heat map,
-    u_in in [0.01, 0.05]; tau in [0.52, 0.56] and P in [0.05, 0.20] (the generator's ranges), all from ``seed``."""
+    u_in in [0.01, 0.05]; tau in [0.52, 0.56] and P in [0.05, 0.20] (the generator's ranges), all from ``seed``.
+    Parameters only, in the form of ``family_params`` (``design_arrays`` / ``device_design`` build the arrays)."""
     rng = np.random.default_rng(seed)
     period = float(rng.uniform(16.0, 32.0))
     thr = float(rng.uniform(-0.4, 0.4))
     u_in = float(rng.uniform(0.01, 0.05))
     tau = float(rng.uniform(0.52, 0.56))
     P = float(rng.uniform(0.05, 0.20))
-    mask = tpms_mask(3, period, thr, core)
-    heat = heat_source_3d(mask.shape, "quad_core", core=core[:2])
-    return dict(mask=mask, heat=heat, tau_fluid=tau, u_inlet_val=u_in, heat_power=P, period=period, threshold=thr,
-                seed=seed, family="gyroid", heat_name="quad_core")
+    return dict(seed=seed, core=tuple(int(c) for c in core), family="gyroid", heat_name="quad_core", tau_fluid=tau,
+                heat_power=P, period=period, threshold=thr, u_inlet_val=u_in)
+
+
+def config2_design(seed, core=(CORE_NX, CORE_NY, CORE_NZ)):
+    """``config2_params`` with the host mask and heat map."""
+    p = config2_params(seed, core)
+    mask, heat = design_arrays(p)
+    return dict(mask=mask, heat=heat, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"],
+                heat_power=p["heat_power"], period=p["period"], threshold=p["threshold"], seed=seed, family="gyroid",
+                heat_name="quad_core")
 
 
 FAMILIES = ("straight", "pinfin", "gyroid", "schwarz_p", "schwarz_d")

@@ -396,6 +396,33 @@ def features(mask, heat, u_in, tau, P, pad_to=None):
     return f
 
 
+def features_device(mask, heat, u_in, tau, P, pad_to=None):
+    """``features`` built on the GPU from device tensors (int8 mask, FP32 heat): the same FP32 values.
+
+    The heat map is divided by a device tensor (an IEEE division per cell, as numpy's) and the coordinate
+    channels are formed on the host with numpy, so every value equals the host builder's."""
+    import torch
+    nx, ny, nz = (int(v) for v in mask.shape)
+    X, Y, Z = pad_to or (round8(nx), round8(ny), round8(nz))
+    dev = mask.device
+    f = torch.zeros((X, Y, Z, CIN), dtype=torch.float32, device=dev)
+    fluid = (mask != BC_SOLID).to(torch.float32)
+    f[:nx, :ny, :nz, 0] = fluid
+    f[:nx, :ny, :nz, 1] = 1.0 - fluid
+    f[:nx, :ny, :nz, 2] = (mask == BC_INLET).to(torch.float32)
+    f[:nx, :ny, :nz, 3] = (mask == BC_OUTLET).to(torch.float32)
+    hm = heat.to(torch.float32)
+    mx = hm.max()
+    f[:nx, :ny, :nz, 4] = torch.where(mx > 0, hm / mx, hm)
+    f[:nx, :ny, :nz, 5] = fluid * float(np.float32(u_in * 30.0))
+    f[:nx, :ny, :nz, 6] = float(np.float32((tau - 0.54) * 50.0))
+    f[:nx, :ny, :nz, 7] = float(np.float32(P * 8.0))
+    f[:nx, :, :, 8] = torch.from_numpy(np.arange(nx, dtype=np.float32) / nx).to(dev)[:, None, None]
+    f[:, :ny, :, 9] = torch.from_numpy(np.arange(ny, dtype=np.float32) / ny).to(dev)[None, :, None]
+    f[:, :, :nz, 10] = torch.from_numpy(np.arange(nz, dtype=np.float32) / nz).to(dev)[None, None, :]
+    return f
+
+
 def targets(mask, rho, u, T, u_in, P, pad_to=None):
     """(X8, Y8, Z8, 5) raw targets and (.., 5) loss weights for one solved design."""
     nx, ny, nz = mask.shape

@@ -27,7 +27,7 @@ import numpy as np
 import torch
 
 from . import surrogate as S
-from .batch import BatchedSolver
+from .batch import BatchedSolver, device_inputs
 from .geometry import HEAT_SOURCE_NAMES, family_design
 
 CORE = (128, 128, 32)  # 148 x 128 x 32 lattices
@@ -54,9 +54,10 @@ def make_designs(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
 
 def _sample(d, fields):
     rho, u, T = fields
-    f = S.features(d["mask"], d["heat"], d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
-    t, w = S.targets(d["mask"], rho, u, T, d["u_inlet_val"], d["heat_power"], PAD)
-    return (torch.from_numpy(f).to(torch.bfloat16).cuda(), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda())
+    mask, heat = device_inputs(dict(d))
+    f = S.features_device(mask, heat, d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
+    t, w = S.targets(mask.cpu().numpy(), rho, u, T, d["u_inlet_val"], d["heat_power"], PAD)
+    return f.to(torch.bfloat16), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda()
 
 
 def generate(designs, slots=64, tol=1e-6, max_steps=60000, log=print, confirm=CONFIRM, stop_rule="reference",
@@ -212,13 +213,13 @@ def predict_fields(net, d):
     post-collision set), so their state is a memory of the initial one: the
     inlet density, hence the pressure level at x = 1 and the converged pressure
     drop, would otherwise depend on the prediction (measured: up to 13 %)."""
-    f = S.features(d["mask"], d["heat"], d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
-    x = torch.from_numpy(f).to(torch.bfloat16).cuda()[None]
+    mask, heat = device_inputs(dict(d))
+    x = S.features_device(mask, heat, d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD).to(torch.bfloat16)[None]
     out = net.forward(x)[0]
-    nx, ny, nz = d["mask"].shape
+    nx, ny, nz = mask.shape
     mean, std = net.norm
     y = out[:nx, :ny, :nz, :S.NOUT] * torch.tensor(std, device="cuda") + torch.tensor(mean, device="cuda")
-    free = torch.from_numpy((d["mask"] != S.BC_SOLID) & (d["mask"] != S.BC_INLET)).cuda()
+    free = (mask != S.BC_SOLID) & (mask != S.BC_INLET)
     u = torch.where(free[..., None], y[..., 0:3] * d["u_inlet_val"], torch.zeros_like(y[..., 0:3]))
     rho = torch.where(free, 1.0 + y[..., 3] * d["u_inlet_val"], torch.ones_like(y[..., 3]))
     T = S.T_IN + y[..., 4] * d["heat_power"]
@@ -284,19 +285,21 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
         predictor = lambda d: predict_fields(net, d)  # noqa: E731
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
+                       metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule)
     queue = []
     t_inf = 0.0
     for d in designs:
         dd = dict(d)
         if predictor is not None:
+            if dd.get("mask") is None:  # parameters only: build the design on the GPU for the prediction
+                device_inputs(dd, bs.batch)
             ti = time.perf_counter()
-            dd["init_fields"] = predictor(d)
+            dd["init_fields"] = predictor(dd)
             torch.cuda.synchronize()
             t_inf += time.perf_counter() - ti
             dd["warm"] = True
         queue.append(dd)
-    bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
-                       metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule)
     if timer is not None:  # events go on the solver's own stream
         timer.stream = bs.batch.stream()
     final = {}
