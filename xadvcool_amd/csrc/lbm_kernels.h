@@ -987,21 +987,47 @@ __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, in
   }
 }
 
-// One block per slot: fixed-order reduction of the per-block partials.
-template <class R>
-__global__ void k_stats_final(const Slot<R>* __restrict__ slots, int bps, const double* __restrict__ partials,
-                              double* __restrict__ out) {
-  const int si = blockIdx.x;
-  const int k = threadIdx.x;
-  if (k >= 16) return;
-  double a = 0.0;
-  if (slots[si].active) {
-    for (int j = 0; j < bps; ++j) {
-      const double w = partials[((int64_t)si * bps + j) * 16 + k];
-      a = (k == 0 || k == 11) ? fmax(a, w) : a + w;
+// Fixed-order fold of nrows rows of NV doubles (row-major) by one block of
+// kFoldThreads threads: the block stages kFoldRows rows at a time in LDS with
+// coalesced loads, then thread k < NV folds them row after row in index order
+// (op 0 sum, 1 max, 2 min), so each result is the same sequence of operations
+// as one thread walking the rows in global memory, without its latency-bound
+// chain of dependent loads.
+constexpr int kFoldThreads = 256;
+constexpr int kFoldRows = 256;
+
+template <int NV>
+__device__ double ordered_fold(const double* __restrict__ rows, int nrows, int op, double a, double* lds) {
+  const int t = threadIdx.x;
+  for (int j0 = 0; j0 < nrows; j0 += kFoldRows) {
+    const int nr = min(kFoldRows, nrows - j0);
+    __syncthreads();  // the previous chunk has been folded
+    for (int i = t; i < nr * NV; i += kFoldThreads) lds[i] = rows[(int64_t)j0 * NV + i];
+    __syncthreads();
+    if (t < NV) {
+      for (int j = 0; j < nr; ++j) {
+        const double w = lds[j * NV + t];
+        a = op == 0 ? a + w : (op == 1 ? fmax(a, w) : fmin(a, w));
+      }
     }
   }
-  out[si * 16 + k] = a;
+  return a;
+}
+
+// One block per slot: fixed-order reduction of the per-block partials.
+template <class R>
+__global__ __launch_bounds__(kFoldThreads) void k_stats_final(const Slot<R>* __restrict__ slots, int bps,
+                                                              const double* __restrict__ partials,
+                                                              double* __restrict__ out) {
+  __shared__ double lds[kFoldRows * 16];
+  const int si = blockIdx.x;
+  const int k = threadIdx.x;
+  if (!slots[si].active) {  // uniform per block
+    if (k < 16) out[si * 16 + k] = 0.0;
+    return;
+  }
+  const double a = ordered_fold<16>(partials + (int64_t)si * bps * 16, bps, (k == 0 || k == 11) ? 1 : 0, 0.0, lds);
+  if (k < 16) out[si * 16 + k] = a;
 }
 
 // --------------------------------------------------------------- metrics ---
@@ -1131,28 +1157,24 @@ __global__ __launch_bounds__(256) void k_checks(Slot<R>* __restrict__ slots, int
   block_reduce<kNChk>(v, op, partials + (int64_t)blockIdx.x * kNChk);
 }
 
-__global__ void k_checks_final(int nblocks, const double* __restrict__ partials, double* __restrict__ out) {
+__global__ __launch_bounds__(kFoldThreads) void k_checks_final(int nblocks, const double* __restrict__ partials,
+                                                               double* __restrict__ out) {
+  __shared__ double lds[kFoldRows * kNChk];
   const int k = threadIdx.x;
-  if (k >= kNChk) return;
-  const int o = kChkOp[k];
-  double a = o == 1 ? -INFINITY : (o == 2 ? INFINITY : 0.0);
-  for (int j = 0; j < nblocks; ++j) {
-    const double w = partials[(int64_t)j * kNChk + k];
-    a = o == 0 ? a + w : (o == 1 ? fmax(a, w) : fmin(a, w));
-  }
-  out[k] = a;
+  const int o = k < kNChk ? kChkOp[k] : 0;
+  const double a0 = o == 1 ? -INFINITY : (o == 2 ? INFINITY : 0.0);
+  const double a = ordered_fold<kNChk>(partials, nblocks, o, a0, lds);
+  if (k < kNChk) out[k] = a;
 }
 
-__global__ void k_metrics_final(int nblocks, const double* __restrict__ partials, double* __restrict__ out) {
+__global__ __launch_bounds__(kFoldThreads) void k_metrics_final(int nblocks, const double* __restrict__ partials,
+                                                                double* __restrict__ out) {
+  __shared__ double lds[kFoldRows * kNMet];
   const int k = threadIdx.x;
-  if (k >= kNMet) return;
-  const int o = kMetOp[k];
-  double a = o == 1 ? -INFINITY : (o == 2 ? INFINITY : 0.0);
-  for (int j = 0; j < nblocks; ++j) {
-    const double w = partials[(int64_t)j * kNMet + k];
-    a = o == 0 ? a + w : (o == 1 ? fmax(a, w) : fmin(a, w));
-  }
-  out[k] = a;
+  const int o = k < kNMet ? kMetOp[k] : 0;
+  const double a0 = o == 1 ? -INFINITY : (o == 2 ? INFINITY : 0.0);
+  const double a = ordered_fold<kNMet>(partials, nblocks, o, a0, lds);
+  if (k < kNMet) out[k] = a;
 }
 
 }  // namespace xadv

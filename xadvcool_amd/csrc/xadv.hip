@@ -811,10 +811,10 @@ int xadv_readout(xadv_batch* b, int with_stats) {
   HIPCHK(hipGetLastError());
   if (with_stats) {
     if (b->d.precision == 64)
-      hipLaunchKernelGGL((xadv::k_stats_final<double>), dim3(b->d.slots), dim3(64), 0, b->stream,
+      hipLaunchKernelGGL((xadv::k_stats_final<double>), dim3(b->d.slots), dim3(xadv::kFoldThreads), 0, b->stream,
                          reinterpret_cast<const Slot<double>*>(b->slots_dev), b->bps_r, b->partials, b->stats_dev);
     else
-      hipLaunchKernelGGL((xadv::k_stats_final<float>), dim3(b->d.slots), dim3(64), 0, b->stream,
+      hipLaunchKernelGGL((xadv::k_stats_final<float>), dim3(b->d.slots), dim3(xadv::kFoldThreads), 0, b->stream,
                          reinterpret_cast<const Slot<float>*>(b->slots_dev), b->bps_r, b->partials, b->stats_dev);
     HIPCHK(hipGetLastError());
   }
@@ -863,7 +863,8 @@ int xadv_metrics(xadv_batch* b, int slot, int core_x0, int core_nx, int core_ny,
   else
     hipLaunchKernelGGL((xadv::k_metrics<float>), dim3(nb), dim3(256), 0, b->stream,
                        reinterpret_cast<Slot<float>*>(b->slots_dev), slot, core_x0, core_nx, core_ny, b->met_partials);
-  hipLaunchKernelGGL(xadv::k_metrics_final, dim3(1), dim3(64), 0, b->stream, nb, b->met_partials, b->met_dev);
+  hipLaunchKernelGGL(xadv::k_metrics_final, dim3(1), dim3(xadv::kFoldThreads), 0, b->stream, nb, b->met_partials,
+                     b->met_dev);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(b->host_small, b->met_dev, xadv::kNMet * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
@@ -884,7 +885,8 @@ int xadv_checks(xadv_batch* b, int slot, double k_fluid, double k_solid, double*
   else
     hipLaunchKernelGGL((xadv::k_checks<float>), dim3(nb), dim3(256), 0, b->stream,
                        reinterpret_cast<Slot<float>*>(b->slots_dev), slot, k_fluid, k_solid, b->met_partials);
-  hipLaunchKernelGGL(xadv::k_checks_final, dim3(1), dim3(64), 0, b->stream, nb, b->met_partials, b->met_dev);
+  hipLaunchKernelGGL(xadv::k_checks_final, dim3(1), dim3(xadv::kFoldThreads), 0, b->stream, nb, b->met_partials,
+                     b->met_dev);
   HIPCHK(hipGetLastError());
   HIPCHK(hipMemcpyAsync(b->host_small, b->met_dev, xadv::kNChk * sizeof(double), hipMemcpyDeviceToHost, b->stream));
   HIPCHK(hipStreamSynchronize(b->stream));
