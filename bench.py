@@ -126,7 +126,7 @@ def cpu_baseline(design, seconds, threads):
     return mask.size * steps / dt / 1e6, steps, dt
 
 
-def profile_passes(npasses, slots=1, seed=1000):
+def profile_passes(npasses, slots=1, seed=1000, **launch):
     """Only fused passes on ``slots`` config-2 designs (seeds from ``seed``; graph chunks of 100, fewer when
     npasses is smaller), for kernel traces and counters.  One slot (~140 MB touched per pass) stays in the
     256 MB Infinity Cache; many slots are HBM-resident."""
@@ -135,7 +135,7 @@ def profile_passes(npasses, slots=1, seed=1000):
     from xadvcool_amd.geometry import config2_design
     designs = [config2_design(seed + i) for i in range(slots)]
     chunk = 100 if npasses >= 100 else max(2, npasses - npasses % 2)
-    batch = LatticeBatch(slots=slots, cap_shape=LATTICE, graph_chunk=chunk)
+    batch = LatticeBatch(slots=slots, cap_shape=LATTICE, graph_chunk=chunk, **launch)
     for s, d in enumerate(designs):
         load_design(batch, d, slot=s)
     batch.step(200, want_fields=False)  # past the start-up transient
@@ -150,7 +150,7 @@ def profile_passes(npasses, slots=1, seed=1000):
         done += n
     ms = timer.total_ms() / max(done, 1)
     nb = sum(algorithmic_bytes_per_pass(d["mask"]) for d in designs)
-    print(json.dumps({"mode": "passes", "slots": slots, "passes": done, "avg_pass_us": round(1000 * ms, 2),
+    print(json.dumps({"mode": "passes", "slots": slots, **launch, "passes": done, "avg_pass_us": round(1000 * ms, 2),
                       "alg_bytes_per_pass": nb, "achieved_GBs": round(nb / (ms * 1e-3) / 1e9, 1)}), flush=True)
     batch.close()
 
@@ -340,6 +340,14 @@ def main():
     ap.add_argument("--copy-mb", type=float, default=512.0)
     ap.add_argument("--designs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=None, help="batch slots (default: 1 for passes, 64 for batch)")
+    ap.add_argument("--launch", choices=["tune", "fixed"], default="tune",
+                    help="solves mode: autotune the fused pass's launch shape before the timed arms, or use the "
+                         "--block/--cpt/--xcd/--sweep flags")
+    ap.add_argument("--sweep", choices=["alternate", "forward"], default="alternate",
+                    help="block order of the fused pass: alternate reverses every even pass (Infinity-Cache reuse)")
+    ap.add_argument("--block", type=int, default=256, help="threads per block of the fused pass")
+    ap.add_argument("--cpt", type=int, default=1, help="lattice cells per thread of the fused pass")
+    ap.add_argument("--xcd", action="store_true", help="map consecutive lattice blocks to one XCD")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
     args = ap.parse_args()
 
@@ -349,7 +357,8 @@ def main():
         sys.exit(2)
 
     if args.mode == "passes":
-        return profile_passes(args.passes, args.slots or 1)
+        return profile_passes(args.passes, args.slots or 1, block=args.block, cells_per_thread=args.cpt,
+                              xcd_remap=args.xcd, alternate_sweep=args.sweep == "alternate")
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
     if args.mode == "ceilings":
@@ -390,7 +399,21 @@ def main():
     # measured best per-design pass time (partly Infinity-Cache resident; one slot leaves the chip underfed,
     # four fall out of the cache).
     nslots = args.slots or 3
-    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps,
+    launch = dict(block=args.block, cells_per_thread=args.cpt, xcd_remap=args.xcd,
+                  alternate_sweep=args.sweep == "alternate")
+    tuned_us = None
+    if args.launch == "tune":
+        # Untimed set-up, like a compile step: the project's autotuner times each launch shape on a batch of
+        # ``nslots`` full-size designs (seeds outside the timed set) and both arms use the fastest.  Every shape
+        # computes the same bits.
+        from xadvcool_amd import autotune
+        tune_designs = [config2_design(sd) for sd in range(900, 900 + nslots)]
+        best, times = autotune.tune(tune_designs, LATTICE, passes=300, save=False,
+                                    candidates=[c for c in autotune.CANDIDATES if c["block"] >= 128])
+        launch = {k: best[k] for k in ("block", "cells_per_thread", "xcd_remap", "alternate_sweep")}
+        tuned_us = round(times[json.dumps(best, sort_keys=True)], 2)
+        print(f"[bench] autotuned launch {launch}: {tuned_us} us/pass", file=sys.stderr, flush=True)
+    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps, launch=launch,
               log=lambda m: print(m, file=sys.stderr, flush=True))  # progress to stderr; stdout is the result line
     if args.warmup:
         w = designs(range(900, 900 + args.warmup))
@@ -438,7 +461,8 @@ def main():
                                "cold then warm-started",
                    "lattice": list(LATTICE), "tol": args.tol, "max_steps": args.max_steps,
                    "stop_rule": "reference d_ke/dT/dQ residuals every 100 steps, flow-through floor off",
-                   "seeds": seeds},
+                   "seeds": seeds, "designs_per_launch": nslots,
+                   "launch": dict(launch, autotuned=args.launch == "tune", tuned_us_per_pass=tuned_us)},
         "cold_start": {"solves_per_hr": round(c_rate, 2), "converged": c_conv, "wall_s": round(cold["wall"], 3),
                        "steps": c_steps, "mlups": round(cells * sum(c_steps) / cold["wall"] / 1e6, 1)},
         "warm_start": {"solves_per_hr": round(w_rate, 2), "converged": w_conv, "wall_s": round(warm["wall"], 3),

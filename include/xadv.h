@@ -58,6 +58,9 @@ typedef struct {
                            XADV_STORAGE_DENSE (every cell) or XADV_STORAGE_COMPACT (non-solid
                            cells only: fluid on, general 0, every lattice nz <= 32).  Results are
                            the same bits either way; compact moves fewer bytes per step. */
+  int alternate_sweep;  /* 1: even passes visit the lattice blocks in reverse order, so a pass
+                           starts on the lines the previous pass touched last (still in the
+                           Infinity Cache when the batch's state is larger than it).  Same bits. */
 } xadv_batch_desc;
 
 #define XADV_STORAGE_AUTO 0

@@ -57,4 +57,7 @@ def test_bad_arguments_are_errors_not_crashes():
     h = ctypes.c_void_p()
     assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
     assert b"precision" in lib.xadv_last_error()
+    d = _lib.BatchDesc(32, 1, 1, 0, 1, 4, 4, 4, 256, 0, 0, 0, 0, 2)  # alternate_sweep is a 0/1 flag
+    assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
+    assert b"alternate_sweep" in lib.xadv_last_error()
     assert lib.xadv_step(None, 1, 0) == _lib.XADV_E_ARG

@@ -24,13 +24,15 @@ def designs():
     return out
 
 
-def test_batch_matches_single_solves(gpu):
+@pytest.mark.parametrize("alternate_sweep", [False, True])
+def test_batch_matches_single_solves(gpu, alternate_sweep):
     from xadvcool_amd.batch import BatchedSolver
     from xadvcool_amd.simulation import run_simulation
     ds = designs()
     cap = tuple(max(d["mask"].shape[k] for d in ds) for k in range(3))
-    print(f"[batch] {len(ds)} designs in 3 slots, cap {cap}", flush=True)
-    bs = BatchedSolver(3, cap, tol_ke=1e-5, tol_T=1e-5, max_steps=3000, collect_fields=True, metrics=False)
+    print(f"[batch] {len(ds)} designs in 3 slots, cap {cap}, alternate_sweep={alternate_sweep}", flush=True)
+    bs = BatchedSolver(3, cap, tol_ke=1e-5, tol_T=1e-5, max_steps=3000, collect_fields=True, metrics=False,
+                       alternate_sweep=alternate_sweep)
     try:
         res = {r["key"]: r for r in bs.run(ds)}
     finally:

@@ -106,15 +106,22 @@ def test_coupled_fp32_matches_oracle(gpu, steps, storage):
     assert_match("T", T, ref.T)
 
 
-@pytest.mark.parametrize("block,cpt,xcd,storage", [(64, 1, False, "compact"), (128, 2, True, "compact"),
-                                                   (256, 1, True, "compact"), (64, 2, False, "compact"),
-                                                   (128, 2, True, "dense"), (256, 1, False, "dense")])
-def test_coupled_launch_shapes(gpu, block, cpt, xcd, storage):
-    """Every launch shape the autotuner may pick computes the same bits."""
-    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd} storage={storage}", flush=True)
+@pytest.mark.parametrize("block,cpt,xcd,alt,storage", [(64, 1, False, False, "compact"),
+                                                       (128, 2, True, False, "compact"),
+                                                       (256, 1, True, False, "compact"),
+                                                       (64, 2, False, False, "compact"),
+                                                       (128, 2, True, False, "dense"),
+                                                       (256, 1, False, False, "dense"),
+                                                       (256, 1, False, True, "compact"),
+                                                       (64, 2, True, True, "compact"),
+                                                       (128, 1, False, True, "dense")])
+def test_coupled_launch_shapes(gpu, block, cpt, xcd, alt, storage):
+    """Every launch shape the autotuner may pick computes the same bits, in either block order."""
+    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd} alt={alt} storage={storage}", flush=True)
     p = coupled_params(seed=2, shape=(23, 13, 11))
     ref = oracle_coupled(p, 13, np.float32)
-    r, u, T = hip_coupled(p, 13, 32, block=block, cells_per_thread=cpt, xcd_remap=xcd, fluid_storage=storage)
+    r, u, T = hip_coupled(p, 13, 32, block=block, cells_per_thread=cpt, xcd_remap=xcd, alternate_sweep=alt,
+                          fluid_storage=storage)
     assert_match("rho", r, ref.rho)
     assert_match("u", u, ref.u)
     assert_match("T", T, ref.T)

@@ -1,12 +1,19 @@
 """Launch-shape autotuner for the fused pass.
 
 Candidates: threads per block (64, 128, 256) x lattice cells per thread
-(1, 2) x XCD-contiguous block mapping (on, off).  Each candidate is timed
-with HIP events on the solver's stream over graph-captured passes of the
-real lattice (or batch); the fastest shape is cached per (lattice shapes,
-device) in a JSON file next to this module (XADV_TUNE_CACHE overrides the
-path).  The shapes change only performance: every candidate computes the
-same bits (the parity tests run all of them).
+(1, 2) x XCD-contiguous block mapping (on, off) x alternating sweep
+direction (even passes visit the blocks in reverse order: on, off).  Each
+candidate is timed with HIP events on the solver's stream over
+graph-captured passes of the real lattice (or batch); the fastest shape is
+cached per (lattice shapes, device) in a JSON file next to this module
+(XADV_TUNE_CACHE overrides the path).  The shapes change only performance:
+every candidate computes the same bits (the parity tests run them).
+
+Which shape wins depends on the working set: with a batch state of a few
+hundred MB (partly resident in the 256 MB Infinity Cache) the XCD mapping
+keeps neighbour lines in one XCD's L2 and wins; with many GB in HBM the
+linear order does.  The alternating sweep starts each pass on the lines the
+previous pass touched last, so it never loses.
 """
 from __future__ import annotations
 
@@ -14,9 +21,9 @@ import itertools
 import json
 import os
 
-CANDIDATES = [dict(block=b, cells_per_thread=c, xcd_remap=x)
-              for b, c, x in itertools.product((64, 128, 256), (1, 2), (True, False))]
-DEFAULT = dict(block=256, cells_per_thread=2, xcd_remap=True)
+CANDIDATES = [dict(block=b, cells_per_thread=c, xcd_remap=x, alternate_sweep=a)
+              for b, c, x, a in itertools.product((64, 128, 256), (1, 2), (True, False), (True, False))]
+DEFAULT = dict(block=256, cells_per_thread=1, xcd_remap=False, alternate_sweep=True)  # the engine's defaults
 _CACHE = os.environ.get("XADV_TUNE_CACHE", os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_cache.json"))
 
 

@@ -70,12 +70,16 @@ struct Slot {
   R* T;
 };
 
-__device__ __forceinline__ int map_block(int G, int xcd) {
+// Block order: bit 0 maps consecutive lattice blocks to one XCD (the grid is
+// padded to a multiple of 8), bit 1 reverses the visiting order.  Both are
+// bijections on [0, gridDim.x); indices >= G belong to no lattice block.
+__device__ __forceinline__ int map_block(int G, int order) {
   int b = blockIdx.x;
-  if (xcd) {
+  if (order & 1) {
     const int per = (G + 7) >> 3;
     b = (b & 7) * per + (b >> 3);
   }
+  if (order & 2) b = (int)gridDim.x - 1 - b;
   return b;
 }
 
@@ -652,11 +656,11 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // Cell k of a thread is block_base + k*blockDim + threadIdx, so every load
 // instruction of a wavefront covers 64 consecutive cells.
 template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP>
-__global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int xcd,
+__global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int order,
                                                              int want_fields) {
   static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
   constexpr bool PRE = XADV_TH_PREFETCH && FL && TH;
-  const int b = map_block(G, xcd);
+  const int b = map_block(G, order);
   if (b >= G) return;
   const int si = b / bps;
   const int lb = b - si * bps;
@@ -913,9 +917,9 @@ __constant__ int kStatOp[16] = {1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0};
 // T of the current step (thermal streaming + moments) and the 16 stat slots
 // of src/fluid_engine3d.py:283-312 and src/thermal_engine3d.py:234-243.
 template <class R, int FL, int TH, int GEN, int NODD>
-__global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, int bps, int G, int xcd,
+__global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, int bps, int G, int order,
                                                  int with_stats, double* __restrict__ partials) {
-  const int b = map_block(G, xcd);
+  const int b = map_block(G, order);
   if (b >= G) return;
   const int si = b / bps;
   const int lb = b - si * bps;

@@ -110,12 +110,13 @@ template <class R, int FL, int TH, int GEN, int CPT, int CMP>
 void launch_pass_c(xadv_batch* b, int nodd, int want_fields) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
   dim3 grid(b->G_launch), block(b->d.block);
+  const int order = (b->d.xcd_remap ? 1 : 0) | ((b->d.alternate_sweep && !nodd) ? 2 : 0);
   if (nodd)
     hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT, CMP>), grid, block, 0, b->stream, sl, b->bps, b->G,
-                       b->d.xcd_remap, want_fields);
+                       order, want_fields);
   else
     hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT, CMP>), grid, block, 0, b->stream, sl, b->bps, b->G,
-                       b->d.xcd_remap, want_fields);
+                       order, want_fields);
 }
 
 // Compact storage exists for the channel kernels with the fluid lattice on.
@@ -504,6 +505,8 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   if (d.graph_chunk < 0 || (d.graph_chunk % 2)) return fail(XADV_E_ARG, "graph_chunk must be even and >= 0");
   if (d.cells_per_thread == 0) d.cells_per_thread = 2;
   if (d.cells_per_thread != 1 && d.cells_per_thread != 2) return fail(XADV_E_ARG, "cells_per_thread must be 1 or 2");
+  if (d.xcd_remap != 0 && d.xcd_remap != 1) return fail(XADV_E_ARG, "xcd_remap must be 0 or 1");
+  if (d.alternate_sweep != 0 && d.alternate_sweep != 1) return fail(XADV_E_ARG, "alternate_sweep must be 0 or 1");
   if (d.fluid_storage < XADV_STORAGE_AUTO || d.fluid_storage > XADV_STORAGE_COMPACT)
     return fail(XADV_E_ARG, "fluid_storage must be XADV_STORAGE_AUTO, _DENSE or _COMPACT");
   const bool can_compact = d.fluid && !d.general;

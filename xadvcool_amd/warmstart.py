@@ -275,18 +275,21 @@ class _Stall:
 
 
 def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None,
-                  confirm=CONFIRM, stop_rule="reference"):
+                  confirm=CONFIRM, stop_rule="reference", launch=None):
     """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
 
     Fallback: a warm design that stalls or diverges is re-run cold in the same call.  Both arms stop when the
     reference's residual test passes on ``confirm`` consecutive checks.  ``timer`` (optional):
-    ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes; its ``stream`` is set to the solver's."""
+    ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes; its ``stream`` is set to the solver's.
+    ``launch``: launch-shape options of the fused pass (``block``, ``cells_per_thread``, ``xcd_remap``,
+    ``alternate_sweep``); they change timing only."""
     if predictor is None and net is not None:
         predictor = lambda d: predict_fields(net, d)  # noqa: E731
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
-                       metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule)
+                       metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule,
+                       **(launch or {}))
     queue = []
     t_inf = 0.0
     for d in designs:
