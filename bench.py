@@ -419,9 +419,9 @@ def main():
         w = designs(range(900, 900 + args.warmup))
         W.solve_designs(w, None, **kw)
         W.solve_designs(w, net, **kw)
-    timer = PassTimer(None)
+    timer = PassTimer(None)  # every pass of both arms (the population a kernel trace of this command averages)
     cold = W.solve_designs(timed, None, timer=timer, **kw)
-    warm = W.solve_designs(timed, net, **kw)
+    warm = W.solve_designs(timed, net, timer=timer, **kw)
 
     def arm(r):
         res = [r["results"][sd] for sd in seeds]
@@ -483,9 +483,10 @@ def main():
                                         f"x{counters['write_scale']} per pass / algorithmic bytes = "
                                         f"{counters['traffic_per_alg_byte']:.3f} on {counters['workload']} "
                                         f"({COUNTERS_FILE})" if counters else None),
-                     "note": f"{nslots} lattices per launch (~140 MB touched per lattice per pass): partly "
-                             "resident in the 256 MB Infinity Cache; the all-HBM figure is bench.py --mode "
-                             "passes --slots 64"},
+                     "passes_timed": npass,
+                     "note": f"HIP events on the solver's stream around every graph chunk of both arms; up to "
+                             f"{nslots} lattices per launch (~140 MB touched per lattice per pass): partly resident "
+                             "in the 256 MB Infinity Cache; the all-HBM figure is bench.py --mode passes --slots 64"},
     }
     if not args.no_cpu:
         d0 = dict(config2_design(seeds[0]), key=seeds[0])

@@ -51,7 +51,8 @@ typedef struct {
   int slots;            /* number of lattice slots */
   int cap_nx, cap_ny, cap_nz;  /* capacity of each slot (largest lattice it can hold) */
   int block;            /* threads per block: 64, 128 or 256; 0 = default (256) */
-  int xcd_remap;        /* 1: map consecutive lattice blocks to one XCD */
+  int xcd_remap;        /* 1: XCD-contiguous block order (each of the 8 XCDs takes one eighth of
+                           every slot, so neighbour lines stay in its L2); 0: linear order */
   int graph_chunk;      /* steps captured per hipGraph (even, >= 2); 0 = no graphs */
   int cells_per_thread; /* lattice cells per thread in the fused pass: 1 or 2; 0 = default (2) */
   int fluid_storage;    /* fluid populations: XADV_STORAGE_AUTO (compact where it can be used),

@@ -6,14 +6,18 @@ direction (even passes visit the blocks in reverse order: on, off).  Each
 candidate is timed with HIP events on the solver's stream over
 graph-captured passes of the real lattice (or batch); the fastest shape is
 cached per (lattice shapes, device) in a JSON file next to this module
-(XADV_TUNE_CACHE overrides the path).  The shapes change only performance:
-every candidate computes the same bits (the parity tests run them).
+(XADV_TUNE_CACHE overrides the path).  Each candidate is scored with every
+slot busy and with one slot idle (time_shape), as in a batched run where
+slots retire.  The shapes change only performance: every candidate computes
+the same bits (the parity tests run them).
 
-Which shape wins depends on the working set: with a batch state of a few
-hundred MB (partly resident in the 256 MB Infinity Cache) the XCD mapping
-keeps neighbour lines in one XCD's L2 and wins; with many GB in HBM the
-linear order does.  The alternating sweep starts each pass on the lines the
-previous pass touched last, so it never loses.
+The alternating sweep starts each pass on the lines the previous pass
+touched last (still in the Infinity Cache when the batch's state is larger
+than it); on 148x128x32 gyroid designs it was never slower.  The
+XCD-contiguous order gives each XCD one eighth of every slot, so the
+heavier eighths (the all-fluid inlet and outlet buffers are in the first and
+last) always land on the same two XCDs: with three such designs per launch
+it measured 79.0 us per pass against 75.3 us in linear order.
 """
 from __future__ import annotations
 
@@ -45,8 +49,12 @@ def cached(shapes, precision=32, fluid=True, thermal=True):
     return load_cache().get(_key(shapes, precision, fluid, thermal))
 
 
-def time_shape(designs, cap_shape, shape, passes=200, precision=32):
-    """Average pass time (us) of one launch shape on a batch holding ``designs`` (list of dicts with mask/heat)."""
+def time_shape(designs, cap_shape, shape, passes=200, precision=32, idle_phase=True):
+    """Average pass time (us) of one launch shape on a batch holding ``designs`` (list of dicts with mask/heat).
+
+    With ``idle_phase`` (and more than one design) the shape is timed twice, with every slot busy and with the
+    last slot idle, and the mean is returned: in a real batched run slots retire and wait for their next
+    design, so an order that parks a slot's work on some XCDs only (and leaves them idle with it) must lose."""
     import torch
     from .engine import LatticeBatch
     b = LatticeBatch(slots=len(designs), cap_shape=cap_shape, graph_chunk=100, precision=precision, **shape)
@@ -57,12 +65,17 @@ def time_shape(designs, cap_shape, shape, passes=200, precision=32):
             b.init(s)
         b.step(100, want_fields=False)
         st = b.stream()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record(st)
-        b.step(passes, want_fields=False)
-        e1.record(st)
-        torch.cuda.synchronize()
-        return 1000.0 * e0.elapsed_time(e1) / passes
+        times = []
+        for phase in range(2 if idle_phase and len(designs) > 1 else 1):
+            if phase == 1:
+                b.set_active(len(designs) - 1, False)
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            b.step(passes, want_fields=False)
+            e1.record(st)
+            torch.cuda.synchronize()
+            times.append(1000.0 * e0.elapsed_time(e1) / passes)
+        return sum(times) / len(times)
     finally:
         b.close()
 

@@ -97,6 +97,31 @@ __host__ __device__ inline uint32_t div_nz(uint32_t c, uint32_t nz, uint32_t mag
   return nz <= 1 ? c : (uint32_t)(((uint64_t)c * magic) >> 32);
 }
 
+// Block order of the fused pass and the readout.  A batch of S slots has bps
+// lattice blocks per slot; lattice block s * bps + lb covers cells
+// [lb * cells_per_block, ...) of slot s.  launch_block(b, grid, bps, order)
+// gives the lattice block that launch block b (of `grid`) works on, or a value
+// >= S * bps for a padding block that does nothing:
+//  * order bit 1 reverses the visiting order (grid - 1 - b);
+//  * order bit 0 is the XCD-contiguous order.  The hardware hands launch block
+//    b to XCD b % 8, so XCD x takes the x-th eighth of every slot (per =
+//    ceil(bps / 8) blocks of it), slot after slot.  A cell's x +- 1 and
+//    y +- 1 neighbours then mostly lie in blocks on the same XCD and its L2,
+//    and a slot that is idle takes an equal share of work from every XCD.
+//    The grid is 8 * per * S blocks.
+// Both orders are bijections from the non-padding launch blocks onto the
+// lattice blocks (tests/block_order_check.cpp checks it exhaustively).
+__host__ __device__ inline int xcd_grid(int bps, int slots) { return 8 * ((bps + 7) >> 3) * slots; }
+__host__ __device__ inline int launch_block(int b, int grid, int bps, int order) {
+  if (order & 2) b = grid - 1 - b;
+  if (!(order & 1)) return b;
+  const int per = (bps + 7) >> 3;
+  const int j = b >> 3;  // position in the XCD's sequence
+  const int s = j / per;
+  const int lb = (b & 7) * per + (j - s * per);
+  return lb < bps ? s * bps + lb : 0x7fffffff;
+}
+
 // Column entry q + rel of a padded table (cols points at lattice column 0).
 __host__ __device__ inline ColInfo load_col(const ColInfo* cols, int rel, uint32_t q) {
 #if defined(__HIP_DEVICE_COMPILE__)

@@ -33,8 +33,9 @@
 //  * Solid cells never run the fluid half (their populations are frozen, as
 //    in the reference) but do run the thermal half (conjugate heat transfer).
 //  * A batch is an array of slots; each block belongs to one slot, so lattices
-//    of different sizes advance in one launch.  Blocks can be remapped so
-//    that consecutive lattice blocks share an XCD (and its L2).
+//    of different sizes advance in one launch.  The block order is linear or
+//    XCD-contiguous (each XCD takes one eighth of every slot), forward or
+//    reversed on even passes (lattice.h, launch_block).
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -70,17 +71,10 @@ struct Slot {
   R* T;
 };
 
-// Block order: bit 0 maps consecutive lattice blocks to one XCD (the grid is
-// padded to a multiple of 8), bit 1 reverses the visiting order.  Both are
-// bijections on [0, gridDim.x); indices >= G belong to no lattice block.
-__device__ __forceinline__ int map_block(int G, int order) {
-  int b = blockIdx.x;
-  if (order & 1) {
-    const int per = (G + 7) >> 3;
-    b = (b & 7) * per + (b >> 3);
-  }
-  if (order & 2) b = (int)gridDim.x - 1 - b;
-  return b;
+// Lattice block of this launch block (block order bits: lattice.h,
+// launch_block); values >= G are padding blocks.
+__device__ __forceinline__ int map_block(int bps, int order) {
+  return launch_block((int)blockIdx.x, (int)gridDim.x, bps, order);
 }
 
 // e_i . u with the zero components left out.  The reference forms
@@ -660,7 +654,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
                                                              int want_fields) {
   static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
   constexpr bool PRE = XADV_TH_PREFETCH && FL && TH;
-  const int b = map_block(G, order);
+  const int b = map_block(bps, order);
   if (b >= G) return;
   const int si = b / bps;
   const int lb = b - si * bps;
@@ -919,7 +913,7 @@ __constant__ int kStatOp[16] = {1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0};
 template <class R, int FL, int TH, int GEN, int NODD>
 __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, int bps, int G, int order,
                                                  int with_stats, double* __restrict__ partials) {
-  const int b = map_block(G, order);
+  const int b = map_block(bps, order);
   if (b >= G) return;
   const int si = b / bps;
   const int lb = b - si * bps;

@@ -529,10 +529,10 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   const int cpb = d.block * d.cells_per_thread;
   b->bps = (int)((b->cap_cells + cpb - 1) / cpb);
   b->G = b->bps * d.slots;
-  b->G_launch = d.xcd_remap ? ((b->G + 7) / 8) * 8 : b->G;
+  b->G_launch = d.xcd_remap ? xadv::xcd_grid(b->bps, d.slots) : b->G;
   b->bps_r = (int)((b->cap_cells + d.block - 1) / d.block);
   b->G_r = b->bps_r * d.slots;
-  b->G_r_launch = d.xcd_remap ? ((b->G_r + 7) / 8) * 8 : b->G_r;
+  b->G_r_launch = d.xcd_remap ? xadv::xcd_grid(b->bps_r, d.slots) : b->G_r;
   b->slot_bytes = d.precision == 64 ? sizeof(Slot<double>) : sizeof(Slot<float>);
   b->loaded.assign(d.slots, 0);
   b->inited.assign(d.slots, 0);
