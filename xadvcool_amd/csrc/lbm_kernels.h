@@ -990,20 +990,46 @@ __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, in
 // coalesced loads, then thread k < NV folds them row after row in index order
 // (op 0 sum, 1 max, 2 min), so each result is the same sequence of operations
 // as one thread walking the rows in global memory, without its latency-bound
-// chain of dependent loads.
+// chain of dependent loads.  Each thread issues all of its loads of a chunk
+// before it stores any (the partials were written by blocks on every XCD, so
+// each load goes out to memory), and the next chunk's loads are in flight
+// while the current one is folded.
 constexpr int kFoldThreads = 256;
 constexpr int kFoldRows = 256;
 
 template <int NV>
 __device__ double ordered_fold(const double* __restrict__ rows, int nrows, int op, double a, double* lds) {
+  constexpr int U = kFoldRows * NV / kFoldThreads;  // values per thread and chunk
+  static_assert(kFoldRows * NV % kFoldThreads == 0, "a chunk must split evenly over the block");
   const int t = threadIdx.x;
+  double nxt[U];
+  auto fetch = [&](int j0) {
+    const int n = min(kFoldRows, nrows - j0) * NV;
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = t + u * kFoldThreads;
+      nxt[u] = i < n ? rows[(int64_t)j0 * NV + i] : 0.0;
+    }
+  };
+  if (nrows > 0) fetch(0);
   for (int j0 = 0; j0 < nrows; j0 += kFoldRows) {
     const int nr = min(kFoldRows, nrows - j0);
     __syncthreads();  // the previous chunk has been folded
-    for (int i = t; i < nr * NV; i += kFoldThreads) lds[i] = rows[(int64_t)j0 * NV + i];
+#pragma unroll
+    for (int u = 0; u < U; ++u) lds[t + u * kFoldThreads] = nxt[u];
     __syncthreads();
+    if (j0 + kFoldRows < nrows) fetch(j0 + kFoldRows);
     if (t < NV) {
-      for (int j = 0; j < nr; ++j) {
+      // Eight LDS reads in flight ahead of eight folds, in row order.
+      int j = 0;
+      for (; j + 8 <= nr; j += 8) {
+        double w[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) w[q] = lds[(j + q) * NV + t];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) a = op == 0 ? a + w[q] : (op == 1 ? fmax(a, w[q]) : fmin(a, w[q]));
+      }
+      for (; j < nr; ++j) {
         const double w = lds[j * NV + t];
         a = op == 0 ? a + w : (op == 1 ? fmax(a, w) : fmin(a, w));
       }
