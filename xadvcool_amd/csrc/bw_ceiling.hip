@@ -9,6 +9,10 @@
 //   soa19_inpl   19 planes read then written back in place (the AA even pass)
 //   soa19_shift  in place, loads from c - s_i and stores to c + s_i with the
 //                D3Q19 offsets of a 128 x 32 plane (the AA odd pass shape)
+//   soa19_inpl4  soa19_inpl with 4 consecutive cells per thread: one 16-byte
+//                load and store per plane
+//   soa19_shift4 soa19_shift with 4 consecutive cells per thread: 16-byte
+//                accesses at the D3Q19 offsets (4-byte aligned only)
 // Prints one JSON line per kernel and size.
 #include <hip/hip_runtime.h>
 
@@ -75,6 +79,33 @@ __global__ void soa19_shift(float* __restrict__ a, uint32_t N, uint32_t lo, uint
   for (int p = 0; p < 19; ++p) a[(size_t)p * N + (c + kShift[p])] = v[p] * 0.5f;
 }
 
+// 16 bytes with 4-byte alignment: the shifted accesses start at any cell.
+typedef float f4a4 __attribute__((ext_vector_type(4), aligned(4)));
+
+// N4: plane stride in floats, a multiple of 4; thread t owns cells 4t .. 4t+3.
+__global__ void soa19_inpl4(float* __restrict__ a, uint32_t N4) {
+  const uint32_t c = 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= N4) return;
+  f4a4 v[19];
+#pragma unroll
+  for (int p = 0; p < 19; ++p) v[p] = *(const f4a4*)(a + (size_t)p * N4 + c);
+#pragma unroll
+  for (int p = 0; p < 19; ++p) *(f4a4*)(a + (size_t)(18 - p) * N4 + c) = v[p] * 0.5f;
+}
+
+__global__ void soa19_shift4(float* __restrict__ a, uint32_t N4, uint32_t lo, uint32_t hi) {
+  const uint32_t c = lo + 4 * (blockIdx.x * blockDim.x + threadIdx.x);
+  if (c >= hi) return;  // lo, hi keep c - kMaxShift and c + 3 + kMaxShift inside [0, N4)
+  f4a4 v[19];
+#pragma unroll
+  for (int p = 0; p < 19; ++p) {
+    const int q = p == 0 ? 0 : ((p & 1) ? p + 1 : p - 1);
+    v[p] = *(const f4a4*)(a + (size_t)q * N4 + (c - kShift[p]));
+  }
+#pragma unroll
+  for (int p = 0; p < 19; ++p) *(f4a4*)(a + (size_t)p * N4 + (c + kShift[p])) = v[p] * 0.5f;
+}
+
 int main() {
   const size_t sizes_mb[] = {92, 512, 2048};
   hipEvent_t e0, e1;
@@ -111,6 +142,12 @@ int main() {
     run("soa19_inpl", 8.0 * 19 * N, [&] { soa19_inpl<<<gs, 256>>>(a, N); });
     const uint32_t lo = 2 * kMaxShift, hi = N - 2 * kMaxShift;
     run("soa19_shift", 8.0 * 19 * (hi - lo), [&] { soa19_shift<<<(hi - lo + 255) / 256, 256>>>(a, N, lo, hi); });
+    const uint32_t N4 = N & ~3u;  // 19 * N4 <= n
+    const unsigned g4c = (N4 / 4 + 255) / 256;
+    run("soa19_inpl4", 8.0 * 19 * N4, [&] { soa19_inpl4<<<g4c, 256>>>(a, N4); });
+    const uint32_t lo4 = 2 * kMaxShift, hi4 = N4 - 2 * kMaxShift;  // both multiples of 4
+    run("soa19_shift4", 8.0 * 19 * (hi4 - lo4),
+        [&] { soa19_shift4<<<((hi4 - lo4) / 4 + 255) / 256, 256>>>(a, N4, lo4, hi4); });
     CK(hipFree(a));
     CK(hipFree(b));
   }
