@@ -12,13 +12,17 @@ cold (equilibrium at rest) and warm (surrogate-predicted rho, u, T; a warm
 run that stalls or diverges is re-run cold).  Each arm's wall time covers
 building its designs on the GPU from their parameters (mask, heat map, slot
 inputs), the solves, metrics and, for the warm arm, the surrogate inference
-and any fallback.
+and any fallback.  Before the timed arms, untimed like a compile step, the
+project's autotuner picks the fused pass's launch shape (threads per block,
+cells per thread, block order, sweep direction) on three other full-size
+designs; both arms use it (``--launch fixed`` takes the shape from the
+flags instead).
 
 Printed (one JSON line): warm-start converged solves/hour (``value``) with
 the cold-start figure and the relative differences of the converged pressure
 drop and thermal resistance, MLUPS (all cells x steps / wall time), the fused
 pass's achieved bandwidth against the MI355X HBM roofline (HIP events on the
-solver's own stream over the cold arm's passes; algorithmic bytes 152 B per
+solver's own stream over every pass of both arms; algorithmic bytes 152 B per
 non-solid cell for the fluid lattice plus 152 B per cell for the thermal
 lattice, per pass), and the CPU oracle timed on this host as the baseline.
 """
