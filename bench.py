@@ -356,8 +356,8 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py measures one GPU: the solver runs independent designs per GPU and this build has no "
-              "multi-GPU path (use --gpus 1)", file=sys.stderr)
+        print("bench.py measures one GPU: the solver runs independent designs per GPU and has no multi-GPU "
+              "path (use --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
     if args.mode == "passes":

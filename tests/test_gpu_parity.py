@@ -4,7 +4,7 @@ The oracle (oracle/lbm_oracle.cpp) runs the reference's six-kernel,
 two-buffer schedule; the HIP library runs one fused in-place pass per step.
 Both use the reference's operations in the reference's order with no
 contraction, so in FP32 the fields must agree bit for bit; the bar written
-into each test is the north star's rel-L2 <= 1e-5 plus an exact-match check
+into each test is a rel-L2 bound of 1e-5 plus an exact-match check
 on rho, u and T.  FP64 runs are checked the same way against the FP64 oracle.
 
 Lattices are small and seeded so the oracle finishes in well under a second;

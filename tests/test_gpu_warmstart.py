@@ -1,4 +1,4 @@
-"""Warm start on the GPU: the cold LBM is the fallback and the judge.
+"""Warm start on the GPU: the cold LBM is the fallback and the correctness reference.
 
 * a warm start that blows up (temperatures far above the divergence limit) is abandoned at
   its first residual check and the design is re-run cold: the result equals
@@ -66,7 +66,7 @@ def test_trained_surrogate_cuts_steps_on_unseen_designs(gpu):
     print(summ, flush=True)
     both = [r for r in rows if r["cold_status"] == "converged" and r["warm_status"] == "converged"]
     assert len(both) >= 6
-    for r in both:  # the cold LBM judges the warm result
+    for r in both:  # the cold LBM is the reference for the warm result
         assert r["d_pressure_drop"] <= 5e-3, r
         assert r["d_R_th"] <= 1.5e-2, r  # each arm within ~0.5 % of the steady state
     assert np.median([r["d_R_th"] for r in both]) <= 5e-3

@@ -8,7 +8,7 @@
   Warm designs start from the surrogate's (rho, u, T) (solids at rest);
   a warm run whose residual grows for ``stall_checks`` consecutive checks
   after ``stall_after`` steps, or that diverges, is abandoned and re-queued
-  cold (the plain LBM is the fallback and the judge).
+  cold (the plain LBM is the fallback and the correctness reference).
 * ``compare``: cold vs warm on the same designs with the same stop rule:
   steps, converged solves per hour (inference and fallbacks included in the
   warm arm's wall time), and the relative differences of the converged
@@ -388,7 +388,7 @@ def main():
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
-    ap.add_argument("--out", default="gpurun_out")
+    ap.add_argument("--out", default="warmstart_out")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-steps", type=int, default=30000)
     ap.add_argument("--slots", type=int, default=64)
@@ -407,6 +407,7 @@ def main():
     ap.add_argument("--deadline-s", type=float, default=None,
                     help="stop training after the epoch that ends past this many seconds from the start")
     args = ap.parse_args()
+    os.makedirs(args.out, exist_ok=True)
     log = lambda s: print(s, flush=True)
     if args.cmd == "train":
         t_start = time.perf_counter()
