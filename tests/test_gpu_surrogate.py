@@ -69,6 +69,45 @@ def test_conv_fwd_dgrad_wgrad(gpu, cin, cout):
     assert torch.all(c.dw.cpu()[:, 27 * cin:] == 0)
 
 
+@pytest.mark.parametrize("cin", [16, 32])
+def test_conv16_lds_path_matches_reference_and_gather_kernel(gpu, cin):
+    """CIN -> 16 layers on lattices with Z = 32 (Y a multiple of 16, or of 8 for 32 channels) take the
+    LDS-halo kernel; so does the data gradient of a 16 -> 16 layer.
+
+    Against PyTorch (forward and data gradient, rtol 1e-3), and bit for bit against the gather
+    kernel: zero-padding z to 34 sends the same convolution through the gather kernel (Z != 32),
+    and its z = 1 .. 32 slice must equal the LDS kernel's output exactly (same MFMA sequence per
+    output element)."""
+    from xadvcool_amd import surrogate as S
+    shape = (2, 3, 32, 32) if cin == 16 else (2, 3, 24, 32)
+    print(f"[nn] conv {cin}->16 LDS-halo path {shape}", flush=True)
+    gen = torch.Generator().manual_seed(1600 + cin)
+    N, X, Y, Z = shape
+    c = S.Conv(cin, 16, gen)
+    c.b.copy_(torch.randn(16, generator=gen))
+    x = torch.randn(*shape, cin, generator=gen)
+    xb = x.to(torch.bfloat16).cuda()
+    out = c.fwd(xb, shape).cpu()
+    W = c.w.cpu()
+    ref = ndhwc(F.conv3d(ncdhw(bf(x)), torch_weight(bf(W), cin, 16), bias=c.b.cpu(), padding=1))
+    assert rel(out, ref) < 1e-3
+    xp = torch.zeros(N, X, Y, Z + 2, cin, dtype=torch.bfloat16, device="cuda")
+    xp[:, :, :, 1:Z + 1] = xb
+    outp = c.fwd(xp, (N, X, Y, Z + 2)).cpu()
+    assert torch.equal(outp[:, :, :, 1:Z + 1], out)
+    g = torch.randn(*shape, 16, generator=gen)
+    gb = g.to(torch.bfloat16).cuda()
+    gin = c.dgrad(gb, shape).cpu()
+    xr = ncdhw(bf(x)).clone().requires_grad_(True)
+    o = F.conv3d(xr, torch_weight(bf(W), cin, 16), bias=c.b.cpu(), padding=1)
+    o.backward(ncdhw(bf(g)))
+    assert rel(gin, ndhwc(xr.grad)) < 1e-3
+    gp = torch.zeros(N, X, Y, Z + 2, 16, dtype=torch.bfloat16, device="cuda")
+    gp[:, :, :, 1:Z + 1] = gb
+    ginp = c.dgrad(gp, (N, X, Y, Z + 2)).cpu()
+    assert torch.equal(ginp[:, :, :, 1:Z + 1], gin)
+
+
 @pytest.mark.parametrize("shape,cin,cout", [((2, 5, 7, 32), 16, 16), ((2, 5, 7, 32), 32, 16),
                                              ((2, 3, 6, 16), 16, 32), ((2, 3, 6, 16), 32, 32),
                                              ((1, 4, 9, 16), 64, 32), ((2, 3, 5, 8), 32, 64),

@@ -154,6 +154,84 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const __bf16* __restrict__ i
   }
 }
 
+// CIN -> 16 convolution for lattices with Z = 32 and Y a multiple of YB (the
+// full-resolution layers): a block owns the YB * 32 voxels of one (n, x) and
+// YB consecutive y columns, all z.  It stages their input halo (x - 1 .. x + 1,
+// YB + 2 y columns, z = -1 .. 32; zeros outside the lattice) in LDS once, and
+// its waves then read every tap's A operand from LDS instead of gathering it
+// from global memory 27 times.  Each wave computes YB / 2 tiles of 16 voxels
+// (half a z column each) with the whole weight matrix held as MFMA B
+// fragments, exactly the MFMA sequence of k_conv_fwd16<CIN> per tile, so the
+// outputs are the same bits.  Halo: 3 * (YB + 2) * 34 voxels of CIN BF16
+// channels (CIN 16, YB 16: 58,752 bytes; CIN 32, YB 8: 65,280 bytes).
+template <int CIN, int YB>
+__global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
+                                                        const float* __restrict__ bias, float* __restrict__ out,
+                                                        Geo g, int Kp) {
+  constexpr int K = 27 * CIN, KS = (K + 31) / 32;
+  constexpr int HY = YB + 2, HZ = 34, HV = 3 * HY * HZ;
+  constexpr int CH = CIN / 8;                 // 16-byte chunks per voxel
+  constexpr int kChunks = HV * CH;
+  constexpr int kPer = (kChunks + 255) / 256;  // per thread
+  constexpr int TPW = YB / 2;                  // tiles per wave
+  static_assert(HV * CIN * 2 <= 65536, "halo must fit the static LDS limit");
+  static_assert(YB % 2 == 0 && YB * 2 == 4 * TPW, "a block is 4 waves of whole tiles");
+  __shared__ __attribute__((aligned(16))) __bf16 halo[HV * CIN];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kg = (lane >> 4) * 8;
+  const int ybl = g.Y / YB;  // blocks per (n, x)
+  const int nx = blockIdx.x / ybl;
+  const int y0 = (blockIdx.x - nx * ybl) * YB;
+  const int n = nx / g.X, x = nx - n * g.X;
+  // Stage the halo: every load of this thread is issued before any store.
+  u16x8 tmp[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int e = threadIdx.x + i * 256;
+    tmp[i] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (e < kChunks) {
+      const int hv = e / CH, part = e - hv * CH;
+      const int ix = hv / (HY * HZ), r = hv - ix * (HY * HZ);
+      const int iy = r / HZ, iz = r - iy * HZ;
+      const int xx = x + ix - 1, yy = y0 + iy - 1, zz = iz - 1;
+      if (xx >= 0 && xx < g.X && yy >= 0 && yy < g.Y && zz >= 0 && zz < g.Z)
+        tmp[i] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + xx) * g.Y + yy) * g.Z + zz) * CIN +
+                                                 part * 8);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int e = threadIdx.x + i * 256;
+    if (e < kChunks) *reinterpret_cast<u16x8*>(halo + e * 8) = tmp[i];
+  }
+  bf16x8 B[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks)
+    B[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)m * Kp + ks * 32 + kg));
+  const float b = bias ? bias[m] : 0.0f;
+  __syncthreads();
+  const int64_t vblock = (int64_t)blockIdx.x * (YB * 32);
+  for (int t = 0; t < TPW; ++t) {
+    const int tt = wave * TPW + t;  // tile of the block: column tt / 2, z from (tt % 2) * 16
+    const int yy = tt >> 1, z0 = (tt & 1) * 16;
+    floatx4 acc = floatx4{0, 0, 0, 0};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      const int k0 = ks * 32 + kg;
+      u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (k0 < K) {
+        const int tap = k0 / CIN, ci = k0 - tap * CIN;
+        const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
+        raw = *reinterpret_cast<const u16x8*>(halo + ((dx * HY + yy + dy) * HZ + z0 + m + dz) * CIN + ci);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), B[ks], acc, 0, 0, 0);
+    }
+    const int64_t v0 = vblock + tt * 16;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * 16 + m] = acc[j] + b;
+  }
+}
+
 // W'[ci][tap' * Cout + co] = W[co][(26 - tap') * Cin + ci], K' padded with zeros.
 __global__ void k_wflip(const __bf16* __restrict__ W, __bf16* __restrict__ Wt, int Cin, int Cout, int Kp,
                         int Kpt) {
@@ -578,6 +656,18 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
   hipStream_t s = (hipStream_t)stream;
   const __bf16* i = (const __bf16*)in;
   const __bf16* w = (const __bf16*)W;
+  if (Cout == 16 && Cin == 16 && Kp == (27 * Cin + 31) / 32 * 32 && Z == 32 && Y % 16 == 0) {
+    // one block per (n, x) and 16 y columns: V / 512 blocks
+    hipLaunchKernelGGL((k_conv_fwd16_lds<16, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias, out, g,
+                       Kp);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
+  if (Cout == 16 && Cin == 32 && Kp == (27 * Cin + 31) / 32 * 32 && Z == 32 && Y % 8 == 0) {
+    // one block per (n, x) and 8 y columns: V / 256 blocks
+    hipLaunchKernelGGL((k_conv_fwd16_lds<32, 8>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias, out, g,
+                       Kp);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
   if (Cout == 16 && (Cin == 16 || Cin == 32) && Kp == (27 * Cin + 31) / 32 * 32) {
     const int tiles = 8;
     const unsigned gw = (unsigned)(((V + 15) / 16 + 4 * tiles - 1) / (4 * tiles));
