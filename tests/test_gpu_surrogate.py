@@ -69,40 +69,41 @@ def test_conv_fwd_dgrad_wgrad(gpu, cin, cout):
     assert torch.all(c.dw.cpu()[:, 27 * cin:] == 0)
 
 
-@pytest.mark.parametrize("cin", [16, 32])
-def test_conv16_lds_path_matches_reference_and_gather_kernel(gpu, cin):
-    """CIN -> 16 layers on lattices with Z = 32 (Y a multiple of 16, or of 8 for 32 channels) take the
-    LDS-halo kernel; so does the data gradient of a 16 -> 16 layer.
+@pytest.mark.parametrize("cin,cout,shape", [(16, 16, (2, 3, 32, 32)), (32, 16, (2, 3, 24, 32)),
+                                             (16, 32, (2, 3, 32, 32)), (16, 32, (2, 3, 16, 16)),
+                                             (32, 32, (2, 3, 16, 16)), (32, 16, (2, 3, 16, 16))])
+def test_conv_lds_paths_match_reference_and_gather_kernel(gpu, cin, cout, shape):
+    """Convolutions on lattices with Z = 32 or 16 (Y a multiple of 16, or of 8 for 32 -> 16 at Z = 32)
+    take the LDS-halo kernels (k_conv_fwd16_lds, k_conv_fwd_lds), forward and data gradient.
 
-    Against PyTorch (forward and data gradient, rtol 1e-3), and bit for bit against the gather
-    kernel: zero-padding z to 34 sends the same convolution through the gather kernel (Z != 32),
-    and its z = 1 .. 32 slice must equal the LDS kernel's output exactly (same MFMA sequence per
-    output element)."""
+    Against PyTorch (rtol 1e-3), and bit for bit against the gather kernels: zero-padding z by
+    two planes sends the same convolution through k_conv_fwd16 / k_conv_fwd (Z is then neither 16
+    nor 32), and the padded result's inner z planes must equal the LDS kernel's output exactly
+    (same MFMA sequence per output element)."""
     from xadvcool_amd import surrogate as S
-    shape = (2, 3, 32, 32) if cin == 16 else (2, 3, 24, 32)
-    print(f"[nn] conv {cin}->16 LDS-halo path {shape}", flush=True)
-    gen = torch.Generator().manual_seed(1600 + cin)
+    print(f"[nn] conv {cin}->{cout} LDS-halo path {shape}", flush=True)
+    gen = torch.Generator().manual_seed(1600 + 7 * cin + cout + shape[3])
     N, X, Y, Z = shape
-    c = S.Conv(cin, 16, gen)
-    c.b.copy_(torch.randn(16, generator=gen))
+    c = S.Conv(cin, cout, gen)
+    c.b.copy_(torch.randn(cout, generator=gen))
     x = torch.randn(*shape, cin, generator=gen)
     xb = x.to(torch.bfloat16).cuda()
     out = c.fwd(xb, shape).cpu()
     W = c.w.cpu()
-    ref = ndhwc(F.conv3d(ncdhw(bf(x)), torch_weight(bf(W), cin, 16), bias=c.b.cpu(), padding=1))
+    ref = ndhwc(F.conv3d(ncdhw(bf(x)), torch_weight(bf(W), cin, cout), bias=c.b.cpu(), padding=1))
     assert rel(out, ref) < 1e-3
     xp = torch.zeros(N, X, Y, Z + 2, cin, dtype=torch.bfloat16, device="cuda")
     xp[:, :, :, 1:Z + 1] = xb
     outp = c.fwd(xp, (N, X, Y, Z + 2)).cpu()
     assert torch.equal(outp[:, :, :, 1:Z + 1], out)
-    g = torch.randn(*shape, 16, generator=gen)
+    g = torch.randn(*shape, cout, generator=gen)
     gb = g.to(torch.bfloat16).cuda()
     gin = c.dgrad(gb, shape).cpu()
     xr = ncdhw(bf(x)).clone().requires_grad_(True)
-    o = F.conv3d(xr, torch_weight(bf(W), cin, 16), bias=c.b.cpu(), padding=1)
+    o = F.conv3d(xr, torch_weight(bf(W), cin, cout), bias=c.b.cpu(), padding=1)
     o.backward(ncdhw(bf(g)))
     assert rel(gin, ndhwc(xr.grad)) < 1e-3
-    gp = torch.zeros(N, X, Y, Z + 2, 16, dtype=torch.bfloat16, device="cuda")
+    gp = torch.zeros(N, X, Y, Z + 2, cout, dtype=torch.bfloat16, device="cuda")
     gp[:, :, :, 1:Z + 1] = gb
     ginp = c.dgrad(gp, (N, X, Y, Z + 2)).cpu()
     assert torch.equal(ginp[:, :, :, 1:Z + 1], gin)

@@ -232,6 +232,97 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
   }
 }
 
+// CIN -> NT * 16 convolution for lattices with Z = ZL and Y a multiple of YB:
+// the input halo of a block (one (n, x), YB y columns, all ZL z; 3 x (YB + 2)
+// x (ZL + 2) voxels, zeros outside the lattice) is staged in LDS once, as in
+// k_conv_fwd16_lds.  Each wave owns YB * ZL / 64 tiles of 16 voxels and walks
+// the K steps in order, loading each step's B fragments once for all of its
+// tiles: every output element gets the same MFMA sequence as in k_conv_fwd<NT>,
+// so the outputs are the same bits.
+template <int CIN, int NT, int ZL, int YB>
+__global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
+                                                      const float* __restrict__ bias, float* __restrict__ out, Geo g,
+                                                      int Kp) {
+  constexpr int K = 27 * CIN, KS = (K + 31) / 32, COUT = NT * 16;
+  constexpr int HY = YB + 2, HZ = ZL + 2, HV = 3 * HY * HZ;
+  constexpr int CH = CIN / 8;
+  constexpr int kChunks = HV * CH;
+  constexpr int kPer = (kChunks + 255) / 256;
+  constexpr int TPC = ZL / 16;             // tiles per z column
+  constexpr int TPW = YB * TPC / 4;        // tiles per wave
+  static_assert(HV * CIN * 2 <= 65536, "halo must fit the static LDS limit");
+  static_assert(ZL % 16 == 0 && (YB * TPC) % 4 == 0, "a block is 4 waves of whole tiles");
+  __shared__ __attribute__((aligned(16))) __bf16 halo[HV * CIN];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kg = (lane >> 4) * 8;
+  const int ybl = g.Y / YB;
+  const int nx = blockIdx.x / ybl;
+  const int y0 = (blockIdx.x - nx * ybl) * YB;
+  const int n = nx / g.X, x = nx - n * g.X;
+  u16x8 tmp[kPer];
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int e = threadIdx.x + i * 256;
+    tmp[i] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (e < kChunks) {
+      const int hv = e / CH, part = e - hv * CH;
+      const int ix = hv / (HY * HZ), r = hv - ix * (HY * HZ);
+      const int iy = r / HZ, iz = r - iy * HZ;
+      const int xx = x + ix - 1, yy = y0 + iy - 1, zz = iz - 1;
+      if (xx >= 0 && xx < g.X && yy >= 0 && yy < g.Y && zz >= 0 && zz < g.Z)
+        tmp[i] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + xx) * g.Y + yy) * g.Z + zz) * CIN +
+                                                 part * 8);
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < kPer; ++i) {
+    const int e = threadIdx.x + i * 256;
+    if (e < kChunks) *reinterpret_cast<u16x8*>(halo + e * 8) = tmp[i];
+  }
+  __syncthreads();
+  floatx4 acc[TPW][NT];
+#pragma unroll
+  for (int tl = 0; tl < TPW; ++tl)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[tl][t] = floatx4{0, 0, 0, 0};
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k0 = ks * 32 + kg;
+    bf16x8 Bt[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      Bt[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)(t * 16 + m) * Kp + k0));
+    int hoff = -1;  // LDS offset of this lane's tap for the block's first tile, or -1 (zero A)
+    if (k0 < K) {
+      const int tap = k0 / CIN, ci = k0 - tap * CIN;
+      const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
+      hoff = ((dx * HY + dy) * HZ + m + dz) * CIN + ci;
+    }
+#pragma unroll
+    for (int tl = 0; tl < TPW; ++tl) {
+      const int tt = wave * TPW + tl;
+      const int yy = tt / TPC, z0 = (tt - yy * TPC) * 16;
+      u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (hoff >= 0) raw = *reinterpret_cast<const u16x8*>(halo + hoff + (yy * HZ + z0) * CIN);
+#pragma unroll
+      for (int t = 0; t < NT; ++t)
+        acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), Bt[t], acc[tl][t], 0,
+                                                            0, 0);
+    }
+  }
+  const int64_t vblock = (int64_t)blockIdx.x * (YB * ZL);
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int co = t * 16 + m;
+    const float b = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int tl = 0; tl < TPW; ++tl) {
+      const int64_t v0 = vblock + (wave * TPW + tl) * 16;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * COUT + co] = acc[tl][t][j] + b;
+    }
+  }
+}
+
 // W'[ci][tap' * Cout + co] = W[co][(26 - tap') * Cin + ci], K' padded with zeros.
 __global__ void k_wflip(const __bf16* __restrict__ W, __bf16* __restrict__ Wt, int Cin, int Cout, int Kp,
                         int Kpt) {
@@ -667,6 +758,29 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
     hipLaunchKernelGGL((k_conv_fwd16_lds<32, 8>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias, out, g,
                        Kp);
     return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
+  if (Kp == (27 * Cin + 31) / 32 * 32 && Y % 16 == 0) {
+    // LDS-halo variants of k_conv_fwd<2> and k_conv_fwd16<32> for the half- and full-resolution layers
+    if (Cout == 32 && Cin == 16 && Z == 32) {
+      hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 32, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Cout == 32 && Cin == 16 && Z == 16) {
+      hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Cout == 32 && Cin == 32 && Z == 16) {
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 2, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Cout == 16 && Cin == 32 && Z == 16) {
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 1, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
   }
   if (Cout == 16 && (Cin == 16 || Cin == 32) && Kp == (27 * Cin + 31) / 32 * 32) {
     const int tiles = 8;
