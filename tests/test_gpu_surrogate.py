@@ -71,14 +71,18 @@ def test_conv_fwd_dgrad_wgrad(gpu, cin, cout):
 
 @pytest.mark.parametrize("cin,cout,shape", [(16, 16, (2, 3, 32, 32)), (32, 16, (2, 3, 24, 32)),
                                              (16, 32, (2, 3, 32, 32)), (16, 32, (2, 3, 16, 16)),
-                                             (32, 32, (2, 3, 16, 16)), (32, 16, (2, 3, 16, 16))])
+                                             (32, 32, (2, 3, 16, 16)), (32, 16, (2, 3, 16, 16)),
+                                             (32, 64, (2, 3, 16, 16)), (64, 32, (2, 3, 16, 16)),
+                                             (32, 64, (2, 3, 16, 8)), (64, 64, (2, 3, 16, 8)),
+                                             (128, 64, (2, 3, 16, 8))])
 def test_conv_lds_paths_match_reference_and_gather_kernel(gpu, cin, cout, shape):
-    """Convolutions on lattices with Z = 32 or 16 (Y a multiple of 16, or of 8 for 32 -> 16 at Z = 32)
-    take the LDS-halo kernels (k_conv_fwd16_lds, k_conv_fwd_lds), forward and data gradient.
+    """Convolutions on lattices with Z = 32, 16 or 8 (Y a multiple of 16, or of 8 for 32 -> 16 at
+    Z = 32) take the LDS-halo kernels (k_conv_fwd16_lds, k_conv_fwd_lds), forward and data gradient
+    (the 128 -> 64 case covers the 64 -> 128 data gradient).
 
     Against PyTorch (rtol 1e-3), and bit for bit against the gather kernels: zero-padding z by
-    two planes sends the same convolution through k_conv_fwd16 / k_conv_fwd (Z is then neither 16
-    nor 32), and the padded result's inner z planes must equal the LDS kernel's output exactly
+    two planes sends the same convolution through k_conv_fwd16 / k_conv_fwd (Z is then none of 8,
+    16, 32), and the padded result's inner z planes must equal the LDS kernel's output exactly
     (same MFMA sequence per output element)."""
     from xadvcool_amd import surrogate as S
     print(f"[nn] conv {cin}->{cout} LDS-halo path {shape}", flush=True)

@@ -235,10 +235,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
 // CIN -> NT * 16 convolution for lattices with Z = ZL and Y a multiple of YB:
 // the input halo of a block (one (n, x), YB y columns, all ZL z; 3 x (YB + 2)
 // x (ZL + 2) voxels, zeros outside the lattice) is staged in LDS once, as in
-// k_conv_fwd16_lds.  Each wave owns YB * ZL / 64 tiles of 16 voxels and walks
-// the K steps in order, loading each step's B fragments once for all of its
-// tiles: every output element gets the same MFMA sequence as in k_conv_fwd<NT>,
-// so the outputs are the same bits.
+// k_conv_fwd16_lds.  Each wave owns YB * ZL / 64 tiles of 16 consecutive
+// voxels (with ZL = 8 a tile spans two z columns) and walks the K steps in
+// order, loading each step's B fragments once for all of its tiles: every
+// output element gets the same MFMA sequence as in k_conv_fwd<NT>, so the
+// outputs are the same bits.
 template <int CIN, int NT, int ZL, int YB>
 __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
                                                       const float* __restrict__ bias, float* __restrict__ out, Geo g,
@@ -248,10 +249,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
   constexpr int CH = CIN / 8;
   constexpr int kChunks = HV * CH;
   constexpr int kPer = (kChunks + 255) / 256;
-  constexpr int TPC = ZL / 16;             // tiles per z column
-  constexpr int TPW = YB * TPC / 4;        // tiles per wave
+  constexpr int TPW = YB * ZL / 64;        // tiles per wave
   static_assert(HV * CIN * 2 <= 65536, "halo must fit the static LDS limit");
-  static_assert(ZL % 16 == 0 && (YB * TPC) % 4 == 0, "a block is 4 waves of whole tiles");
+  static_assert((ZL == 8 || ZL % 16 == 0) && TPW * 64 == YB * ZL, "a block is 4 waves of whole tiles");
   __shared__ __attribute__((aligned(16))) __bf16 halo[HV * CIN];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int m = lane & 15, kg = (lane >> 4) * 8;
@@ -291,18 +291,18 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
 #pragma unroll
     for (int t = 0; t < NT; ++t)
       Bt[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)(t * 16 + m) * Kp + k0));
-    int hoff = -1;  // LDS offset of this lane's tap for the block's first tile, or -1 (zero A)
+    int hoff = -1;  // LDS offset of this lane's tap relative to its voxel, or -1 (zero A)
     if (k0 < K) {
       const int tap = k0 / CIN, ci = k0 - tap * CIN;
       const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
-      hoff = ((dx * HY + dy) * HZ + m + dz) * CIN + ci;
+      hoff = ((dx * HY + dy) * HZ + dz) * CIN + ci;
     }
 #pragma unroll
     for (int tl = 0; tl < TPW; ++tl) {
-      const int tt = wave * TPW + tl;
-      const int yy = tt / TPC, z0 = (tt - yy * TPC) * 16;
+      const int off = (wave * TPW + tl) * 16 + m;  // this lane's voxel in the block
+      const int yy = off / ZL, z = off - yy * ZL;
       u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (hoff >= 0) raw = *reinterpret_cast<const u16x8*>(halo + hoff + (yy * HZ + z0) * CIN);
+      if (hoff >= 0) raw = *reinterpret_cast<const u16x8*>(halo + hoff + (yy * HZ + z) * CIN);
 #pragma unroll
       for (int t = 0; t < NT; ++t)
         acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), Bt[t], acc[tl][t], 0,
@@ -779,6 +779,31 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
     if (Cout == 16 && Cin == 32 && Z == 16) {
       hipLaunchKernelGGL((k_conv_fwd_lds<32, 1, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
                          out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Cout == 64 && Cin == 32 && Z == 16) {
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Cout == 32 && Cin == 64 && Z == 16) {  // 4 y columns: the 64-channel halo fits the LDS limit
+      hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 16, 4>), dim3((unsigned)(V / 64)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Z == 8 && Cin == 32 && Cout == 64) {
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 8, 16>), dim3((unsigned)(V / 128)), dim3(256), 0, s, i, w, bias,
+                         out, g, Kp);
+      return hipGetLastError() == hipSuccess ? 0 : 2;
+    }
+    if (Z == 8 && Cin == 64 && (Cout == 32 || Cout == 64 || Cout == 128)) {  // 8 y columns per block
+      const dim3 grid((unsigned)(V / 64));
+      if (Cout == 32)
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
+      else if (Cout == 64)
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 4, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
+      else
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 8, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
   }
