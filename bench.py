@@ -326,6 +326,40 @@ def config1_bench(cpu_steps_1=100, cpu_steps_all=500):
     print(json.dumps(out), flush=True)
 
 
+def config2_bench(seed=0, tol=1e-6, max_steps=30000, cpu_seconds=12.0):
+    """Full-size single solve (148x128x32 gyroid TPMS, quad_core heat map, parameters from the seed) through the
+    drop-in run_simulation, to the residual tolerance (reference residuals, flow-through floor off) or max_steps;
+    the oracle's per-step rate on the same design at 1 thread and at all threads (bounded samples)."""
+    import torch
+    from oracle import oracle as orc
+    from xadvcool_amd.geometry import config2_design
+    from xadvcool_amd.simulation import run_simulation
+    d = config2_design(seed)
+    kw = dict(tau_fluid=d["tau_fluid"], heat_power=d["heat_power"], u_inlet_val=d["u_inlet_val"],
+              heat_source=d["heat"], tol_ke=tol, tol_T=tol, silent=True, min_steps_rule=False)
+    run_simulation(d["mask"], max_steps=200, **kw)  # load kernels, build graphs
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    r = run_simulation(d["mask"], max_steps=max_steps, **kw)
+    gpu_s = time.perf_counter() - t0
+    cells = int(d["mask"].size)
+    out = {"mode": "config2", "lattice": list(d["mask"].shape), "seed": seed, "family": d["family"],
+           "period": d["period"], "threshold": d["threshold"], "tau_fluid": d["tau_fluid"],
+           "u_inlet_val": d["u_inlet_val"], "heat_power": d["heat_power"], "tol": tol, "max_steps": max_steps,
+           "steps": r["steps"], "converged": bool(r["converged"]),
+           "gpu": {"wall_s": round(gpu_s, 3), "mlups": round(cells * r["steps"] / gpu_s / 1e6, 1),
+                   "us_per_step": round(1e6 * gpu_s / max(r["steps"], 1), 2),
+                   "note": "run_simulation end to end: init, graph replays, residual checks every 100 steps, "
+                           "field copies"}}
+    for threads, sec in ((1, cpu_seconds), (orc.max_threads(), cpu_seconds / 2)):
+        mlups, n, t = cpu_baseline(d, sec, threads)
+        out[f"oracle_{threads}t"] = {"steps": n, "wall_s": round(t, 2), "mlups": round(mlups, 3),
+                                     "s_per_step": round(t / max(n, 1), 4),
+                                     "solve_s_at_this_rate": round(r["steps"] * t / max(n, 1), 1)}
+    out["host_threads"] = orc.max_threads()
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,12 +369,13 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1", "train-step",
-                                       "designs"],
+    ap.add_argument("--mode", choices=["solves", "passes", "copy", "ceilings", "batch", "config1", "config2",
+                                       "train-step", "designs"],
                     default="solves",
                     help="solves: the benchmark; passes / copy: profiling drivers (pure fused passes; a plain copy); "
                          "ceilings: bandwidth of plain copies and of the pass's access shapes")
     ap.add_argument("--passes", type=int, default=400)
+    ap.add_argument("--seed", type=int, default=0, help="config2 mode: design seed")
     ap.add_argument("--copy-mb", type=float, default=512.0)
     ap.add_argument("--designs", type=int, default=64)
     ap.add_argument("--slots", type=int, default=None, help="batch slots (default: 1 for passes, 64 for batch)")
@@ -373,6 +408,8 @@ def main():
         sys.exit(subprocess.run([exe]).returncode)
     if args.mode == "config1":
         return config1_bench()
+    if args.mode == "config2":
+        return config2_bench(seed=args.seed, tol=args.tol, max_steps=args.max_steps, cpu_seconds=args.cpu_seconds)
     if args.mode == "train-step":
         return train_step_bench()
     if args.mode == "designs":
