@@ -166,6 +166,41 @@ def test_pool_up_silu(gpu):
     assert rel(gx.float().cpu(), xr.grad) < 5e-3
 
 
+def _offset_copy(t):
+    """The same values one element into a fresh buffer: the pointer is then not 16-byte aligned, so the
+    launchers take the scalar elementwise kernels instead of the vectorised ones."""
+    buf = torch.empty(t.numel() + 8, dtype=t.dtype, device=t.device)
+    v = buf[1:1 + t.numel()].view(t.shape)
+    v.copy_(t)
+    return v
+
+
+@pytest.mark.parametrize("shape", [(2, 8, 4, 6, 16), (1, 4, 8, 2, 32)])
+def test_vectorised_elementwise_kernels_match_scalar(gpu, shape):
+    """Pooling, upsampling, SiLU and the adds: the vectorised kernels give the scalar kernels' bits
+    (also on element counts that leave a tail: 1001 and 4099 values)."""
+    from xadvcool_amd import surrogate as S
+    gen = torch.Generator().manual_seed(11)
+    N, X, Y, Z, C = shape
+    cs = (N, X // 2, Y // 2, Z // 2)
+    xb = torch.randn(*shape, generator=gen).to(torch.bfloat16).cuda()
+    p = S.pool_fwd(xb, cs)
+    assert torch.equal(p.cpu(), S.pool_fwd(_offset_copy(xb), cs).cpu())
+    assert torch.equal(S.up_fwd(p, shape[:4]).cpu(), S.up_fwd(_offset_copy(p), shape[:4]).cpu())
+    g = torch.randn(*shape, generator=gen).cuda()
+    gc = S.up_bwd(g, cs)
+    assert torch.equal(gc.cpu(), S.up_bwd(_offset_copy(g), cs).cpu())
+    assert torch.equal(S.pool_bwd(gc, shape[:4]).cpu(), S.pool_bwd(_offset_copy(gc), shape[:4]).cpu())
+    for n in (g.numel(), 1001, 4099):
+        x = torch.randn(n, generator=gen).cuda() * 4
+        gy = torch.randn(n, generator=gen).cuda()
+        assert torch.equal(S.silu_fwd(x).cpu(), S.silu_fwd(_offset_copy(x)).cpu())
+        assert torch.equal(S.silu_bwd(x, gy).cpu(), S.silu_bwd(_offset_copy(x), gy).cpu())
+        a, b = x.to(torch.bfloat16), gy.to(torch.bfloat16)
+        assert torch.equal(S.add_bf16(a, b).cpu(), S.add_bf16(_offset_copy(a), b).cpu())
+        assert torch.equal(S.add_f32(x, gy).cpu(), S.add_f32(_offset_copy(x), gy).cpu())
+
+
 def _torch_net(net, x):
     """CPU FP32 copy of UNet3D.forward with the same BF16 rounding points (autograd-capable)."""
     Ws = {k: torch_weight(bf(c.w.cpu()), c.cin, c.cout).clone().requires_grad_(True) for k, c in net.conv.items()}

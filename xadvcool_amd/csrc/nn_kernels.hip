@@ -15,6 +15,8 @@
 // Ba (2015) with bias correction; master weights and moments are FP32.
 #include <hip/hip_runtime.h>
 
+#include <initializer_list>
+
 #include <cmath>
 #include <cstdint>
 
@@ -556,7 +558,18 @@ __global__ void k_wgrad_tile_reduce(const float* __restrict__ part, float* __res
   float s = 0.0f;
   if (k < 27 * Cin) {
     const int grp = k / (9 * Cin), loc = k - grp * 9 * Cin;
-    for (int b = 0; b < nblk; ++b) s += part[(((int64_t)grp * nblk + b) * Cout + co) * (9 * Cin) + loc];
+    const int64_t stride = (int64_t)Cout * (9 * Cin);
+    const float* p = part + ((int64_t)grp * nblk * Cout + co) * (9 * Cin) + loc;
+    // block partials summed in block order; 16 loads in flight ahead of the adds
+    int b = 0;
+    for (; b + 16 <= nblk; b += 16) {
+      float w[16];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) w[q] = p[(int64_t)(b + q) * stride];
+#pragma unroll
+      for (int q = 0; q < 16; ++q) s += w[q];
+    }
+    for (; b < nblk; ++b) s += p[(int64_t)b * stride];
   }
   dW[i] = s;
 }
@@ -595,7 +608,15 @@ __global__ void k_bias_final(const float* __restrict__ part, float* __restrict__
   const int c = threadIdx.x;
   if (c >= C) return;
   float a = 0.0f;
-  for (int b = 0; b < nb; ++b) a += part[(int64_t)b * C + c];
+  int b = 0;  // block partials summed in block order; 16 loads in flight ahead of the adds
+  for (; b + 16 <= nb; b += 16) {
+    float w[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) w[q] = part[(int64_t)(b + q) * C + c];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) a += w[q];
+  }
+  for (; b < nb; ++b) a += part[(int64_t)b * C + c];
   db[c] = a;
 }
 
@@ -669,6 +690,141 @@ __global__ void k_up_fwd(const __bf16* __restrict__ in, __bf16* __restrict__ out
   const int CX = gf_geo.X / 2, CY = gf_geo.Y / 2, CZ = gf_geo.Z / 2;
   out[i] = in[((((int64_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2) * C + c];
 }
+// ------------------------------------------------- vectorised elementwise ---
+// The same arithmetic as the scalar kernels above, element for element (so the
+// same bits), with 16-byte accesses and 32-bit index math: one thread per 4
+// fp32 or 8 bf16 values, or per voxel and 8 (bf16) / 4 (fp32) channels.  The
+// host uses them when the element count fits 32-bit indices and the pointers
+// are 16-byte aligned; the scalar kernels remain for everything else.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void dec32(uint32_t v, int X, int Y, int Z, int& n, int& x, int& y, int& z) {
+  z = (int)(v % (uint32_t)Z);
+  uint32_t r = v / (uint32_t)Z;
+  y = (int)(r % (uint32_t)Y);
+  r /= (uint32_t)Y;
+  x = (int)(r % (uint32_t)X);
+  n = (int)(r / (uint32_t)X);
+}
+
+// n4 = n / 4 full groups; the tail (n % 4 elements) is done by the first threads.
+__global__ void k_silu_fwd4(const float* __restrict__ x, __bf16* __restrict__ y, uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n4 = n >> 2;
+  if (i < n4) {
+    const f32x4 v = reinterpret_cast<const f32x4*>(x)[i];
+    u16x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = __builtin_bit_cast(unsigned short, f2bf(silu(v[k])));
+    reinterpret_cast<u16x4*>(y)[i] = o;
+  }
+  const uint32_t t = 4 * n4 + i;
+  if (i < (n & 3)) y[t] = f2bf(silu(x[t]));
+}
+__global__ void k_silu_bwd4(const float* __restrict__ x, const float* __restrict__ gy, __bf16* __restrict__ gxb,
+                            uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n4 = n >> 2;
+  if (i < n4) {
+    const f32x4 v = reinterpret_cast<const f32x4*>(x)[i];
+    const f32x4 g = reinterpret_cast<const f32x4*>(gy)[i];
+    u16x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) o[k] = __builtin_bit_cast(unsigned short, f2bf(g[k] * dsilu(v[k])));
+    reinterpret_cast<u16x4*>(gxb)[i] = o;
+  }
+  const uint32_t t = 4 * n4 + i;
+  if (i < (n & 3)) gxb[t] = f2bf(gy[t] * dsilu(x[t]));
+}
+__global__ void k_add_f32_4(const float* __restrict__ a, const float* __restrict__ b, float* __restrict__ y,
+                            uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n4 = n >> 2;
+  if (i < n4) reinterpret_cast<f32x4*>(y)[i] = reinterpret_cast<const f32x4*>(a)[i] + reinterpret_cast<const f32x4*>(b)[i];
+  const uint32_t t = 4 * n4 + i;
+  if (i < (n & 3)) y[t] = a[t] + b[t];
+}
+__global__ void k_add_bf16_8(const __bf16* __restrict__ a, const __bf16* __restrict__ b, __bf16* __restrict__ y,
+                             uint32_t n) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t n8 = n >> 3;
+  if (i < n8) {
+    const bf16x8 va = __builtin_bit_cast(bf16x8, reinterpret_cast<const u16x8*>(a)[i]);
+    const bf16x8 vb = __builtin_bit_cast(bf16x8, reinterpret_cast<const u16x8*>(b)[i]);
+    u16x8 o;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = __builtin_bit_cast(unsigned short, f2bf(bf2f(va[k]) + bf2f(vb[k])));
+    reinterpret_cast<u16x8*>(y)[i] = o;
+  }
+  const uint32_t t = 8 * n8 + i;
+  if (i < (n & 7)) y[t] = f2bf(bf2f(a[t]) + bf2f(b[t]));
+}
+// nearest x2 upsampling, one thread per fine voxel and 8 channels (C % 8 == 0)
+__global__ void k_up_fwd8(const __bf16* __restrict__ in, __bf16* __restrict__ out, Geo gf, int C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C8 = (uint32_t)C >> 3;
+  if (i >= (uint32_t)gf.V() * C8) return;
+  const uint32_t v = i / C8, c8 = i - v * C8;
+  int n, x, y, z;
+  dec32(v, gf.X, gf.Y, gf.Z, n, x, y, z);
+  const int CX = gf.X / 2, CY = gf.Y / 2, CZ = gf.Z / 2;
+  const uint32_t cv = (((uint32_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2;
+  reinterpret_cast<u16x8*>(out)[i] = reinterpret_cast<const u16x8*>(in)[cv * C8 + c8];
+}
+// grad of upsampling, one thread per coarse voxel and 4 channels (C % 4 == 0): children summed in the
+// scalar kernel's order
+__global__ void k_up_bwd4(const float* __restrict__ gf, float* __restrict__ gc_, Geo gc, int C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C4 = (uint32_t)C >> 2;
+  if (i >= (uint32_t)gc.V() * C4) return;
+  const uint32_t v = i / C4, c4 = i - v * C4;
+  int n, x, y, z;
+  dec32(v, gc.X, gc.Y, gc.Z, n, x, y, z);
+  const int FX = gc.X * 2, FY = gc.Y * 2, FZ = gc.Z * 2;
+  f32x4 s = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int d = 0; d < 8; ++d) {
+    const int xx = 2 * x + (d >> 2), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d & 1);
+    s += reinterpret_cast<const f32x4*>(gf)[((((uint32_t)n * FX + xx) * FY + yy) * FZ + zz) * C4 + c4];
+  }
+  reinterpret_cast<f32x4*>(gc_)[i] = s;
+}
+// 2x2x2 average pooling, one thread per coarse voxel and 8 channels (C % 8 == 0)
+__global__ void k_pool_fwd8(const __bf16* __restrict__ in, __bf16* __restrict__ out, Geo gc, int C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C8 = (uint32_t)C >> 3;
+  if (i >= (uint32_t)gc.V() * C8) return;
+  const uint32_t v = i / C8, c8 = i - v * C8;
+  int n, x, y, z;
+  dec32(v, gc.X, gc.Y, gc.Z, n, x, y, z);
+  const int FX = gc.X * 2, FY = gc.Y * 2, FZ = gc.Z * 2;
+  float s[8] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+  for (int d = 0; d < 8; ++d) {
+    const int xx = 2 * x + (d >> 2), yy = 2 * y + ((d >> 1) & 1), zz = 2 * z + (d & 1);
+    const bf16x8 w = __builtin_bit_cast(
+        bf16x8, reinterpret_cast<const u16x8*>(in)[((((uint32_t)n * FX + xx) * FY + yy) * FZ + zz) * C8 + c8]);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) s[k] += bf2f(w[k]);
+  }
+  u16x8 o;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o[k] = __builtin_bit_cast(unsigned short, f2bf(s[k] * 0.125f));
+  reinterpret_cast<u16x8*>(out)[i] = o;
+}
+// grad of pooling, one thread per fine voxel and 4 channels (C % 4 == 0)
+__global__ void k_pool_bwd4(const float* __restrict__ gc_, float* __restrict__ gf, Geo gfg, int C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C4 = (uint32_t)C >> 2;
+  if (i >= (uint32_t)gfg.V() * C4) return;
+  const uint32_t v = i / C4, c4 = i - v * C4;
+  int n, x, y, z;
+  dec32(v, gfg.X, gfg.Y, gfg.Z, n, x, y, z);
+  const int CX = gfg.X / 2, CY = gfg.Y / 2, CZ = gfg.Z / 2;
+  const f32x4 g = reinterpret_cast<const f32x4*>(gc_)[((((uint32_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2) * C4 +
+                                                      c4];
+  reinterpret_cast<f32x4*>(gf)[i] = 0.125f * g;
+}
+
 // grad of upsampling: g_coarse = sum of the 8 fine children (fp32)
 __global__ void k_up_bwd(const float* __restrict__ gf, float* __restrict__ gc_, Geo gc, int C) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -731,6 +887,14 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 }
 
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
+
+// The vectorised elementwise kernels index elements with 32 bits and access 16 bytes at a time.
+inline bool vec_ok(int64_t nelem, std::initializer_list<const void*> ptrs) {
+  if (nelem <= 0 || nelem >= (int64_t(1) << 31)) return false;
+  for (const void* p : ptrs)
+    if (reinterpret_cast<uintptr_t>(p) % 16) return false;
+  return true;
+}
 
 }  // namespace
 
@@ -944,11 +1108,19 @@ int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void*
 int64_t xnn_bias_part_floats(int64_t V, int C) { return bias_blocks(V) * C; }
 
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream) {
-  hipLaunchKernelGGL(k_silu_fwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y, n);
+  if (vec_ok(n, {x, y}))
+    hipLaunchKernelGGL(k_silu_fwd4, dim3(nblk((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y,
+                       (uint32_t)n);
+  else
+    hipLaunchKernelGGL(k_silu_fwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y, n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_silu_bwd(const float* x, const float* gy, void* gxb, int64_t n, void* stream) {
-  hipLaunchKernelGGL(k_silu_bwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, gy, (__bf16*)gxb, n);
+  if (vec_ok(n, {x, gy, gxb}))
+    hipLaunchKernelGGL(k_silu_bwd4, dim3(nblk((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, x, gy,
+                       (__bf16*)gxb, (uint32_t)n);
+  else
+    hipLaunchKernelGGL(k_silu_bwd, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, gy, (__bf16*)gxb, n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_to_bf16(const float* x, void* y, int64_t n, void* stream) {
@@ -956,34 +1128,59 @@ int xnn_to_bf16(const float* x, void* y, int64_t n, void* stream) {
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_add_f32(const float* a, const float* b, float* y, int64_t n, void* stream) {
-  hipLaunchKernelGGL(k_add_f32, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, a, b, y, n);
+  if (vec_ok(n, {a, b, y}))
+    hipLaunchKernelGGL(k_add_f32_4, dim3(nblk((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, a, b, y,
+                       (uint32_t)n);
+  else
+    hipLaunchKernelGGL(k_add_f32, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, a, b, y, n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_add_bf16(const void* a, const void* b, void* y, int64_t n, void* stream) {
-  hipLaunchKernelGGL(k_add_bf16, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)a,
-                     (const __bf16*)b, (__bf16*)y, n);
+  if (vec_ok(n, {a, b, y}))
+    hipLaunchKernelGGL(k_add_bf16_8, dim3(nblk((n + 7) / 8)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)a,
+                       (const __bf16*)b, (__bf16*)y, (uint32_t)n);
+  else
+    hipLaunchKernelGGL(k_add_bf16, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)a,
+                       (const __bf16*)b, (__bf16*)y, n);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_pool_fwd(const void* in, void* out, int N, int CX, int CY, int CZ, int C, void* stream) {
   Geo gc{CX, CY, CZ, N};
-  hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)in,
-                     (__bf16*)out, gc, C);
+  if (C % 8 == 0 && vec_ok(8 * gc.V() * C, {in, out}))  // the fine input has 8x the coarse elements
+    hipLaunchKernelGGL(k_pool_fwd8, dim3(nblk(gc.V() * (C / 8))), dim3(256), 0, (hipStream_t)stream,
+                       (const __bf16*)in, (__bf16*)out, gc, C);
+  else
+    hipLaunchKernelGGL(k_pool_fwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)in,
+                       (__bf16*)out, gc, C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_pool_bwd(const float* gcoarse, float* gfine, int N, int FX, int FY, int FZ, int C, void* stream) {
   Geo gf{FX, FY, FZ, N};
-  hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, gcoarse, gfine, gf, C);
+  if (C % 4 == 0 && FX % 2 == 0 && FY % 2 == 0 && FZ % 2 == 0 && vec_ok(gf.V() * C, {gcoarse, gfine}))
+    hipLaunchKernelGGL(k_pool_bwd4, dim3(nblk(gf.V() * (C / 4))), dim3(256), 0, (hipStream_t)stream, gcoarse,
+                       gfine, gf, C);
+  else
+    hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, gcoarse, gfine, gf,
+                       C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_up_fwd(const void* in, void* out, int N, int FX, int FY, int FZ, int C, void* stream) {
   Geo gf{FX, FY, FZ, N};
-  hipLaunchKernelGGL(k_up_fwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)in,
-                     (__bf16*)out, gf, C);
+  if (C % 8 == 0 && FX % 2 == 0 && FY % 2 == 0 && FZ % 2 == 0 && vec_ok(gf.V() * C, {in, out}))
+    hipLaunchKernelGGL(k_up_fwd8, dim3(nblk(gf.V() * (C / 8))), dim3(256), 0, (hipStream_t)stream,
+                       (const __bf16*)in, (__bf16*)out, gf, C);
+  else
+    hipLaunchKernelGGL(k_up_fwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, (const __bf16*)in,
+                       (__bf16*)out, gf, C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_up_bwd(const float* gfine, float* gcoarse, int N, int CX, int CY, int CZ, int C, void* stream) {
   Geo gc{CX, CY, CZ, N};
-  hipLaunchKernelGGL(k_up_bwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, gfine, gcoarse, gc, C);
+  if (C % 4 == 0 && vec_ok(8 * gc.V() * C, {gfine, gcoarse}))  // the fine gradient has 8x the elements
+    hipLaunchKernelGGL(k_up_bwd4, dim3(nblk(gc.V() * (C / 4))), dim3(256), 0, (hipStream_t)stream, gfine,
+                       gcoarse, gc, C);
+  else
+    hipLaunchKernelGGL(k_up_bwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, gfine, gcoarse, gc, C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_mse(const float* pred, const float* target, const float* wmask, float* grad, float* part, int64_t V, int C,
