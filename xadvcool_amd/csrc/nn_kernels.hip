@@ -464,36 +464,69 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
 #pragma unroll
       for (int w = 0; w < CIT; ++w) acc[a][u][w] = floatx4{0, 0, 0, 0};
 
-  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  // Staging loads of a tile go to registers first (all of them in flight together), and the next
+  // tile's are issued before this tile's MFMAs; the LDS contents, and so the MFMA operands, are those
+  // of element-by-element staging.
+  constexpr int NEG = OC * ZZ * (Cout / 8), NEX = SC * ZZ * (Cin / 8);
+  constexpr int NG = (NEG + 255) / 256, NX = (NEX + 255) / 256;
+  u16x8 rg[NG], rx[NX];
+  auto fetch = [&](int64_t tile) {
     const int n = (int)(tile / ((int64_t)ntx * nty));
     const int r = (int)(tile - (int64_t)n * ntx * nty);
     const int x0 = (r / nty) * kWTX, y0 = (r % nty) * kWTY;
-    // upstream gradient: OC columns x ZZ voxels x Cout / 8 groups of 8 channels
-    for (int e = threadIdx.x; e < OC * ZZ * (Cout / 8); e += 256) {
-      const int cg = e % (Cout / 8), vz = e / (Cout / 8);
-      const int z = vz % ZZ, oc = vz / ZZ;
-      const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
-      u16x8 v = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (x < g.X && y < g.Y)
-        v = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * CoutT + co0 +
-                                             cg * 8);
 #pragma unroll
-      for (int c = 0; c < 8; ++c) reinterpret_cast<unsigned short*>(&sg[cg * 8 + c][0])[oc * ZZ + z] = v[c];
+    for (int q = 0; q < NG; ++q) {  // upstream gradient: OC columns x ZZ voxels x Cout / 8 groups of 8 channels
+      const int e = threadIdx.x + q * 256;
+      rg[q] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (e < NEG) {
+        const int cg = e % (Cout / 8), vz = e / (Cout / 8);
+        const int z = vz % ZZ, oc = vz / ZZ;
+        const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
+        if (x < g.X && y < g.Y)
+          rg[q] = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * CoutT +
+                                                  co0 + cg * 8);
+      }
     }
-    // input of the source columns at x + dx, three z-shifted copies
-    for (int e = threadIdx.x; e < SC * ZZ * (Cin / 8); e += 256) {
-      const int cg = e % (Cin / 8), vz = e / (Cin / 8);
-      const int z = vz % ZZ, sc = vz / ZZ;
-      const int x = x0 + sc / SY + dx, y = y0 + sc % SY - 1;
-      u16x8 v = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (x >= 0 && x < g.X && y >= 0 && y < g.Y)
-        v = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cin + cg * 8);
-      const int o = sc * ZZ + z;
 #pragma unroll
-      for (int c = 0; c < 8; ++c) {
-        if (z + 1 < ZZ) reinterpret_cast<unsigned short*>(&sx[0][cg * 8 + c][0])[o + 1] = v[c];  // dz = -1
-        reinterpret_cast<unsigned short*>(&sx[1][cg * 8 + c][0])[o] = v[c];                      // dz =  0
-        if (z > 0) reinterpret_cast<unsigned short*>(&sx[2][cg * 8 + c][0])[o - 1] = v[c];       // dz = +1
+    for (int q = 0; q < NX; ++q) {  // input of the source columns at x + dx
+      const int e = threadIdx.x + q * 256;
+      rx[q] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+      if (e < NEX) {
+        const int cg = e % (Cin / 8), vz = e / (Cin / 8);
+        const int z = vz % ZZ, sc = vz / ZZ;
+        const int x = x0 + sc / SY + dx, y = y0 + sc % SY - 1;
+        if (x >= 0 && x < g.X && y >= 0 && y < g.Y)
+          rx[q] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cin +
+                                                  cg * 8);
+      }
+    }
+  };
+  if (blockIdx.x < ntiles) fetch(blockIdx.x);
+  for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+#pragma unroll
+    for (int q = 0; q < NG; ++q) {
+      const int e = threadIdx.x + q * 256;
+      if (e < NEG) {
+        const int cg = e % (Cout / 8), vz = e / (Cout / 8);
+        const int z = vz % ZZ, oc = vz / ZZ;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) reinterpret_cast<unsigned short*>(&sg[cg * 8 + c][0])[oc * ZZ + z] = rg[q][c];
+      }
+    }
+    // three z-shifted copies of the input
+#pragma unroll
+    for (int q = 0; q < NX; ++q) {
+      const int e = threadIdx.x + q * 256;
+      if (e < NEX) {
+        const int cg = e % (Cin / 8), vz = e / (Cin / 8);
+        const int z = vz % ZZ, sc = vz / ZZ;
+        const int o = sc * ZZ + z;
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+          if (z + 1 < ZZ) reinterpret_cast<unsigned short*>(&sx[0][cg * 8 + c][0])[o + 1] = rx[q][c];  // dz = -1
+          reinterpret_cast<unsigned short*>(&sx[1][cg * 8 + c][0])[o] = rx[q][c];                      // dz =  0
+          if (z > 0) reinterpret_cast<unsigned short*>(&sx[2][cg * 8 + c][0])[o - 1] = rx[q][c];       // dz = +1
+        }
       }
     }
     // positions whose source z lies outside [0, ZZ): zero (never written above)
@@ -502,6 +535,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
       reinterpret_cast<unsigned short*>(&sx[0][ci][0])[sc * ZZ] = 0;
       reinterpret_cast<unsigned short*>(&sx[2][ci][0])[sc * ZZ + ZZ - 1] = 0;
     }
+    if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);  // in flight during this tile's MFMAs
     __syncthreads();
     const int a = lane >> 4, m = lane & 15;
 #pragma unroll
