@@ -275,21 +275,22 @@ class _Stall:
 
 
 def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None,
-                  confirm=CONFIRM, stop_rule="reference", launch=None):
+                  confirm=CONFIRM, stop_rule="reference", launch=None, min_steps_rule=False):
     """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
 
     Fallback: a warm design that stalls or diverges is re-run cold in the same call.  Both arms stop when the
     reference's residual test passes on ``confirm`` consecutive checks.  ``timer`` (optional):
     ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes; its ``stream`` is set to the solver's.
     ``launch``: launch-shape options of the fused pass (``block``, ``cells_per_thread``, ``xcd_remap``,
-    ``alternate_sweep``); they change timing only."""
+    ``alternate_sweep``); they change timing only.  ``min_steps_rule``: the reference's flow-through floor
+    (src/simulation.py:104; off by default here, on in the reference)."""
     if predictor is None and net is not None:
         predictor = lambda d: predict_fields(net, d)  # noqa: E731
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
                        metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule,
-                       **(launch or {}))
+                       min_steps_rule=min_steps_rule, **(launch or {}))
     queue = []
     t_inf = 0.0
     for d in designs:
@@ -342,9 +343,11 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
     return dict(results=final, wall=wall, inference_s=t_inf, fallbacks=fallbacks, lups=lups)
 
 
-def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print, confirm=CONFIRM, stop_rule="reference"):
-    cold = solve_designs(designs, None, slots, tol, max_steps, log, confirm=confirm, stop_rule=stop_rule)
-    warm = solve_designs(designs, net, slots, tol, max_steps, log, confirm=confirm, stop_rule=stop_rule)
+def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print, confirm=CONFIRM, stop_rule="reference",
+            min_steps_rule=False):
+    kw = dict(confirm=confirm, stop_rule=stop_rule, min_steps_rule=min_steps_rule)
+    cold = solve_designs(designs, None, slots, tol, max_steps, log, **kw)
+    warm = solve_designs(designs, net, slots, tol, max_steps, log, **kw)
     rows = []
     for d in designs:
         c, w = cold["results"][d["key"]], warm["results"][d["key"]]
@@ -402,6 +405,8 @@ def main():
                     help="also train on designs stopped at --gen-max-steps (not on diverged ones)")
     ap.add_argument("--init-weights", default=None, help="continue training from these weights (and their norm)")
     ap.add_argument("--warmup", type=int, default=100, help="linear learning-rate warm-up steps")
+    ap.add_argument("--floor", action="store_true",
+                    help="compare: the reference's flow-through floor on (no stop before it; src/simulation.py:104)")
     ap.add_argument("--families", default=None,
                     help="compare: comma-separated families (straight, pinfin, gyroid, schwarz_p, schwarz_d) to keep")
     ap.add_argument("--deadline-s", type=float, default=None,
@@ -442,8 +447,9 @@ def main():
         fams = set(args.families.split(",")) if args.families else None
         designs = make_designs(COMPARE_SEED0 if args.seed0 is None else args.seed0, args.n, families=fams)
         summ, rows = compare(net, designs, slots=args.slots, tol=args.tol, max_steps=args.max_steps, log=log,
-                             stop_rule=args.stop_rule)
+                             stop_rule=args.stop_rule, min_steps_rule=args.floor)
         summ["stop_rule"] = args.stop_rule
+        summ["flow_through_floor"] = bool(args.floor)
         by_family = {}
         for r in rows:
             fam = by_family.setdefault(r["family"], {"designs": 0, "cold_converged": 0, "warm_converged": 0,
