@@ -227,13 +227,17 @@ def _torch_net(net, x):
     return ndhwc(cv("head", d1)), Ws, Bs
 
 
-def test_network_forward_and_gradients(gpu):
+@pytest.mark.parametrize("shape", [(2, 16, 8, 8), (1, 8, 32, 32)])
+def test_network_forward_and_gradients(gpu, shape):
+    """The whole network against torch.autograd.  The (1, 8, 32, 32) lattice takes the LDS-halo
+    convolutions at the two finest levels (Z = 32 and 16) and the vectorised elementwise kernels."""
     from xadvcool_amd import surrogate as S
+    print(f"[nn] network forward and gradients on {shape}", flush=True)
     torch.manual_seed(0)
     net = S.UNet3D(seed=1)
-    x = torch.randn(2, 16, 8, 8, 16) * 0.5
-    tgt = torch.randn(2, 16, 8, 8, 5)
-    w = (torch.rand(2, 16, 8, 8, 5) > 0.3).float()
+    x = torch.randn(*shape, 16) * 0.5
+    tgt = torch.randn(*shape, 5)
+    w = (torch.rand(*shape, 5) > 0.3).float()
     out = net.forward(x.to(torch.bfloat16).cuda(), keep=True)
     ref, Ws, Bs = _torch_net(net, x)
     assert rel(out.cpu(), ref) < 2e-2
