@@ -1,30 +1,35 @@
 """Benchmark: converged cold-plate solves per hour on one MI355X, warm-started vs plain LBM.
 
-Workload (one "step" = one solve of one synthetic design; 24 designs by default): full-size single
-solves on a 148 x 128 x 32 lattice — gyroid TPMS core
-(sin x cos y + sin y cos z + sin z cos x < t, cell period in [16, 32], t in
-[-0.4, 0.4]), quad_core heat map on the base, inlet u in [0.01, 0.05], tau in
-[0.52, 0.56], heat power in [0.05, 0.20], all drawn from the design seed;
-FP32 D3Q19 fluid + D3Q19 thermal lattices; run to the residual tolerance
-(the reference's d_ke / dT / dQ residuals every 100 steps, floor off) or
-30,000 steps.  The same designs are solved twice, one after the other:
-cold (equilibrium at rest) and warm (surrogate-predicted rho, u, T; a warm
-run that stalls or diverges is re-run cold).  Each arm's wall time covers
-building its designs on the GPU from their parameters (mask, heat map, slot
-inputs), the solves, metrics and, for the warm arm, the surrogate inference
-and any fallback.  Before the timed arms, untimed like a compile step, the
-project's autotuner picks the fused pass's launch shape (threads per block,
-cells per thread, block order, sweep direction) on three other full-size
-designs; both arms use it (``--launch fixed`` takes the shape from the
-flags instead).
+Workload (default): surrogate-warm-started batched solving of 256 unseen
+seeded designs on 148 x 128 x 32 lattices — all five core families (straight
+channels, staggered pin fins, gyroid, Schwarz P and Schwarz D TPMS) and all
+ten heat maps, tau, heat power and inlet velocity drawn from each design's
+seed (seeds from 1,000,000; the surrogate never saw them).  FP32 D3Q19 fluid
++ D3Q19 thermal lattices.  Every design is solved twice, one arm after the
+other, three lattices per launch with retirement and refill: cold
+(equilibrium at rest) and warm (surrogate-predicted rho, u, T; a warm run
+that stalls, diverges, or converges to fields failing the physics
+acceptance checks is re-run cold).  Stop rule of both arms: the reference's
+residual formulas applied to the temperature rise and the absorbed heat
+(``--stop-rule rise``; the reference's own residuals with ``reference``),
+checked every 100 steps, below 1e-6 on five consecutive checks, flow-through
+floor off, at most 30,000 steps.  Each arm's wall time covers building its
+designs on the GPU from their parameters, the solves, metrics, checks and,
+for the warm arm, the surrogate inference and any fallback.  ``--steps``
+does not change the workload (a step is one design solved to the stop
+rule); ``--designs`` does.  The fused pass runs 256 threads x 2 cells,
+linear block order, alternating sweep (``--launch tune`` runs the project's
+autotuner first instead).
 
-Printed (one JSON line): warm-start converged solves/hour (``value``) with
-the cold-start figure and the relative differences of the converged pressure
-drop and thermal resistance, MLUPS (all cells x steps / wall time), the fused
-pass's achieved bandwidth against the MI355X HBM roofline (HIP events on the
-solver's own stream over every pass of both arms; algorithmic bytes 152 B per
-non-solid cell for the fluid lattice plus 152 B per cell for the thermal
-lattice, per pass), and the CPU oracle timed on this host as the baseline.
+Printed (one JSON line): warm-start converged solves/hour (``value``), the
+cold-start figure and their ratio, the agreement of the converged pressure
+drop and thermal resistance where both arms converged (count within 0.5 %,
+median and maximum differences), per-family counts, MLUPS (all cells x
+steps / wall time), the fused pass's achieved bandwidth against the MI355X
+HBM roofline (HIP events on the solver's own stream over every pass of both
+arms; algorithmic bytes 152 B per non-solid cell for the fluid lattice plus
+152 B per cell for the thermal lattice, per pass), and the CPU oracle timed
+on this host as the baseline.
 """
 from __future__ import annotations
 
@@ -363,8 +368,9 @@ def config2_bench(seed=0, tol=1e-6, max_steps=30000, cpu_seconds=12.0):
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=24, help="timed designs (each solved cold, then warm)")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed solves before the timed ones")
+    ap.add_argument("--steps", type=int, default=256,
+                    help="reported as given; the workload is --designs designs (a step = one design solved)")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed designs (1,000 steps each arm) before the timed ones")
     ap.add_argument("--max-steps", type=int, default=30000)
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
@@ -377,18 +383,25 @@ def main():
     ap.add_argument("--passes", type=int, default=400)
     ap.add_argument("--seed", type=int, default=0, help="config2 mode: design seed")
     ap.add_argument("--copy-mb", type=float, default=512.0)
-    ap.add_argument("--designs", type=int, default=64)
+    ap.add_argument("--designs", type=int, default=None,
+                    help="solves: timed designs (default 256); batch / designs modes: design count (default 64)")
+    ap.add_argument("--stop-rule", choices=["rise", "reference"], default="rise",
+                    help="solves: residuals of both arms (rise: the reference's formulas on the temperature rise "
+                         "and the absorbed heat)")
+    ap.add_argument("--rows", default=None, help="solves: also write the per-design rows to this JSON file")
     ap.add_argument("--slots", type=int, default=None, help="batch slots (default: 1 for passes, 64 for batch)")
-    ap.add_argument("--launch", choices=["tune", "fixed"], default="tune",
+    ap.add_argument("--launch", choices=["tune", "fixed"], default="fixed",
                     help="solves mode: autotune the fused pass's launch shape before the timed arms, or use the "
                          "--block/--cpt/--xcd/--sweep flags")
     ap.add_argument("--sweep", choices=["alternate", "forward"], default="alternate",
                     help="block order of the fused pass: alternate reverses every even pass (Infinity-Cache reuse)")
     ap.add_argument("--block", type=int, default=256, help="threads per block of the fused pass")
-    ap.add_argument("--cpt", type=int, default=1, help="lattice cells per thread of the fused pass")
+    ap.add_argument("--cpt", type=int, default=2, help="lattice cells per thread of the fused pass")
     ap.add_argument("--xcd", action="store_true", help="map consecutive lattice blocks to one XCD")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
     args = ap.parse_args()
+    if args.mode == "solves" and args.designs is None:
+        args.designs = 256
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the solver runs independent designs per GPU and has no multi-GPU "
@@ -401,10 +414,13 @@ def main():
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
     if args.mode == "ceilings":
+        # The bandwidth probe is a measurement aid, not part of the library: compiled here on demand.
         import subprocess
-        exe = os.path.join(ROOT, "xadvcool_amd", "bw_ceiling")
-        if not os.path.exists(exe):
-            raise SystemExit(f"{exe} is missing: run __graft_entry__.build() first")
+        from xadvcool_amd.build import HIPCC_FLAGS
+        src, exe = os.path.join(ROOT, "bw_ceiling.hip"), os.path.join(ROOT, "bw_ceiling")
+        if not os.path.exists(exe) or os.path.getmtime(exe) < os.path.getmtime(src):
+            flags = [f for f in HIPCC_FLAGS if f not in ("-shared", "-fPIC")]
+            subprocess.run([os.environ.get("HIPCC", "/opt/rocm/bin/hipcc"), *flags, "-o", exe, src], check=True)
         sys.exit(subprocess.run([exe]).returncode)
     if args.mode == "config1":
         return config1_bench()
@@ -413,40 +429,29 @@ def main():
     if args.mode == "train-step":
         return train_step_bench()
     if args.mode == "designs":
-        return designs_bench(args.designs)
+        return designs_bench(args.designs or 64)
     if args.mode == "batch":
-        print(json.dumps({"mode": "batch", **batch_bench(args.designs, args.slots or 64, args.max_steps, args.tol)}),
+        print(json.dumps({"mode": "batch", **batch_bench(args.designs or 64, args.slots or 64, args.max_steps,
+                                                         args.tol)}),
               flush=True)
         return
 
     from xadvcool_amd import surrogate as S
     from xadvcool_amd import warmstart as W
-    from xadvcool_amd.geometry import config2_design, config2_params
+    from xadvcool_amd.geometry import config2_design
 
-    def designs(seeds):
-        # Parameters only: each arm builds its designs on the GPU (xadv_geometry_mask, xadv_heat_map) and loads
-        # them from device memory (xadv_instance_load_device), inside its timed wall.
-        out = []
-        for sd in seeds:
-            p = config2_params(sd)
-            out.append(dict(params=p, key=sd, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"],
-                            heat_power=p["heat_power"], family=p["family"], heat_name=p["heat_name"]))
-        return out
-
-    seeds = [1000 + i for i in range(args.steps)]
-    timed = designs(seeds)
+    # The surrogate-warm-started batched workload: unseen seeded designs of all five families and all ten heat
+    # maps on 148x128x32, parameters only (each arm builds its designs on the GPU inside its wall time).
+    # --steps does not change it: a "step" here is one design solved to the stop rule.
+    timed = W.make_design_params(W.COMPARE_SEED0, args.designs)
     net = S.UNet3D.load_file(args.weights)
-    # Designs are solved three at a time (continuous batching): three 148x128x32 lattices per launch is the
-    # measured best per-design pass time (partly Infinity-Cache resident; one slot leaves the chip underfed,
-    # four fall out of the cache).
     nslots = args.slots or 3
     launch = dict(block=args.block, cells_per_thread=args.cpt, xcd_remap=args.xcd,
                   alternate_sweep=args.sweep == "alternate")
     tuned_us = None
     if args.launch == "tune":
-        # Untimed set-up, like a compile step: the project's autotuner times each launch shape on a batch of
-        # ``nslots`` full-size designs (seeds outside the timed set) and both arms use the fastest.  Every shape
-        # computes the same bits.
+        # Untimed set-up, like a compile step: the project's autotuner times each launch shape on ``nslots``
+        # full-size designs outside the timed set.  Every shape computes the same bits.
         from xadvcool_amd import autotune
         tune_designs = [config2_design(sd) for sd in range(900, 900 + nslots)]
         best, times = autotune.tune(tune_designs, LATTICE, passes=300, save=False,
@@ -454,65 +459,63 @@ def main():
         launch = {k: best[k] for k in ("block", "cells_per_thread", "xcd_remap", "alternate_sweep")}
         tuned_us = round(times[json.dumps(best, sort_keys=True)], 2)
         print(f"[bench] autotuned launch {launch}: {tuned_us} us/pass", file=sys.stderr, flush=True)
-    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps, launch=launch,
-              log=lambda m: print(m, file=sys.stderr, flush=True))  # progress to stderr; stdout is the result line
-    if args.warmup:
-        w = designs(range(900, 900 + args.warmup))
-        W.solve_designs(w, None, **kw)
-        W.solve_designs(w, net, **kw)
+    kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps, launch=launch, stop_rule=args.stop_rule,
+              confirm=W.CONFIRM, log=lambda m: print(m, file=sys.stderr, flush=True))
+    if args.warmup:  # untimed: kernel loads, graph captures, the surrogate's first run
+        w = W.make_design_params(900_000, args.warmup)
+        W.solve_designs(w, None, **dict(kw, max_steps=1000))
+        W.solve_designs(w, net, **dict(kw, max_steps=1000))
     timer = PassTimer(None)  # every pass of both arms (the population a kernel trace of this command averages)
+    t_start = time.perf_counter()
     cold = W.solve_designs(timed, None, timer=timer, **kw)
     warm = W.solve_designs(timed, net, timer=timer, **kw)
-
-    def arm(r):
-        res = [r["results"][sd] for sd in seeds]
-        n = sum(1 for x in res if x["converged"])
-        return res, n, 3600.0 * n / r["wall"]
-
-    cres, c_conv, c_rate = arm(cold)
-    wres, w_conv, w_rate = arm(warm)
-    d_dp, d_rth = [], []
-    for c, w in zip(cres, wres):
-        if c["converged"] and w["converged"]:
-            d_dp.append(abs(w["metrics"]["pressure_drop"] / c["metrics"]["pressure_drop"] - 1.0))
-            d_rth.append(abs(w["metrics"]["R_th"] / c["metrics"]["R_th"] - 1.0))
+    t_arms = time.perf_counter() - t_start
+    rows = W.compare_rows(timed, cold, warm)
+    summ = W.summarize(rows, cold, warm)
     cells = int(np.prod(LATTICE))
-    c_steps = [x["steps"] for x in cres]
-    w_steps = [x["steps"] + x.get("abandoned_steps", 0) for x in wres]
+    c_steps = [r["cold_steps"] for r in rows]
+    w_steps = [r["warm_steps"] for r in rows]
     pass_ms, npass, alg_bytes = timer.totals()
     avg_pass_ms = pass_ms / max(npass, 1)
     achieved = alg_bytes / (pass_ms * 1e-3) / 1e9
     counters = pass_counters()
-
+    c_rate, w_rate = summ["cold"]["solves_per_hr"], summ["warm"]["solves_per_hr"]
+    rule = ("rise residuals (the reference's d_ke / dT / dQ formulas on the temperature rise max T - T_in and the "
+            "absorbed heat sum(u_x T) - T_in sum(u_x))" if args.stop_rule == "rise"
+            else "the reference's d_ke / dT / dQ residuals")
     line = {
         "metric": "converged solves/hr, warm-start vs plain LBM; MLUPS and % of HBM peak",
-        "value": round(w_rate, 2),
-        "unit": "converged solves/hr (surrogate warm start, cold fallback, inference included)",
+        "value": w_rate,
+        "unit": "converged solves/hr (surrogate warm start, physics gate, cold fallback, inference included)",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1000.0 * warm["wall"] / max(args.steps, 1), 2),
+        "ms_per_step": round(1000.0 * warm["wall"] / max(len(timed), 1), 2),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic (seeded gyroid designs, quad_core heat map; surrogate trained on the solver's own "
-                "solutions of other seeds)",
-        "config": {"workload": "full-size single solves, 148x128x32 gyroid TPMS, to residual tol or 30000 steps, "
-                               "cold then warm-started",
-                   "lattice": list(LATTICE), "tol": args.tol, "max_steps": args.max_steps,
-                   "stop_rule": "reference d_ke/dT/dQ residuals every 100 steps, flow-through floor off",
-                   "seeds": seeds, "designs_per_launch": nslots,
+        "data": "synthetic: seeded unseen designs (seeds from 1,000,000), all five families and all ten heat maps; "
+                "surrogate trained only on the solver's own converged solutions of other seeds",
+        "config": {"workload": f"surrogate-warm-started batched solving: {len(timed)} unseen 148x128x32 designs "
+                               "(straight, pin fin, gyroid, Schwarz P, Schwarz D; ten heat maps), each solved cold "
+                               "and warm to the stop rule",
+                   "designs": len(timed), "lattice": list(LATTICE), "tol": args.tol, "max_steps": args.max_steps,
+                   "stop_rule": f"{rule} every 100 steps below tol on {W.CONFIRM} consecutive checks, "
+                                "flow-through floor off, both arms",
+                   "step_definition": "one design solved to the stop rule (--steps does not change the workload)",
+                   "designs_per_launch": nslots,
+                   "families": {f: v["designs"] for f, v in summ["by_family"].items()},
                    "launch": dict(launch, autotuned=args.launch == "tune", tuned_us_per_pass=tuned_us)},
-        "cold_start": {"solves_per_hr": round(c_rate, 2), "converged": c_conv, "wall_s": round(cold["wall"], 3),
-                       "steps": c_steps, "mlups": round(cells * sum(c_steps) / cold["wall"] / 1e6, 1)},
-        "warm_start": {"solves_per_hr": round(w_rate, 2), "converged": w_conv, "wall_s": round(warm["wall"], 3),
-                       "inference_s": round(warm["inference_s"], 3), "fallbacks": len(warm["fallbacks"]),
-                       "steps": w_steps, "both_converged": len(d_dp),
-                       "max_rel_d_pressure_drop": max(d_dp, default=None),
-                       "max_rel_d_R_th": max(d_rth, default=None)},
-        "warm_vs_cold": round(w_rate / c_rate, 3) if c_rate > 0 else None,
+        "cold_start": dict(summ["cold"], mlups=round(cells * sum(c_steps) / cold["wall"] / 1e6, 1)),
+        "warm_start": dict(summ["warm"], mlups=round(cells * sum(w_steps) / warm["wall"] / 1e6, 1)),
+        "warm_vs_cold": summ["speedup_solves_per_hr"],
+        "acceptance": {k: summ[k] for k in ("both_converged", "within_0.5pct", "warm_only_converged",
+                                            "max_d_pressure_drop", "median_d_pressure_drop", "max_d_R_th",
+                                            "median_d_R_th", "cold_converged_failing_gate")},
+        "by_family": summ["by_family"],
         "mlups": round(cells * (sum(c_steps) + sum(w_steps)) / (cold["wall"] + warm["wall"]) / 1e6, 1),
+        "timed_s": round(t_arms, 1),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": (int(counters["traffic_per_alg_byte"] * alg_bytes / max(npass, 1))
@@ -530,14 +533,14 @@ def main():
                              "in the 256 MB Infinity Cache; the all-HBM figure is bench.py --mode passes --slots 64"},
     }
     if not args.no_cpu:
-        d0 = dict(config2_design(seeds[0]), key=seeds[0])
+        d0 = dict(config2_design(0), key=0)  # a full-size gyroid design: the oracle's per-step rate
         mlups1, n1, t1 = cpu_baseline(d0, args.cpu_seconds, 1)
         cpu_s = sum(c_steps) * cells / (mlups1 * 1e6)  # the cold arm's lattice updates at the oracle's rate
         line["cpu_baseline"] = {
-            "value": round(3600.0 * c_conv / cpu_s, 4),
+            "value": round(3600.0 * summ["cold"]["converged"] / cpu_s, 4),
             "unit": "converged solves/hr (the cold arm's designs and steps at the oracle's measured MLUPS)",
             "cores": 1, "kind": "port", "mlups": round(mlups1, 3),
-            "sample": f"oracle six-kernel FP32 schedule, 148x128x32 design seed 1000, {n1} steps in {t1:.1f} s, "
+            "sample": f"oracle six-kernel FP32 schedule on a 148x128x32 gyroid design, {n1} steps in {t1:.1f} s, "
                       f"1 thread",
         }
         from oracle import oracle as orc
@@ -545,8 +548,11 @@ def main():
         mlups_n, n_n, t_n = cpu_baseline(d0, args.cpu_seconds / 2, nt)
         line["cpu_baseline"]["all_threads"] = {
             "threads": nt, "mlups": round(mlups_n, 3),
-            "solves_per_hr": round(3600.0 * c_conv / (sum(c_steps) * cells / (mlups_n * 1e6)), 4),
+            "solves_per_hr": round(3600.0 * summ["cold"]["converged"] / (sum(c_steps) * cells / (mlups_n * 1e6)), 4),
             "sample": f"{n_n} steps in {t_n:.1f} s"}
+    if args.rows:
+        with open(args.rows, "w") as f:
+            json.dump({"line": line, "rows": rows}, f)
     print(json.dumps(line), flush=True)
 
 

@@ -164,14 +164,52 @@ def normalized_heat(heat_source, shape):
     return hm
 
 
+def pump_update(u_in, sum_rho_in, sum_rho_out, c_in, c_out, target_power, mutation=0):
+    """One constant-power pump update (src/simulation.py:139-146).
+
+    The reference's operands are float32: the stats come back as a float32
+    array (stats_warp.numpy(), :128), u_in is read back from the float32
+    device array (:140), and NumPy's promotion keeps float32 when a float32
+    meets a Python float (0.005, 3.0, target_power, the clip bounds).  So
+    every operation here rounds to float32, in the reference's order; the
+    stats arrive as doubles and are rounded to float32 first, as the
+    reference's FP32 sums are.  Returns the new inlet velocity (np.float32).
+
+    ``mutation`` plants a mistake for the tests that must catch it:
+    1 = the arithmetic in float64 (rounded to float32 only at the end),
+    2 = no clip, 3 = the correction's sign flipped, 4 = the /3 of dP dropped.
+    """
+    if mutation == 1:
+        s8, s9 = float(np.float32(c_in)), float(np.float32(c_out))
+        rin = float(np.float32(sum_rho_in)) / s8 if s8 > 0 else 1.0
+        rout = float(np.float32(sum_rho_out)) / s9 if s9 > 0 else 1.0
+        power = (rin - rout) / 3.0 * (float(np.float32(u_in)) * s8)
+        return np.float32(np.clip(float(np.float32(u_in)) + 0.005 * (target_power - power), 0.0001, 0.15))
+    f = np.float32
+    u = f(u_in)
+    s5, s6, s8, s9 = f(sum_rho_in), f(sum_rho_out), f(c_in), f(c_out)
+    rho_in_avg = f(s5 / s8) if s8 > 0 else f(1.0)
+    rho_out_avg = f(s6 / s9) if s9 > 0 else f(1.0)
+    dP = f(f(rho_in_avg - rho_out_avg) / f(3.0)) if mutation != 4 else f(rho_in_avg - rho_out_avg)
+    power = f(dP * f(u * s8))
+    step = f(f(0.005) * f(f(target_power) - power))
+    if mutation == 3:
+        step = -step
+    if mutation == 2:
+        return f(u + step)
+    return f(min(max(f(u + step), f(0.0001)), f(0.15)))
+
+
 def run_simulation(mask, max_steps=40000, tau_fluid=0.52, heat_power=0.1, t_inlet=25.0, u_inlet_val=0.03,
                    stats_every=100, tol_ke=1e-7, tol_T=1e-7, pump_mode="constant_flow", target_power=1.0,
                    device="cpu", silent=True, heat_source=None, Pr=7.0, k_ratio=628.0, dtype=np.float32,
-                   min_steps_rule=True):
+                   min_steps_rule=True, pump_mutation=0):
     """src/simulation.py:14-162 on the oracle kernels.
 
     ``min_steps_rule=False`` drops the flow-through floor (simulation.py:104);
-    everything else is the reference loop.
+    everything else is the reference loop.  ``pump_mutation`` plants a pump
+    mistake for the tests: 1-4 as in pump_update, 5 = the controller gated at
+    step >= 2000 instead of > 2000.
     """
     mask = np.asarray(mask)
     nx, ny, nz = mask.shape
@@ -204,12 +242,10 @@ def run_simulation(mask, max_steps=40000, tau_fluid=0.52, heat_power=0.1, t_inle
             dT_conv = abs(max_T - prev_max_T) / (prev_max_T + 1e-9) / stats_every if step > stats_every else 0.0
             dQ_conv = abs(Q_out - prev_Q_out) / (abs(prev_Q_out) + 1e-12) / stats_every if step > stats_every else 1.0
             prev_ke, prev_max_T, prev_Q_out = ke, max_T, Q_out
-            if pump_mode == "constant_power" and step > 2000:
-                rho_in_avg = s[5] / s[8] if s[8] > 0 else 1.0
-                rho_out_avg = s[6] / s[9] if s[9] > 0 else 1.0
-                dP = (rho_in_avg - rho_out_avg) / 3.0
-                power = dP * (float(u_in) * s[8])
-                u_in = np.float32(np.clip(float(u_in) + 0.005 * (target_power - power), 0.0001, 0.15))
+            gate = step >= 2000 if pump_mutation == 5 else step > 2000
+            if pump_mode == "constant_power" and gate:
+                u_in = pump_update(u_in, s[5], s[6], s[8], s[9], target_power,
+                                   mutation=pump_mutation if pump_mutation < 5 else 0)
             if step > min_steps and d_ke < tol_ke and dT_conv < tol_T and dQ_conv < tol_T:
                 break
             if np.isnan(max_v) or max_v > 0.17 or np.isnan(max_T) or max_T > 1e4:

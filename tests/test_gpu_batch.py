@@ -1,14 +1,29 @@
-"""Batched solving: every slot of a mixed batch reproduces its own single solve bit for bit.
+"""Batched solving: every slot of a mixed batch reproduces the oracle's solve of its design bit for bit.
 
 Designs of different lattice sizes share one fused launch per step; slots
 retire at their own convergence and take queued designs (continuous
-batching).  Each result must match run_simulation on the same design (same
-stop step, identical FP32 fields), which the oracle pins separately.
+batching).  Each result must match the oracle's restatement of
+run_simulation (oracle/oracle.py: the reference's six-kernel schedule and
+loop) on the same design — same stop step, identical FP32 fields — and the
+HIP single solve of the same design.
 """
+import functools
+
 import numpy as np
 import pytest
 
+from oracle import oracle as orc
+
 pytestmark = pytest.mark.gpu
+
+
+@functools.lru_cache(maxsize=None)
+def oracle_solve(i):
+    d = designs()[i]
+    orc.set_threads(orc.max_threads())
+    return orc.run_simulation(d["mask"], max_steps=3000, tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
+                              u_inlet_val=d["u_inlet_val"], tol_ke=1e-5, tol_T=1e-5, heat_source=d["heat"],
+                              min_steps_rule=False)
 
 
 def designs():
@@ -43,7 +58,12 @@ def test_batch_matches_single_solves(gpu, alternate_sweep):
         ref = run_simulation(d["mask"], max_steps=3000, tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
                              u_inlet_val=d["u_inlet_val"], tol_ke=1e-5, tol_T=1e-5, silent=True,
                              heat_source=d["heat"], min_steps_rule=False)
-        print(f"[batch] design {d['key']}: steps {r['steps']} ({r['status']}) vs {ref['steps']}", flush=True)
-        assert r["steps"] == ref["steps"]
+        orc_ref = oracle_solve(d["key"])
+        print(f"[batch] design {d['key']}: steps {r['steps']} ({r['status']}) vs single {ref['steps']}, "
+              f"oracle {orc_ref['steps']}", flush=True)
+        assert r["steps"] == ref["steps"] == orc_ref["steps"]
+        assert r["converged"] == orc_ref["converged"]
         rho, u, T = r["fields"]
         assert np.array_equal(rho, ref["rho"]) and np.array_equal(u, ref["u"]) and np.array_equal(T, ref["T"])
+        for name, got, want in (("rho", rho, orc_ref["rho"]), ("u", u, orc_ref["u"]), ("T", T, orc_ref["T"])):
+            assert np.array_equal(got, want), f"design {d['key']}: {name} differs from the oracle"

@@ -1,0 +1,79 @@
+"""The batched workload at its own size: 64 designs of every family on uneven lattices in one launch per step.
+
+64 seeded designs (straight channels, staggered pin fins, gyroid, Schwarz P and
+Schwarz D cores; uniform, dual/quad-core, Gaussian-hotspot and peripheral-ring
+heat maps) on lattices with nx in [96, 148], ny in [64, 128], nz in [16, 32],
+all loaded into one 64-slot batch and advanced 200 steps together.  Then:
+
+* the smallest and the largest lattice of the batch equal the oracle's
+  solve of the same design (the reference's six-kernel schedule, FP32) bit
+  for bit in rho, u and T;
+* one more slot equals the HIP single solve of its design;
+* every slot's fields are finite, and the device physics-check reductions of
+  the smallest and largest slots equal the numpy restatement of
+  analyze_dataset.py:184-325 on the same fields (after 200 steps the flow is
+  still developing, so the checks' pass / fail verdicts are not asserted).
+"""
+import numpy as np
+import pytest
+
+from oracle import checks_ref
+from oracle import oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 200
+SEED0 = 2000  # the batched workload's seeds (bench.py --mode batch)
+
+
+def test_batch_of_64_uneven_lattices(gpu):
+    from xadvcool_amd.batch import BatchedSolver
+    from xadvcool_amd.engine import normalized_heat
+    from xadvcool_amd.geometry import family_design
+    from xadvcool_amd.simulation import run_simulation
+    ds = [dict(family_design(SEED0 + i), key=SEED0 + i) for i in range(64)]
+    shapes = [d["mask"].shape for d in ds]
+    assert all(96 <= s[0] <= 148 and 64 <= s[1] <= 128 and 16 <= s[2] <= 32 for s in shapes)
+    fams = {d["family"] for d in ds}
+    assert fams == {"straight", "pinfin", "gyroid", "schwarz_p", "schwarz_d"}, fams
+    print(f"[config3] 64 designs, {sum(int(np.prod(s)) for s in shapes)} cells, families {sorted(fams)}",
+          flush=True)
+    bs = BatchedSolver(64, (148, 128, 32), tol_ke=0.0, tol_T=0.0, max_steps=STEPS, collect_fields=True,
+                       metrics=True, checks=True)
+    try:
+        res = {r["key"]: r for r in bs.run(ds)}
+    finally:
+        bs.close()
+    assert len(res) == 64
+    for d in ds:
+        r = res[d["key"]]
+        assert r["steps"] == STEPS and r["status"] == "max_steps", (d["key"], r["status"], r["steps"])
+        for a in r["fields"]:
+            assert np.isfinite(a).all(), d["key"]
+        assert all(np.isfinite(v) for v in r["checks"].values() if not isinstance(v, bool)), d["key"]
+    cells = [int(np.prod(s)) for s in shapes]
+    small, large = int(np.argmin(cells)), int(np.argmax(cells))
+    orc.set_threads(orc.max_threads())
+    for i in (small, large):
+        d = ds[i]
+        print(f"[config3] oracle on design {d['key']} {d['family']} {d['mask'].shape}", flush=True)
+        want = orc.run_simulation(d["mask"], max_steps=STEPS, tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
+                                  u_inlet_val=d["u_inlet_val"], tol_ke=0.0, tol_T=0.0, heat_source=d["heat"],
+                                  min_steps_rule=False)
+        rho, u, T = res[d["key"]]["fields"]
+        for name, got, ref in (("rho", rho, want["rho"]), ("u", u, want["u"]), ("T", T, want["T"])):
+            assert np.array_equal(got, ref), f"design {d['key']}: {name} differs from the oracle"
+        hs = normalized_heat(d["heat"], d["mask"].shape)
+        ref_c = checks_ref.validate_fields(d["mask"], u, rho, T, hs, {"tau_fluid": d["tau_fluid"],
+                                                                      "heat_power": d["heat_power"]})
+        got_c = res[d["key"]]["checks"]
+        for k in ("mass_err", "mach", "rho_dev", "T_lower_bound", "delta_P", "mean_div", "max_div",
+                  "Q_source_rate", "Q_conv", "energy_err"):
+            assert got_c[k] == pytest.approx(ref_c[k], rel=2e-4, abs=1e-7), (d["key"], k)
+    mid = sorted(range(64), key=lambda i: cells[i])[32]
+    d = ds[mid]
+    one = run_simulation(d["mask"], max_steps=STEPS, tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
+                         u_inlet_val=d["u_inlet_val"], tol_ke=0.0, tol_T=0.0, silent=True, heat_source=d["heat"],
+                         min_steps_rule=False)
+    rho, u, T = res[d["key"]]["fields"]
+    assert np.array_equal(rho, one["rho"]) and np.array_equal(u, one["u"]) and np.array_equal(T, one["T"])

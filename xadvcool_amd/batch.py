@@ -63,7 +63,7 @@ class BatchedSolver:
                  max_steps=30000, min_steps_rule=False, Pr=7.0, k_ratio=628.0, t_inlet=25.0, precision=32,
                  block=256, cells_per_thread=1, xcd_remap=False, alternate_sweep=True, fluid_storage="auto",
                  collect_fields=False,
-                 metrics=True, stall_check=None, confirm=1, stop_rule="reference"):
+                 metrics=True, stall_check=None, confirm=1, stop_rule="reference", checks=False):
         self.slots = int(slots)
         self.cap_shape = tuple(cap_shape)
         self.batch = LatticeBatch(slots=self.slots, cap_shape=self.cap_shape, precision=precision, block=block,
@@ -76,6 +76,8 @@ class BatchedSolver:
         self.Pr, self.k_ratio, self.t_inlet = Pr, k_ratio, t_inlet
         self.collect_fields = collect_fields
         self.metrics = metrics
+        # Physics acceptance checks (analyze_dataset.py:184-325) of each retired slot's fields, on the device.
+        self.checks = checks
         # Optional callable(design, step, residuals) -> True to abandon a run
         # (used by the warm-start fallback to detect a stalled or growing residual).
         self.stall_check = stall_check
@@ -124,6 +126,12 @@ class BatchedSolver:
             core = (mask.shape[0] - 20, mask.shape[1])
             out["metrics"] = device_metrics(self.batch, s, mask, tau_fluid=d["tau_fluid"],
                                             u_inlet_val=d["u_inlet_val"], core=core, Pr=self.Pr)
+        if self.checks:
+            from .checks import device_checks
+            m = out.get("metrics", {})
+            out["checks"] = device_checks(self.batch, s, d["mask"], d["heat"], tau_fluid=d["tau_fluid"],
+                                          heat_power=d["heat_power"], Nu=m.get("Nu", 0.0),
+                                          f_friction=m.get("f_friction", 0.0), Pr=self.Pr, k_ratio=self.k_ratio)
         if self.collect_fields:
             rho, u, T = self.batch.fields(s)
             out["fields"] = (rho.cpu().numpy(), u.cpu().numpy(), T.cpu().numpy())

@@ -14,8 +14,6 @@ SRC = os.path.join(HERE, "csrc", "xadv.hip")
 LIB = os.path.join(HERE, "libxadv.so")
 NN_SRC = os.path.join(HERE, "csrc", "nn_kernels.hip")
 NN_LIB = os.path.join(HERE, "libxadv_nn.so")
-CEIL_SRC = os.path.join(HERE, "csrc", "bw_ceiling.hip")
-CEIL_EXE = os.path.join(HERE, "bw_ceiling")  # bandwidth-ceiling probe run by bench.py --mode ceilings
 ARCH = os.environ.get("XADV_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off: every + and * rounds separately, in source order, so the
@@ -47,7 +45,6 @@ def _hipcc(src, lib, verbose, shared=True):
 def build(verbose: bool = False) -> str:
     """Build the solver library and the surrogate library; returns the solver library's path."""
     _hipcc(NN_SRC, NN_LIB, verbose)
-    _hipcc(CEIL_SRC, CEIL_EXE, verbose, shared=False)
     return _hipcc(SRC, LIB, verbose)
 
 

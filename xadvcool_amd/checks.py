@@ -76,9 +76,25 @@ def conductivities(tau_fluid, Pr=7.0, k_ratio=628.0):
 
 def device_checks(batch, slot, mask, heat, *, tau_fluid, heat_power, Nu=0.0, f_friction=0.0, Re=1.0, Pr=7.0,
                   k_ratio=628.0):
-    """Checks of the slot's current fields (call after the solve's last readout)."""
+    """Checks of the slot's current fields (call after the solve's last readout).  ``heat``: the normalised heat
+    map as a host array or a device tensor."""
     kf, ks = conductivities(tau_fluid, Pr, k_ratio)
     r = batch.checks_raw(slot, kf, ks)
-    return close_checks(r, plane_n=mask.shape[1] * mask.shape[2], heat_sum=float(np.sum(heat, dtype=np.float64)),
+    if hasattr(heat, "double"):  # a torch tensor (on the device)
+        heat_sum = float(heat.double().sum())
+    else:
+        heat_sum = float(np.sum(heat, dtype=np.float64))
+    return close_checks(r, plane_n=mask.shape[1] * mask.shape[2], heat_sum=heat_sum,
                         tau_fluid=tau_fluid, heat_power=heat_power, Nu=Nu, f_friction=f_friction, Re=Re, Pr=Pr,
                         k_ratio=k_ratio)
+
+
+# The acceptance gate of a warm-started result: the field checks of validate_sample at their warning
+# levels (analyze_dataset.py:111-120) and the pressure-drop sign.  The Nusselt and f*Re ranges are left
+# out: they judge the design, not whether the solve reached its steady state.
+GATE = ("mass_warn", "mach_warn", "rho_dev_warn", "T_lb_warn", "delta_P_pass", "div_warn", "energy_warn")
+
+
+def gate_failures(c):
+    """Names of the gate checks that ``c`` (a close_checks / device_checks dict) fails; empty when it passes."""
+    return [k for k in GATE if not c[k]]

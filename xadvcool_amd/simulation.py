@@ -85,6 +85,27 @@ def min_steps_floor(nx, max_steps, u_inlet_val):
     return min(int(0.6 * max_steps), max(5000, int(2.0 * nx / max(u_inlet_val, 1e-6))))
 
 
+def pump_update(u_in, sum_rho_in, sum_rho_out, c_in, c_out, target_power):
+    """One update of the constant-power pump (src/simulation.py:139-146), in float32 as the reference does it.
+
+    In the reference every operand is float32: the stats are read back as a
+    float32 array, u_in from the float32 device scalar, and NumPy keeps
+    float32 when it meets a Python float (0.005, 3.0, target_power, the clip
+    bounds).  Here the stats are FP64 sums, rounded to float32 first; then
+    each operation rounds to float32 in the reference's order:
+    u += 0.005 (P_target - dP u c_in) with dP = (mean rho(x=1) - mean rho(x=nx-2)) / 3,
+    clipped to [1e-4, 0.15].  Returns np.float32."""
+    f32 = np.float32
+    u = f32(u_in)
+    rin, rout, cin, cout = f32(sum_rho_in), f32(sum_rho_out), f32(c_in), f32(c_out)
+    rho_in_avg = f32(rin / cin) if cin > 0 else f32(1.0)
+    rho_out_avg = f32(rout / cout) if cout > 0 else f32(1.0)
+    dP = f32(f32(rho_in_avg - rho_out_avg) / f32(3.0))
+    power = f32(dP * f32(u * cin))
+    u_new = f32(u + f32(f32(0.005) * f32(f32(target_power) - power)))
+    return f32(min(max(u_new, f32(0.0001)), f32(0.15)))
+
+
 def solve_on_batch(batch, slot, *, nx, max_steps, tau_fluid, u_inlet_val, stats_every=100, tol_ke=1e-7, tol_T=1e-7,
                    pump_mode="constant_flow", target_power=1.0, min_steps_rule=True, pbar=None, Dh=0.0,
                    timer=None):
@@ -114,11 +135,8 @@ def solve_on_batch(batch, slot, *, nx, max_steps, tau_fluid, u_inlet_val, stats_
         s = dict(zip(STAT_NAMES, batch.stats()[slot].tolist()))
         res.update(step, s)
         if pump_mode == "constant_power" and step > 2000:
-            rho_in_avg = s["sum_rho_in"] / s["c_in"] if s["c_in"] > 0 else 1.0
-            rho_out_avg = s["sum_rho_out"] / s["c_out"] if s["c_out"] > 0 else 1.0
-            dP_measured = (rho_in_avg - rho_out_avg) / 3.0
-            power_measured = dP_measured * (u_in_eff * s["c_in"])
-            u_in_eff = float(np.float32(np.clip(u_in_eff + 0.005 * (target_power - power_measured), 0.0001, 0.15)))
+            u_in_eff = float(pump_update(u_in_eff, s["sum_rho_in"], s["sum_rho_out"], s["c_in"], s["c_out"],
+                                         target_power))
             batch.set_inlet_velocity(slot, u_in_eff)
         if pbar is not None:
             pbar.set_postfix({"Re": f"{(u_in_eff * Dh) / ((tau_fluid - 0.5) / 3.0):.1f}", "MaxV": f"{res.max_v:.4f}",

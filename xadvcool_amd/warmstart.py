@@ -28,7 +28,8 @@ import torch
 
 from . import surrogate as S
 from .batch import BatchedSolver, device_inputs
-from .geometry import HEAT_SOURCE_NAMES, family_design
+from .checks import gate_failures
+from .geometry import HEAT_SOURCE_NAMES, family_design, family_params
 
 CORE = (128, 128, 32)  # 148 x 128 x 32 lattices
 PAD = (S.round8(CORE[0] + 20), S.round8(CORE[1]), S.round8(CORE[2]))
@@ -40,15 +41,26 @@ DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warm
 def make_designs(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
     """n seeded 148x128x32 designs from seed0 on; with ``families``, the first n of those families."""
     out = []
+    for p in make_design_params(seed0, n, heats, families):
+        d = family_design(p["key"], core=CORE, heat_name=p["heat_name"])
+        d["key"] = p["key"]
+        out.append(d)
+    return out
+
+
+def make_design_params(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
+    """The designs of ``make_designs`` as parameters only (``params`` plus the solver's scalars): the
+    batched solver builds each one on the GPU when its slot starts."""
+    out = []
     s = seed0
     while len(out) < n:
         rng = np.random.default_rng(10_000_019 * (s + 1))
         heat = heats[int(rng.integers(0, len(heats)))]
-        d = family_design(s, core=CORE, heat_name=heat)
-        d["key"] = s
+        p = family_params(s, core=CORE, heat_name=heat)
+        if families is None or p["family"] in families:
+            out.append(dict(params=p, key=s, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"],
+                            heat_power=p["heat_power"], family=p["family"], heat_name=p["heat_name"]))
         s += 1
-        if families is None or d["family"] in families:
-            out.append(d)
     return out
 
 
@@ -275,11 +287,13 @@ class _Stall:
 
 
 def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None,
-                  confirm=CONFIRM, stop_rule="reference", launch=None, min_steps_rule=False):
+                  confirm=CONFIRM, stop_rule="reference", launch=None, min_steps_rule=False, gate=True):
     """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
 
-    Fallback: a warm design that stalls or diverges is re-run cold in the same call.  Both arms stop when the
-    reference's residual test passes on ``confirm`` consecutive checks.  ``timer`` (optional):
+    Fallback: a warm design that stalls or diverges is re-run cold in the same call, and so is a warm design
+    that converges to fields failing the physics acceptance gate (``gate``: checks.GATE, the field checks of
+    analyze_dataset.py:184-325 reduced on the device; every result carries its ``checks``).  Both arms stop
+    when the reference's residual test passes on ``confirm`` consecutive checks.  ``timer`` (optional):
     ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes; its ``stream`` is set to the solver's.
     ``launch``: launch-shape options of the fused pass (``block``, ``cells_per_thread``, ``xcd_remap``,
     ``alternate_sweep``); they change timing only.  ``min_steps_rule``: the reference's flow-through floor
@@ -290,7 +304,7 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
     t0 = time.perf_counter()
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
                        metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule,
-                       min_steps_rule=min_steps_rule, **(launch or {}))
+                       min_steps_rule=min_steps_rule, checks=gate, **(launch or {}))
     queue = []
     t_inf = 0.0
     for d in designs:
@@ -325,12 +339,19 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
             pending = []
             for r in res:
                 k = r["key"]
-                if predictor is not None and k not in fell_back and r["status"] in ("stalled", "diverged"):
-                    # the warm start did not lead to convergence: rerun this design cold
+                why = None
+                if predictor is not None and k not in fell_back:
+                    if r["status"] in ("stalled", "diverged"):
+                        why = r["status"]  # the warm start did not lead to convergence
+                    elif gate and r["converged"]:
+                        failed = gate_failures(r["checks"])
+                        if failed:  # converged to fields that fail the physics checks
+                            why = "checks:" + ",".join(failed)
+                if why is not None:  # rerun this design cold
                     cold = {kk: v for kk, v in by_key[k].items() if kk not in ("init_fields", "warm")}
                     pending.append(cold)
                     fell_back.add(k)
-                    fallbacks.append((k, r["status"], r["steps"]))
+                    fallbacks.append((k, why, r["steps"]))
                     final[k] = {"abandoned_steps": r["steps"]}
                 else:
                     r["abandoned_steps"] = final.get(k, {}).get("abandoned_steps", 0)
@@ -343,40 +364,83 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
     return dict(results=final, wall=wall, inference_s=t_inf, fallbacks=fallbacks, lups=lups)
 
 
-def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print, confirm=CONFIRM, stop_rule="reference",
-            min_steps_rule=False):
-    kw = dict(confirm=confirm, stop_rule=stop_rule, min_steps_rule=min_steps_rule)
-    cold = solve_designs(designs, None, slots, tol, max_steps, log, **kw)
-    warm = solve_designs(designs, net, slots, tol, max_steps, log, **kw)
+def compare_rows(designs, cold, warm):
+    """Per-design comparison of a cold and a warm ``solve_designs`` run over the same designs."""
     rows = []
     for d in designs:
         c, w = cold["results"][d["key"]], warm["results"][d["key"]]
-        row = dict(key=d["key"], family=d["family"], heat=d["heat_name"], cold_steps=c["steps"],
-                   warm_steps=w["steps"] + w.get("abandoned_steps", 0), cold_status=c["status"],
-                   warm_status=w["status"])
+        row = dict(key=d["key"], family=d["family"], heat=d["heat_name"], u_in=d["u_inlet_val"],
+                   cold_steps=c["steps"], warm_steps=w["steps"] + w.get("abandoned_steps", 0),
+                   cold_status=c["status"], warm_status=w["status"],
+                   cold_gate=gate_failures(c["checks"]) if "checks" in c else None,
+                   warm_gate=gate_failures(w["checks"]) if "checks" in w else None)
         if c["converged"] and w["converged"]:
             for m in ("pressure_drop", "R_th", "Nu", "P_pump"):
                 a, b = w["metrics"][m], c["metrics"][m]
                 row["d_" + m] = abs(a - b) / max(abs(b), 1e-30)
         rows.append(row)
+    return rows
+
+
+def summarize(rows, cold, warm):
+    """Solves per hour of both arms, their ratio, and the agreement of the converged results.
+
+    Converged results are compared where both arms converged ("both"); a warm result whose cold run hit
+    the step cap has no cold result to judge it and is counted apart ("warm_only": accepted by the physics
+    gate alone)."""
+    def arm(a, key):
+        n = sum(r[key] == "converged" for r in rows)
+        return {"converged": n, "wall_s": round(a["wall"], 2), "solves_per_hr": round(3600 * n / a["wall"], 1),
+                "mean_steps": float(np.mean([r[key.replace("status", "steps")] for r in rows]))}
+
     both = [r for r in rows if "d_R_th" in r]
+    within = [r for r in both if r["d_pressure_drop"] <= 5e-3 and r["d_R_th"] <= 5e-3]
+    reasons = {}
+    for _, why, _ in warm["fallbacks"]:
+        key = why.split(":")[0]
+        reasons[key] = reasons.get(key, 0) + 1
     summ = {
-        "designs": len(designs),
-        "cold": {"converged": sum(r["converged"] for r in cold["results"].values()), "wall_s": round(cold["wall"], 2),
-                 "solves_per_hr": round(3600 * sum(r["converged"] for r in cold["results"].values()) / cold["wall"], 1),
-                 "mean_steps": float(np.mean([r["cold_steps"] for r in rows]))},
-        "warm": {"converged": sum(r["converged"] for r in warm["results"].values()), "wall_s": round(warm["wall"], 2),
-                 "inference_s": round(warm["inference_s"], 2), "fallbacks": len(warm["fallbacks"]),
-                 "solves_per_hr": round(3600 * sum(r["converged"] for r in warm["results"].values()) / warm["wall"], 1),
-                 "mean_steps": float(np.mean([r["warm_steps"] for r in rows]))},
+        "designs": len(rows),
+        "cold": arm(cold, "cold_status"),
+        "warm": dict(arm(warm, "warm_status"), inference_s=round(warm["inference_s"], 2),
+                     fallbacks=len(warm["fallbacks"]), fallback_reasons=reasons),
         "both_converged": len(both),
+        "within_0.5pct": len(within),
+        "warm_only_converged": sum(1 for r in rows if r["warm_status"] == "converged"
+                                   and r["cold_status"] != "converged"),
         "max_d_pressure_drop": max((r["d_pressure_drop"] for r in both), default=None),
+        "median_d_pressure_drop": float(np.median([r["d_pressure_drop"] for r in both])) if both else None,
         "max_d_R_th": max((r["d_R_th"] for r in both), default=None),
         "median_d_R_th": float(np.median([r["d_R_th"] for r in both])) if both else None,
-        "within_0.5pct": sum(1 for r in both if r["d_pressure_drop"] <= 5e-3 and r["d_R_th"] <= 5e-3),
+        "cold_converged_failing_gate": sum(1 for r in rows if r["cold_status"] == "converged" and r["cold_gate"]),
     }
-    summ["speedup_solves_per_hr"] = round(summ["warm"]["solves_per_hr"] / max(summ["cold"]["solves_per_hr"], 1e-9), 3)
-    return summ, rows
+    summ["speedup_solves_per_hr"] = round(summ["warm"]["solves_per_hr"] / max(summ["cold"]["solves_per_hr"], 1e-9),
+                                          3)
+    by_family = {}
+    for r in rows:
+        fam = by_family.setdefault(r["family"], {"designs": 0, "cold_converged": 0, "warm_converged": 0,
+                                                 "both": 0, "within_0.5pct": 0, "cold_steps": 0, "warm_steps": 0})
+        fam["designs"] += 1
+        fam["cold_converged"] += r["cold_status"] == "converged"
+        fam["warm_converged"] += r["warm_status"] == "converged"
+        fam["cold_steps"] += r["cold_steps"]
+        fam["warm_steps"] += r["warm_steps"]
+        if "d_R_th" in r:
+            fam["both"] += 1
+            fam["within_0.5pct"] += r["d_R_th"] <= 5e-3 and r["d_pressure_drop"] <= 5e-3
+    for fam in by_family.values():  # steps-weighted: each arm's share of the GPU time goes with its steps
+        fam["steps_ratio_cold_over_warm"] = round(fam["cold_steps"] / max(fam["warm_steps"], 1), 3)
+    summ["by_family"] = by_family
+    return summ
+
+
+def compare(net, designs, slots=64, tol=1e-6, max_steps=30000, log=print, confirm=CONFIRM, stop_rule="reference",
+            min_steps_rule=False):
+    kw = dict(confirm=confirm, stop_rule=stop_rule, min_steps_rule=min_steps_rule)
+    cold = solve_designs(designs, None, slots, tol, max_steps, log, **kw)
+    warm = solve_designs(designs, net, slots, tol, max_steps, log, **kw)
+    rows = compare_rows(designs, cold, warm)
+    return summarize(rows, cold, warm), rows
 
 
 def main():
@@ -450,17 +514,6 @@ def main():
                              stop_rule=args.stop_rule, min_steps_rule=args.floor)
         summ["stop_rule"] = args.stop_rule
         summ["flow_through_floor"] = bool(args.floor)
-        by_family = {}
-        for r in rows:
-            fam = by_family.setdefault(r["family"], {"designs": 0, "cold_converged": 0, "warm_converged": 0,
-                                                     "both": 0, "within_0.5pct": 0})
-            fam["designs"] += 1
-            fam["cold_converged"] += r["cold_status"] == "converged"
-            fam["warm_converged"] += r["warm_status"] == "converged"
-            if "d_R_th" in r:
-                fam["both"] += 1
-                fam["within_0.5pct"] += r["d_R_th"] <= 5e-3 and r["d_pressure_drop"] <= 5e-3
-        summ["by_family"] = by_family
         with open(f"{args.out}/warmstart_compare.json", "w") as f:
             json.dump({"summary": summ, "rows": rows}, f)
         log(json.dumps(summ))
