@@ -33,6 +33,7 @@ int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void*
 int64_t xnn_bias_part_floats(int64_t V, int C);
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream);
 int xnn_silu_bwd(const float* x, const float* gy, void* gxb, int64_t n, void* stream);
+int xnn_silu_bwd_f32(const float* x, const float* gy, float* gx, int64_t n, void* stream);
 int xnn_to_bf16(const float* x, void* y, int64_t n, void* stream);
 int xnn_add_f32(const float* a, const float* b, float* y, int64_t n, void* stream);
 int xnn_add_bf16(const void* a, const void* b, void* y, int64_t n, void* stream);
@@ -49,6 +50,23 @@ int xnn_mse_blocks(int64_t V, int C);
 /* Adam with bias correction on FP32 master weights; wb (may be NULL) receives the BF16 copy. */
 int xnn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, float lr, float b1, float b2,
              float eps, int step, float wd, void* stream);
+
+/* FiLM modulation of pre-activations p (N, Vn, C) FP32 by per-sample rows film (N, L):
+ * y = p * (1 + film[n][off + c]) + film[n][off + C + c]  (Perez et al. 2018).  The reference's benchmark
+ * U-Net conditions its bottleneck on the design scalars the same way (benchmark/models/unet3d.py:34-45,83-84). */
+int xnn_film_fwd(const float* p, const float* film, float* y, int N, int64_t Vn, int C, int off, int L,
+                 void* stream);
+/* Backward: gx = gy * (1 + gam); dfilm[n][off + c] = sum_v gy * p, dfilm[n][off + C + c] = sum_v gy (per-block
+ * partials in part, xnn_film_part_floats(N, Vn, C) floats, summed in a fixed order).  C divides 256. */
+int xnn_film_bwd(const float* p, const float* gy, const float* film, float* gx, float* part, float* dfilm, int N,
+                 int64_t Vn, int C, int off, int L, void* stream);
+int64_t xnn_film_part_floats(int N, int64_t Vn, int C);
+/* Fully connected FP32 layer: pre = W x + b (W: O x I), post = silu(pre) if act else pre (post may be NULL). */
+int xnn_dense_fwd(const float* x, const float* W, const float* b, float* pre, float* post, int N, int I, int O,
+                  int act, void* stream);
+/* Its gradients from gy (w.r.t. post): dW, db, and gx (w.r.t. x; NULL to skip). */
+int xnn_dense_bwd(const float* x, const float* W, const float* pre, const float* gy, float* dW, float* db, float* gx,
+                  int N, int I, int O, int act, void* stream);
 
 #ifdef __cplusplus
 }

@@ -12,7 +12,11 @@ all loaded into one 64-slot batch and advanced 200 steps together.  Then:
 * every slot's fields are finite, and the device physics-check reductions of
   the smallest and largest slots equal the numpy restatement of
   analyze_dataset.py:184-325 on the same fields (after 200 steps the flow is
-  still developing, so the checks' pass / fail verdicts are not asserted).
+  still developing, so the checks' pass / fail verdicts are not asserted);
+* a slot the solver retires as diverged (the reference's rule: max |u| over
+  fluid cells above 0.17, or NaN; the reference's generator has such designs
+  too, u_in reaches 0.05 in narrow channels) diverges in the oracle at the
+  same step.
 """
 import numpy as np
 import pytest
@@ -45,15 +49,27 @@ def test_batch_of_64_uneven_lattices(gpu):
     finally:
         bs.close()
     assert len(res) == 64
+    orc.set_threads(orc.max_threads())
+    diverged = [d for d in ds if res[d["key"]]["status"] == "diverged"]
+    assert len(diverged) <= 8, [d["key"] for d in diverged]
     for d in ds:
         r = res[d["key"]]
+        if r["status"] == "diverged":
+            print(f"[config3] design {d['key']} ({d['family']}, u_in {d['u_inlet_val']:.4f}) diverged at step "
+                  f"{r['steps']}: max |u| {r['max_v']:.3f}; oracle on the same design", flush=True)
+            want = orc.run_simulation(d["mask"], max_steps=STEPS, tau_fluid=d["tau_fluid"],
+                                      heat_power=d["heat_power"], u_inlet_val=d["u_inlet_val"], tol_ke=0.0,
+                                      tol_T=0.0, heat_source=d["heat"], min_steps_rule=False)
+            s = orc.stats(want["rho"], want["u"], want["T"], d["mask"])
+            assert want["steps"] == r["steps"] and (np.isnan(s[0]) or s[0] > 0.17 or np.isnan(s[11]))
+            continue
         assert r["steps"] == STEPS and r["status"] == "max_steps", (d["key"], r["status"], r["steps"])
         for a in r["fields"]:
             assert np.isfinite(a).all(), d["key"]
         assert all(np.isfinite(v) for v in r["checks"].values() if not isinstance(v, bool)), d["key"]
-    cells = [int(np.prod(s)) for s in shapes]
-    small, large = int(np.argmin(cells)), int(np.argmax(cells))
-    orc.set_threads(orc.max_threads())
+    cells = [int(np.prod(s)) if res[d["key"]]["status"] != "diverged" else -1 for s, d in zip(shapes, ds)]
+    large = int(np.argmax(cells))
+    small = int(np.argmin([c if c > 0 else 1 << 40 for c in cells]))
     for i in (small, large):
         d = ds[i]
         print(f"[config3] oracle on design {d['key']} {d['family']} {d['mask'].shape}", flush=True)
@@ -70,7 +86,7 @@ def test_batch_of_64_uneven_lattices(gpu):
         for k in ("mass_err", "mach", "rho_dev", "T_lower_bound", "delta_P", "mean_div", "max_div",
                   "Q_source_rate", "Q_conv", "energy_err"):
             assert got_c[k] == pytest.approx(ref_c[k], rel=2e-4, abs=1e-7), (d["key"], k)
-    mid = sorted(range(64), key=lambda i: cells[i])[32]
+    mid = sorted((i for i in range(64) if cells[i] > 0), key=lambda i: cells[i])[24]
     d = ds[mid]
     one = run_simulation(d["mask"], max_steps=STEPS, tau_fluid=d["tau_fluid"], heat_power=d["heat_power"],
                          u_inlet_val=d["u_inlet_val"], tol_ke=0.0, tol_T=0.0, silent=True, heat_source=d["heat"],

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 import torch
 
-from test_gpu_surrogate import _torch_net, rel, torch_weight
+from test_gpu_surrogate import _torch_net, randomize_zero_layers, rel, torch_weight
 
 pytestmark = pytest.mark.gpu
 
@@ -33,28 +33,32 @@ def test_full_size_forward_and_gradients(gpu):
     from xadvcool_amd import surrogate as S
     from xadvcool_amd import warmstart as W
     net = S.UNet3D.load_file(W.DEFAULT_WEIGHTS)
-    (x, t, w), = _samples(1)
+    randomize_zero_layers(net)  # any layer still at its zero start gets values, so every gradient counts
+    (x, t, w, sc), = _samples(1)
     assert tuple(x.shape) == (152, 128, 32, S.CIN)
     mean, std = (torch.tensor(a, device="cuda") for a in net.norm)
     tn = ((t - mean) / std).contiguous()
     print("[config4] full-size forward + backward, batch 1", flush=True)
-    out = net.forward(x[None].contiguous(), keep=True)
+    out = net.forward(x[None].contiguous(), sc[None].contiguous(), keep=True)
     loss, g = S.mse_loss(out, tn[None].contiguous(), w[None].contiguous())
     net.zero_grad()
     net.backward(g)
-    ref, Ws, Bs = _torch_net(net, x[None].float().cpu())
+    ref, Ws, Bs, D = _torch_net(net, x[None].float().cpu(), sc[None].cpu())
     assert rel(out.cpu(), ref.detach()) < 2e-2
     tc, wc = tn[None].cpu(), w[None].cpu()
     lref = ((ref[..., :S.NOUT] - tc) ** 2 * wc).sum() / wc.sum()
     lref.backward()
     assert abs(loss.item() - lref.item()) / lref.item() < 2e-2
-    for k, c in net.conv.items():
-        gw = torch_weight(c.dw.cpu(), c.cin, c.cout)
-        e = rel(gw, Ws[k].grad)
-        cos = float(torch.nn.functional.cosine_similarity(gw.flatten(), Ws[k].grad.flatten(), dim=0))
+    grads = [(k, torch_weight(c.dw.cpu(), c.cin, c.cout), Ws[k].grad, c.db.cpu(), Bs[k].grad)
+             for k, c in net.conv.items()]
+    grads += [(k, getattr(net, k).dw.cpu(), D[k][0].grad, getattr(net, k).db.cpu(), D[k][1].grad)
+              for k in ("film1", "film2")]
+    for k, gw, rw, gb, rb in grads:
+        e = rel(gw, rw)
+        cos = float(torch.nn.functional.cosine_similarity(gw.flatten(), rw.flatten(), dim=0))
         print(f"[config4] grad {k}: rel {e:.3e} cos {cos:.5f}", flush=True)
         assert e < 6e-2 and cos > 0.998, k
-        assert rel(c.db.cpu(), Bs[k].grad) < 6e-2, k
+        assert rel(gb, rb) < 6e-2, k
 
 
 def test_batch8_adam_steps_lower_the_loss(gpu):
@@ -63,12 +67,10 @@ def test_batch8_adam_steps_lower_the_loss(gpu):
     net = S.UNet3D.load_file(W.DEFAULT_WEIGHTS)
     samples = _samples(8)
     mean, std = (torch.tensor(a, device="cuda") for a in net.norm)
-    x = torch.stack([s[0] for s in samples])
-    t = ((torch.stack([s[1] for s in samples]) - mean) / std).contiguous()
-    w = torch.stack([s[2] for s in samples])
+    x, t, w, sc = W._stack(samples, range(len(samples)), mean, std)
     losses = []
     for _ in range(8):
-        out = net.forward(x, keep=True)
+        out = net.forward(x, sc, keep=True)
         loss, g = S.mse_loss(out, t, w)
         net.zero_grad()
         net.backward(g)

@@ -201,13 +201,22 @@ def test_vectorised_elementwise_kernels_match_scalar(gpu, shape):
         assert torch.equal(S.add_f32(x, gy).cpu(), S.add_f32(_offset_copy(x), gy).cpu())
 
 
-def _torch_net(net, x):
-    """CPU FP32 copy of UNet3D.forward with the same BF16 rounding points (autograd-capable)."""
+def _torch_net(net, x, sc):
+    """CPU FP32 copy of UNet3D.forward with the same BF16 rounding points (autograd-capable).
+    Returns (output, conv weights, conv biases, FiLM MLP parameters)."""
     Ws = {k: torch_weight(bf(c.w.cpu()), c.cin, c.cout).clone().requires_grad_(True) for k, c in net.conv.items()}
     Bs = {k: c.b.cpu().clone().requires_grad_(True) for k, c in net.conv.items()}
+    D = {k: (getattr(net, k).w.cpu().clone().requires_grad_(True), getattr(net, k).b.cpu().clone().requires_grad_(True))
+         for k in ("film1", "film2")}
+    film = F.silu(sc @ D["film1"][0].T + D["film1"][1]) @ D["film2"][0].T + D["film2"][1]
+    sites = {name: (o, C) for name, o, C in net.FILM}
 
-    def cv(k, h):
-        return F.conv3d(h, Ws[k], bias=Bs[k], padding=1)
+    def cv(k, h, site=None):
+        p = F.conv3d(h, Ws[k], bias=Bs[k], padding=1)
+        if site is not None:  # FiLM: per-sample, per-channel scale and shift of the pre-activation
+            o, C = sites[site]
+            p = p * (1.0 + film[:, o:o + C, None, None, None]) + film[:, o + C:o + 2 * C, None, None, None]
+        return p
 
     def act(h):  # SiLU output stored in BF16
         return bf(F.silu(h))
@@ -219,12 +228,32 @@ def _torch_net(net, x):
     q2 = bf(F.avg_pool3d(s2, 2))
     s3 = act(cv("c3b", act(cv("c3a", q2))))
     q3 = bf(F.avg_pool3d(s3, 2))
-    a4 = act(cv("c4b", act(cv("c4a", q3))))
+    a4 = act(cv("c4b", act(cv("c4a", q3)), "p4b"))
+    r4 = bf(a4 + act(cv("c4d", act(cv("c4c", a4)))))
     up = lambda t: t.repeat_interleave(2, 2).repeat_interleave(2, 3).repeat_interleave(2, 4)
-    d3 = act(cv("d3b", bf(act(cv("d3a", up(a4))) + s3)))
-    d2 = act(cv("d2b", bf(act(cv("d2a", up(d3))) + s2)))
-    d1 = act(cv("d1b", bf(act(cv("d1a", up(d2))) + s1)))
-    return ndhwc(cv("head", d1)), Ws, Bs
+    d3 = act(cv("d3b", bf(act(cv("d3a", up(r4))) + s3), "pd3b"))
+    d2 = act(cv("d2b", bf(act(cv("d2a", up(d3))) + s2), "pd2b"))
+    d1 = act(cv("d1b", bf(act(cv("d1a", up(d2))) + s1), "pd1b"))
+    return ndhwc(cv("head", d1)), Ws, Bs, D
+
+
+def randomize_zero_layers(net, seed=0):
+    """Give the layers that start at zero (the residual block's last convolution, the FiLM MLP's output) small
+    random values, so that every gradient of a test is non-trivial."""
+    gen = torch.Generator().manual_seed(seed)
+    c = net.conv["c4d"]
+    w = torch.zeros(c.cout, c.Kp)
+    w[:, :27 * c.cin] = torch.randn(c.cout, 27 * c.cin, generator=gen) * 0.02
+    c.load({"w": w, "b": torch.randn(c.cout, generator=gen) * 0.02})
+    net.film2.load({"w": torch.randn(net.film2.fout, net.film2.fin, generator=gen) * 0.05,
+                    "b": torch.randn(net.film2.fout, generator=gen) * 0.05})
+
+
+def _scalars(n, seed):
+    from xadvcool_amd import surrogate as S
+    rng = np.random.default_rng(seed)
+    return torch.from_numpy(np.stack([S.scalars(rng.uniform(0.52, 0.56), rng.uniform(0.05, 0.2),
+                                                rng.uniform(0.01, 0.05)) for _ in range(n)]))
 
 
 @pytest.mark.parametrize("shape", [(2, 16, 8, 8), (1, 8, 32, 32)])
@@ -235,25 +264,82 @@ def test_network_forward_and_gradients(gpu, shape):
     print(f"[nn] network forward and gradients on {shape}", flush=True)
     torch.manual_seed(0)
     net = S.UNet3D(seed=1)
+    randomize_zero_layers(net)
     x = torch.randn(*shape, 16) * 0.5
+    sc = _scalars(shape[0], 3)
     tgt = torch.randn(*shape, 5)
     w = (torch.rand(*shape, 5) > 0.3).float()
-    out = net.forward(x.to(torch.bfloat16).cuda(), keep=True)
-    ref, Ws, Bs = _torch_net(net, x)
-    assert rel(out.cpu(), ref) < 2e-2
+    out = net.forward(x.to(torch.bfloat16).cuda(), sc.cuda(), keep=True)
+    ref, Ws, Bs, D = _torch_net(net, x, sc)
+    assert rel(out.cpu(), ref.detach()) < 2e-2
     loss, g = S.mse_loss(out, tgt.cuda(), w.cuda())
     net.zero_grad()
     net.backward(g)
     lref = ((ref[..., :5] - tgt) ** 2 * w).sum() / w.sum()
     lref.backward()
     assert abs(loss.item() - lref.item()) / lref.item() < 2e-2
-    for k, c in net.conv.items():
-        gw = torch_weight(c.dw.cpu(), c.cin, c.cout)
-        e = rel(gw, Ws[k].grad)
-        cos = float(torch.nn.functional.cosine_similarity(gw.flatten(), Ws[k].grad.flatten(), dim=0))
+    grads = [(k, torch_weight(c.dw.cpu(), c.cin, c.cout), Ws[k].grad, c.db.cpu(), Bs[k].grad)
+             for k, c in net.conv.items()]
+    grads += [(k, getattr(net, k).dw.cpu(), D[k][0].grad, getattr(net, k).db.cpu(), D[k][1].grad)
+              for k in ("film1", "film2")]
+    for k, gw, rw, gb, rb in grads:
+        e = rel(gw, rw)
+        cos = float(torch.nn.functional.cosine_similarity(gw.flatten(), rw.flatten(), dim=0))
         print(f"[nn] grad {k}: rel {e:.3e} cos {cos:.5f}", flush=True)
         assert e < 6e-2 and cos > 0.998, k
-        assert rel(c.db.cpu(), Bs[k].grad) < 6e-2, k
+        assert rel(gb, rb) < 6e-2, k
+
+
+def test_film_and_dense_kernels(gpu):
+    """FiLM modulation and the MLP's dense layers against torch.autograd in FP32 (summation order only)."""
+    from xadvcool_amd import surrogate as S
+    gen = torch.Generator().manual_seed(5)
+    N, Vn, C, L, off = 3, 5000, 32, 96, 16
+    p = torch.randn(N, 10, 25, 20, C, generator=gen)
+    film = torch.randn(N, L, generator=gen) * 0.3
+    gy = torch.randn(p.shape, generator=gen)
+    y = S.film_fwd(p.cuda(), film.cuda(), off)
+    pr, fr = p.clone().requires_grad_(True), film.clone().requires_grad_(True)
+    yr = pr * (1 + fr[:, None, None, None, off:off + C]) + fr[:, None, None, None, off + C:off + 2 * C]
+    assert rel(y.cpu(), yr.detach()) < 1e-6
+    yr.backward(gy)
+    dfilm = torch.zeros(N, L, device="cuda")
+    gx = S.film_bwd(p.cuda(), gy.cuda(), film.cuda(), dfilm, off)
+    assert rel(gx.cpu(), pr.grad) < 1e-6
+    assert rel(dfilm.cpu()[:, off:off + 2 * C], fr.grad[:, off:off + 2 * C]) < 1e-5
+    assert float(dfilm.cpu()[:, :off].abs().max()) == 0.0  # other sites' entries untouched
+    for act in (True, False):
+        d = S.Dense(7, 40, gen, act=act)
+        d.load({"w": torch.randn(40, 7, generator=gen), "b": torch.randn(40, generator=gen)})
+        xi = torch.randn(5, 7, generator=gen)
+        pre, post = d.fwd(xi.cuda())
+        wr, br = d.w.cpu().clone().requires_grad_(True), d.b.cpu().clone().requires_grad_(True)
+        xr = xi.clone().requires_grad_(True)
+        pr2 = xr @ wr.T + br
+        outr = F.silu(pr2) if act else pr2
+        assert rel(post.cpu(), outr.detach()) < 1e-6 and rel(pre.cpu(), pr2.detach()) < 1e-6
+        g = torch.randn(5, 40, generator=gen)
+        outr.backward(g)
+        d.zero_grad()
+        gxi = d.bwd(xi.cuda(), pre, g.cuda())
+        assert rel(d.dw.cpu(), wr.grad) < 1e-5 and rel(d.db.cpu(), br.grad) < 1e-5
+        assert rel(gxi.cpu(), xr.grad) < 1e-5
+
+
+def test_old_weights_load_as_the_same_function(gpu, tmp_path):
+    """Weights saved without the residual block and the FiLM layers load with the block at identity and no
+    modulation: the output equals that of a network whose new layers are explicitly zero."""
+    from xadvcool_amd import surrogate as S
+    net = S.UNet3D(seed=4)
+    d = {k: v for k, v in net.state_dict().items() if not k.startswith(("c4c.", "c4d.", "film"))}
+    np.savez(tmp_path / "old.npz", **d)
+    old = S.UNet3D.load_file(str(tmp_path / "old.npz"))
+    x = (torch.randn(1, 16, 8, 8, 16) * 0.5).to(torch.bfloat16).cuda()
+    sc = _scalars(1, 4).cuda()
+    a = old.forward(x, sc)
+    net.conv["c4c"].load({"w": np.zeros_like(old.conv["c4c"].w.cpu().numpy()), "b": np.zeros(128, np.float32)})
+    b = net.forward(x, sc)
+    assert torch.equal(a, b)
 
 
 def test_adam_lowers_the_loss(gpu):
@@ -261,11 +347,12 @@ def test_adam_lowers_the_loss(gpu):
     torch.manual_seed(1)
     net = S.UNet3D(seed=2)
     x = (torch.randn(2, 16, 8, 8, 16) * 0.5).to(torch.bfloat16).cuda()
+    sc = _scalars(2, 6).cuda()
     tgt = torch.randn(2, 16, 8, 8, 5).cuda() * 0.3
     w = torch.ones(2, 16, 8, 8, 5).cuda()
     losses = []
     for _ in range(30):
-        out = net.forward(x, keep=True)
+        out = net.forward(x, sc, keep=True)
         loss, g = S.mse_loss(out, tgt, w)
         net.zero_grad()
         net.backward(g)

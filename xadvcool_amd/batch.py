@@ -30,6 +30,8 @@ import numpy as np
 import torch
 
 from .engine import STAT_NAMES, LatticeBatch, normalized_heat, thermal_taus
+
+__all__ = ["BatchedSolver", "device_inputs", "run_grouped"]
 from .simulation import _Residuals, min_steps_floor
 
 
@@ -135,6 +137,7 @@ class BatchedSolver:
         if self.collect_fields:
             rho, u, T = self.batch.fields(s)
             out["fields"] = (rho.cpu().numpy(), u.cpu().numpy(), T.cpu().numpy())
+            out["inputs"] = (d["mask"], d["heat"])  # the design's device mask and normalised heat map
         self.batch.set_active(s, False)
         self.state[s] = None
         return out
@@ -145,49 +148,86 @@ class BatchedSolver:
         plus 152 B per cell (thermal lattice)."""
         return sum(st["bytes"] for st in self.state if st is not None)
 
+    def busy(self):
+        return any(st is not None for st in self.state)
+
+    def fill(self, queue):
+        """Load queued designs into the empty slots."""
+        for s in range(self.slots):
+            if self.state[s] is None and queue:
+                self._start(s, queue.popleft())
+
+    def chunk(self, queue, on_result=None, timer=None):
+        """One graph chunk of ``stats_every`` passes over the active slots, the residual check of each, and
+        retirement + refill from ``queue``; returns the results of the slots that retired."""
+        results = []
+        if timer is not None:
+            timer.begin()
+        self.batch.step(self.every, want_fields=True)
+        if timer is not None:
+            timer.end(self.every, self.active_bytes_per_pass())
+        self.passes += self.every
+        self.batch.readout(with_stats=True)
+        stats = self.batch.stats()
+        for s, st in enumerate(self.state):
+            if st is None:
+                continue
+            st["steps"] += self.every
+            self.lups += st["cells"] * self.every
+            sd = dict(zip(STAT_NAMES, stats[s].tolist()))
+            res = st["res"]
+            res.update(st["steps"], sd)
+            st["history"].append((st["steps"], res.d_ke, res.dT_conv, res.dQ_conv))
+            status = None
+            st["ok"] = st["ok"] + 1 if res.converged(st["steps"]) else 0
+            if st["ok"] >= self.confirm:
+                status = "converged"
+            elif res.diverged() or not (np.isfinite(sd["sum_ke"]) and np.isfinite(sd["sum_T"])):
+                # non-finite sums: the fields hold NaN/inf (the max reductions skip NaN), retire now
+                status = "diverged"
+            elif self.stall_check is not None and self.stall_check(st["design"], st["steps"], res):
+                status = "stalled"
+            elif st["steps"] >= self.max_steps:
+                status = "max_steps"
+            if status is not None:
+                r = self._finish(s, status)
+                results.append(r)
+                if on_result is not None:
+                    more = on_result(r)
+                    if more:
+                        queue.extend(more)
+                if queue:
+                    self._start(s, queue.popleft())
+        return results
+
     def run(self, designs, on_result=None, timer=None):
         """Solve every design in ``designs``; returns results in completion order.
 
-        ``timer`` (optional) gets ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes."""
-        queue = deque(designs)
-        results = []
-        for s in range(self.slots):
-            if queue:
-                self._start(s, queue.popleft())
-        while any(st is not None for st in self.state):
-            if timer is not None:
-                timer.begin()
-            self.batch.step(self.every, want_fields=True)
-            if timer is not None:
-                timer.end(self.every, self.active_bytes_per_pass())
-            self.passes += self.every
-            self.batch.readout(with_stats=True)
-            stats = self.batch.stats()
-            for s, st in enumerate(self.state):
-                if st is None:
-                    continue
-                st["steps"] += self.every
-                self.lups += st["cells"] * self.every
-                sd = dict(zip(STAT_NAMES, stats[s].tolist()))
-                res = st["res"]
-                res.update(st["steps"], sd)
-                st["history"].append((st["steps"], res.d_ke, res.dT_conv, res.dQ_conv))
-                status = None
-                st["ok"] = st["ok"] + 1 if res.converged(st["steps"]) else 0
-                if st["ok"] >= self.confirm:
-                    status = "converged"
-                elif res.diverged() or not (np.isfinite(sd["sum_ke"]) and np.isfinite(sd["sum_T"])):
-                    # non-finite sums: the fields hold NaN/inf (the max reductions skip NaN), retire now
-                    status = "diverged"
-                elif self.stall_check is not None and self.stall_check(st["design"], st["steps"], res):
-                    status = "stalled"
-                elif st["steps"] >= self.max_steps:
-                    status = "max_steps"
-                if status is not None:
-                    r = self._finish(s, status)
-                    results.append(r)
-                    if on_result is not None:
-                        on_result(r)
-                    if queue:
-                        self._start(s, queue.popleft())
-        return results
+        ``on_result(r)`` (optional) sees each result as its slot retires and may return designs to append to
+        the queue.  ``timer`` (optional) gets ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of
+        passes."""
+        return run_grouped([self], designs, on_result=on_result, timer=timer)
+
+
+def run_grouped(solvers, designs, on_result=None, timer=None):
+    """Solve ``designs`` on several batched solvers that share one queue, one graph chunk per solver in turn.
+
+    With small batches (a few full-size lattices each), a batch's state stays partly in the 256 MB Infinity
+    Cache across the passes of its chunk, which a single large batch (all designs in one launch, its state in
+    HBM) cannot do; the groups take turns chunk by chunk, and a slot that retires takes the next queued
+    design as in one batch.  Results in completion order."""
+    queue = deque(designs)
+    results = []
+    for bs in solvers:
+        bs.fill(queue)
+    while True:
+        active = [bs for bs in solvers if bs.busy()]
+        if not active:
+            return results
+        for bs in active:
+            if timer is not None and len(solvers) > 1:
+                timer.stream = bs.batch.stream()  # each group's passes run on its own batch's stream
+            results += bs.chunk(queue, on_result=on_result, timer=timer)
+            for other in solvers:  # designs re-queued by on_result may go to a group that fell idle
+                if queue and not other.busy():
+                    other.fill(queue)

@@ -89,10 +89,18 @@ def device_checks(batch, slot, mask, heat, *, tau_fluid, heat_power, Nu=0.0, f_f
                         k_ratio=k_ratio)
 
 
-# The acceptance gate of a warm-started result: the field checks of validate_sample at their warning
-# levels (analyze_dataset.py:111-120) and the pressure-drop sign.  The Nusselt and f*Re ranges are left
-# out: they judge the design, not whether the solve reached its steady state.
-GATE = ("mass_warn", "mach_warn", "rho_dev_warn", "T_lb_warn", "delta_P_pass", "div_warn", "energy_warn")
+# The acceptance gate of a warm-started result: the checks of validate_sample (analyze_dataset.py:111-120)
+# that every converged cold solve of this scheme passes, at their warning levels: mass balance between the
+# planes one cell inside the inlet and outlet, Mach number, pressure-drop sign, mean divergence.  Measured
+# on 20 converged cold solves of all five families (148x128x32, rise residuals at 1e-6): mass error at most
+# 7e-4, Mach at most 0.25, every pressure drop positive, mean |div u| at most 1.1e-3.  The other checks are
+# computed and reported but not gated, because converged cold solves of the same scheme fail them: the
+# flux-based energy balance (16 of 20; the heat that leaves through the inlet face's solid cells by
+# conduction is not in it), the Mach-scaled density deviation (4 of 20) and the fluid temperature lower
+# bound (1 of 20, -0.91 below the inlet: the thermal lattice's undershoot at tau close to 1/2); their
+# values agree between the cold and warm results of the same design to within 0.007.  The Nusselt and
+# f*Re ranges judge the design, not the solve.
+GATE = ("mass_warn", "mach_warn", "delta_P_pass", "div_warn")
 
 
 def gate_failures(c):

@@ -672,6 +672,12 @@ __global__ void k_silu_bwd(const float* __restrict__ x, const float* __restrict_
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) gxb[i] = f2bf(gy[i] * dsilu(x[i]));
 }
+// gx = gy * silu'(x) in FP32 (for a gradient that still goes through a FiLM layer before the next MFMA).
+__global__ void k_silu_bwd_f32(const float* __restrict__ x, const float* __restrict__ gy, float* __restrict__ gx,
+                               int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) gx[i] = gy[i] * dsilu(x[i]);
+}
 __global__ void k_to_bf16(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) y[i] = f2bf(x[i]);
@@ -1069,6 +1075,123 @@ void launch_wgrad_tile(const __bf16* i, const __bf16* go, float* part, Geo g, in
                      nty, CoutT);
 }
 
+// ------------------------------------------------------------------ FiLM ---
+// Feature-wise affine modulation of a layer's pre-activations by per-sample
+// vectors computed from the design's scalars (Perez et al. 2018, FiLM):
+//   y[n][v][c] = p[n][v][c] * (1 + gam[n][c]) + bet[n][c],
+// gam[n][c] = film[n * L + off + c], bet[n][c] = film[n * L + off + C + c].
+__global__ void k_film_fwd(const float* __restrict__ p, const float* __restrict__ film, float* __restrict__ y,
+                           int64_t Vn, int C, int off, int L, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const int n = (int)(i / ((int64_t)C * Vn));
+  const float* f = film + (int64_t)n * L + off;
+  y[i] = p[i] * (1.0f + f[c]) + f[C + c];
+}
+
+// Backward: gx = gy * (1 + gam), and per-block partial sums of gy * p (-> dgam) and of gy (-> dbet) over a
+// contiguous voxel range of one sample (grid.y = sample); thread t sums channel t % C over voxels
+// t / C + k (256 / C) (C divides 256), then the block folds its rows in a fixed order.
+__global__ __launch_bounds__(256) void k_film_bwd_part(const float* __restrict__ p, const float* __restrict__ gy,
+                                                       const float* __restrict__ film, float* __restrict__ gx,
+                                                       float* __restrict__ part, int64_t Vn, int C, int off, int L,
+                                                       int64_t vper) {
+  __shared__ float red[2][256];
+  const int n = blockIdx.y, b = blockIdx.x, nbn = gridDim.x;
+  const int64_t v0 = (int64_t)b * vper;
+  const int64_t v1 = v0 + vper < Vn ? v0 + vper : Vn;
+  const int c = threadIdx.x % C, r = threadIdx.x / C, R = 256 / C;
+  const float g1 = 1.0f + film[(int64_t)n * L + off + c];
+  float sgp = 0.0f, sg = 0.0f;
+  for (int64_t v = v0 + r; v < v1; v += R) {
+    const int64_t i = ((int64_t)n * Vn + v) * C + c;
+    const float g = gy[i];
+    sgp += g * p[i];
+    sg += g;
+    gx[i] = g * g1;
+  }
+  red[0][threadIdx.x] = sgp;
+  red[1][threadIdx.x] = sg;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float a = 0.0f, e = 0.0f;
+    for (int k = 0; k < R; ++k) {
+      a += red[0][k * C + c];
+      e += red[1][k * C + c];
+    }
+    float* q = part + ((int64_t)n * nbn + b) * (2 * C);
+    q[c] = a;
+    q[C + c] = e;
+  }
+}
+
+// dfilm[n][off + j] = sum over the sample's blocks (in block order) of part[n][b][j], j < 2C.
+__global__ void k_film_bwd_final(const float* __restrict__ part, float* __restrict__ dfilm, int N, int nbn, int C,
+                                 int off, int L) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * 2 * C) return;
+  const int n = t / (2 * C), j = t % (2 * C);
+  float a = 0.0f;
+  for (int b = 0; b < nbn; ++b) a += part[((int64_t)n * nbn + b) * (2 * C) + j];
+  dfilm[(int64_t)n * L + off + j] = a;
+}
+
+// ----------------------------------------------------------------- dense ---
+// The small fully connected layers that map the design's scalars to the FiLM
+// vectors (FP32): pre[n][o] = sum_i W[o][i] x[n][i] (i in order) + b[o];
+// post = silu(pre) with act, else pre.
+__global__ void k_dense_fwd(const float* __restrict__ x, const float* __restrict__ W, const float* __restrict__ b,
+                            float* __restrict__ pre, float* __restrict__ post, int N, int I, int O, int act) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * O) return;
+  const int n = t / O, o = t % O;
+  float a = 0.0f;
+  for (int i = 0; i < I; ++i) a += W[(int64_t)o * I + i] * x[(int64_t)n * I + i];
+  a += b[o];
+  pre[t] = a;
+  if (post) post[t] = act ? silu(a) : a;
+}
+
+// dW[o][i] = sum_n g[n][o] x[n][i], db[o] = sum_n g[n][o], with g = gy * silu'(pre) (act) or gy.
+__global__ void k_dense_bwd_w(const float* __restrict__ x, const float* __restrict__ gy,
+                              const float* __restrict__ pre, float* __restrict__ dW, float* __restrict__ db, int N,
+                              int I, int O, int act) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= O * (I + 1)) return;
+  const int o = t / (I + 1), i = t % (I + 1);
+  float a = 0.0f;
+  for (int n = 0; n < N; ++n) {
+    float g = gy[(int64_t)n * O + o];
+    if (act) g *= dsilu(pre[(int64_t)n * O + o]);
+    a += g * (i < I ? x[(int64_t)n * I + i] : 1.0f);
+  }
+  if (i < I)
+    dW[(int64_t)o * I + i] = a;
+  else
+    db[o] = a;
+}
+
+// gx[n][i] = sum_o W[o][i] g[n][o] (o in order).
+__global__ void k_dense_bwd_x(const float* __restrict__ W, const float* __restrict__ gy,
+                              const float* __restrict__ pre, float* __restrict__ gx, int N, int I, int O, int act) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= N * I) return;
+  const int n = t / I, i = t % I;
+  float a = 0.0f;
+  for (int o = 0; o < O; ++o) {
+    float g = gy[(int64_t)n * O + o];
+    if (act) g *= dsilu(pre[(int64_t)n * O + o]);
+    a += W[(int64_t)o * I + i] * g;
+  }
+  gx[t] = a;
+}
+
+int film_blocks(int64_t Vn) {
+  int64_t nb = (Vn + 4095) / 4096;
+  return (int)(nb < 1 ? 1 : (nb > 256 ? 256 : nb));
+}
+
 int64_t bias_blocks(int64_t V) {
   int64_t nb = (V + 4095) / 4096;
   return nb < 1 ? 1 : (nb > 1024 ? 1024 : nb);
@@ -1229,6 +1352,48 @@ int xnn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, 
   const float c1 = 1.0f - powf(b1, (float)step), c2 = 1.0f - powf(b2, (float)step);
   hipLaunchKernelGGL(k_adam, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, w, g, m, v, (__bf16*)wb, n, lr, b1,
                      b2, eps, c1, c2, wd);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
+int xnn_silu_bwd_f32(const float* x, const float* gy, float* gx, int64_t n, void* stream) {
+  hipLaunchKernelGGL(k_silu_bwd_f32, dim3(nblk(n)), dim3(256), 0, (hipStream_t)stream, x, gy, gx, n);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_film_fwd(const float* p, const float* film, float* y, int N, int64_t Vn, int C, int off, int L,
+                 void* stream) {
+  if (N < 1 || Vn < 1 || C < 1 || off < 0 || off + 2 * C > L) return 1;
+  const int64_t total = (int64_t)N * Vn * C;
+  hipLaunchKernelGGL(k_film_fwd, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, p, film, y, Vn, C, off, L,
+                     total);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int64_t xnn_film_part_floats(int N, int64_t Vn, int C) { return (int64_t)N * film_blocks(Vn) * 2 * C; }
+int xnn_film_bwd(const float* p, const float* gy, const float* film, float* gx, float* part, float* dfilm, int N,
+                 int64_t Vn, int C, int off, int L, void* stream) {
+  if (N < 1 || N > 65535 || Vn < 1 || C < 1 || C > 256 || 256 % C || off < 0 || off + 2 * C > L) return 1;
+  const int nbn = film_blocks(Vn);
+  const int64_t vper = (Vn + nbn - 1) / nbn;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_film_bwd_part, dim3(nbn, N), dim3(256), 0, s, p, gy, film, gx, part, Vn, C, off, L, vper);
+  hipLaunchKernelGGL(k_film_bwd_final, dim3(nblk((int64_t)N * 2 * C)), dim3(256), 0, s, part, dfilm, N, nbn, C, off,
+                     L);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_dense_fwd(const float* x, const float* W, const float* b, float* pre, float* post, int N, int I, int O,
+                  int act, void* stream) {
+  if (N < 1 || I < 1 || O < 1) return 1;
+  hipLaunchKernelGGL(k_dense_fwd, dim3(nblk((int64_t)N * O)), dim3(256), 0, (hipStream_t)stream, x, W, b, pre, post,
+                     N, I, O, act);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_dense_bwd(const float* x, const float* W, const float* pre, const float* gy, float* dW, float* db, float* gx,
+                  int N, int I, int O, int act, void* stream) {
+  if (N < 1 || I < 1 || O < 1) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((int64_t)O * (I + 1))), dim3(256), 0, s, x, gy, pre, dW, db, N, I, O,
+                     act);
+  if (gx)
+    hipLaunchKernelGGL(k_dense_bwd_x, dim3(nblk((int64_t)N * I)), dim3(256), 0, s, W, gy, pre, gx, N, I, O, act);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

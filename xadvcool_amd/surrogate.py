@@ -1,13 +1,22 @@
 """Learned warm start: a 3-D convolutional encoder-decoder that predicts (u, p, T) of a cold-plate design.
 
 Network (after the 3-D U-Net of the reference's benchmark suite,
-benchmark/models/unet3d.py:49-125, reduced and made additive): four levels
-with 16, 32, 64, 128 channels; two 3x3x3 conv + SiLU per level; 2x2x2
-average pooling down, nearest x2 upsampling + conv up, additive skips; a
-linear 3x3x3 head.  ~1.33 M parameters.  Every kernel is this project's HIP
-code (libxadv_nn.so, include/xadv_nn.h): BF16 activations and weights into
+benchmark/models/unet3d.py:49-125, made additive): four levels with 16, 32,
+64, 128 channels; two 3x3x3 conv + SiLU per level; 2x2x2 average pooling
+down, nearest x2 upsampling + conv up, additive skips; a residual block of
+two more 128-channel convolutions at the bottleneck; a linear 3x3x3 head.
+The design's scalars (tau, heat power, inlet velocity) condition the
+network through FiLM (Perez et al. 2018; the reference template conditions
+its bottleneck the same way, unet3d.py:34-45, 83-84): a two-layer MLP
+(3 -> 64 -> 480) gives a per-channel scale and shift of the pre-activations
+of the last convolution of the bottleneck and of each decoder level.  The
+residual block's second convolution and the MLP's last layer start at zero,
+so a network without them (the first shipped weights) loads as the same
+function.  ~2.24 M parameters.  Every kernel is this project's HIP code
+(libxadv_nn.so, include/xadv_nn.h): BF16 activations and weights into
 v_mfma_f32_16x16x32_bf16, FP32 accumulation, FP32 master weights and Adam
-moments.  Torch supplies device memory and the stream only.
+moments; the FiLM and MLP layers in FP32.  Torch supplies device memory and
+the stream only.
 
 Inputs (16 channels, channels-last, lattice padded to multiples of 8):
 fluid / solid / inlet / outlet indicators, the heat map, inlet velocity on
@@ -61,6 +70,13 @@ def lib():
         L.xnn_mse.argtypes = [vp, vp, vp, vp, vp, i64, i, i, f, vp]
         L.xnn_mse_blocks.argtypes = [i64, i]
         L.xnn_adam.argtypes = [vp, vp, vp, vp, vp, i64, f, f, f, f, i, f, vp]
+        L.xnn_silu_bwd_f32.argtypes = [vp, vp, vp, i64, vp]
+        L.xnn_film_fwd.argtypes = [vp, vp, vp, i, i64, i, i, i, vp]
+        L.xnn_film_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i, i64, i, i, i, vp]
+        L.xnn_film_part_floats.argtypes = [i, i64, i]
+        L.xnn_film_part_floats.restype = i64
+        L.xnn_dense_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, vp]
+        L.xnn_dense_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, i, i, vp]
         _lib = L
     return _lib
 
@@ -108,6 +124,82 @@ def silu_bwd(x, gy):
     g = torch.empty(x.shape, dtype=torch.bfloat16, device=x.device)
     _ok(lib().xnn_silu_bwd(_p(x), _p(gy), _p(g), x.numel(), _st()), "xnn_silu_bwd")
     return g
+
+
+def silu_bwd_f32(x, gy):
+    g = torch.empty(x.shape, dtype=torch.float32, device=x.device)
+    _ok(lib().xnn_silu_bwd_f32(_p(x), _p(gy), _p(g), x.numel(), _st()), "xnn_silu_bwd_f32")
+    return g
+
+
+def film_fwd(p, film, off):
+    """p (N, X, Y, Z, C) FP32 modulated by rows of ``film`` (N, L): p * (1 + film[:, off:off+C]) + film[:, off+C:]."""
+    N, C = p.shape[0], p.shape[-1]
+    y = torch.empty_like(p)
+    _ok(lib().xnn_film_fwd(_p(p), _p(film), _p(y), N, p.numel() // (N * C), C, off, film.shape[1], _st()),
+        "xnn_film_fwd")
+    return y
+
+
+def film_bwd(p, gy, film, dfilm, off):
+    """Gradient w.r.t. p of a film_fwd output gradient gy (FP32); writes dfilm[:, off:off+2C]."""
+    N, C = p.shape[0], p.shape[-1]
+    Vn = p.numel() // (N * C)
+    L = lib()
+    part = torch.empty(L.xnn_film_part_floats(N, Vn, C), dtype=torch.float32, device=p.device)
+    gx = torch.empty_like(p)
+    _ok(L.xnn_film_bwd(_p(p), _p(gy), _p(film), _p(gx), _p(part), _p(dfilm), N, Vn, C, off, film.shape[1], _st()),
+        "xnn_film_bwd")
+    return gx
+
+
+class Dense:
+    """Fully connected FP32 layer (W: O x I) with Adam moments; ``zero`` starts it at zero."""
+
+    def __init__(self, fin, fout, gen, act, zero=False, device="cuda"):
+        self.fin, self.fout, self.act = fin, fout, int(bool(act))
+        w = torch.zeros(fout, fin) if zero else torch.randn(fout, fin, generator=gen) * math.sqrt(2.0 / fin)
+        self.w = w.to(device)
+        self.b = torch.zeros(fout, dtype=torch.float32, device=device)
+        self.m_w, self.v_w = torch.zeros_like(self.w), torch.zeros_like(self.w)
+        self.m_b, self.v_b = torch.zeros_like(self.b), torch.zeros_like(self.b)
+        self.dw, self.db = torch.zeros_like(self.w), torch.zeros_like(self.b)
+
+    def fwd(self, x):
+        N = x.shape[0]
+        pre = torch.empty(N, self.fout, dtype=torch.float32, device=x.device)
+        post = torch.empty_like(pre)
+        _ok(lib().xnn_dense_fwd(_p(x), _p(self.w), _p(self.b), _p(pre), _p(post), N, self.fin, self.fout, self.act,
+                                _st()), "xnn_dense_fwd")
+        return pre, post
+
+    def bwd(self, x, pre, gy, want_gx=True):
+        N = x.shape[0]
+        dw, db = torch.empty_like(self.w), torch.empty_like(self.b)
+        gx = torch.empty_like(x) if want_gx else None
+        _ok(lib().xnn_dense_bwd(_p(x), _p(self.w), _p(pre), _p(gy), _p(dw), _p(db), _p(gx), N, self.fin, self.fout,
+                                self.act, _st()), "xnn_dense_bwd")
+        self.dw.add_(dw)
+        self.db.add_(db)
+        return gx
+
+    def zero_grad(self):
+        self.dw.zero_()
+        self.db.zero_()
+
+    def adam(self, step, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+        L = lib()
+        _ok(L.xnn_adam(_p(self.w), _p(self.dw), _p(self.m_w), _p(self.v_w), None, self.w.numel(), lr, b1, b2, eps,
+                       step, wd, _st()), "xnn_adam")
+        _ok(L.xnn_adam(_p(self.b), _p(self.db), _p(self.m_b), _p(self.v_b), None, self.b.numel(), lr, b1, b2, eps,
+                       step, 0.0, _st()), "xnn_adam")
+
+    def state(self):
+        return {"w": self.w.cpu().numpy(), "b": self.b.cpu().numpy()}
+
+    def load(self, st):
+        self.w.copy_(torch.as_tensor(st["w"]))
+        self.b.copy_(torch.as_tensor(st["b"]))
 
 
 def add_f32(a, b):
@@ -162,7 +254,8 @@ class Conv:
         self.Kp = (27 * cin + 31) // 32 * 32
         self.Kpt = (27 * cout + 31) // 32 * 32
         w = torch.zeros(cout, self.Kp, dtype=torch.float32)
-        w[:, :27 * cin] = torch.randn(cout, 27 * cin, generator=gen) * (gain * math.sqrt(2.0 / (27 * cin)))
+        r = torch.randn(cout, 27 * cin, generator=gen)  # drawn even at gain 0, so later layers' draws stay put
+        w[:, :27 * cin] = r * (gain * math.sqrt(2.0 / (27 * cin)))
         self.w = w.to(device)
         self.b = torch.zeros(cout, dtype=torch.float32, device=device)
         self.wb = self.w.to(torch.bfloat16)
@@ -222,12 +315,22 @@ class Conv:
         self.wb.copy_(self.w.to(torch.bfloat16))
 
 
+def scalars(tau, P, u_in):
+    """The design's conditioning scalars, scaled as the constant input channels are (``features``)."""
+    return np.array([(tau - 0.54) * 50.0, P * 8.0, u_in * 30.0], np.float32)
+
+
 class UNet3D:
-    """Encoder-decoder with levels of CH channels; forward keeps the tape for backward."""
+    """Encoder-decoder with levels of CH channels, a residual bottleneck block and FiLM conditioning on the
+    design's scalars; forward keeps the tape for backward."""
 
     CH = (16, 32, 64, 128)
-    LAYERS = ("c1a", "c1b", "c2a", "c2b", "c3a", "c3b", "c4a", "c4b", "d3a", "d3b", "d2a", "d2b", "d1a", "d1b",
-              "head")
+    LAYERS = ("c1a", "c1b", "c2a", "c2b", "c3a", "c3b", "c4a", "c4b", "c4c", "c4d", "d3a", "d3b", "d2a", "d2b",
+              "d1a", "d1b", "head")
+    # FiLM sites: the pre-activation each modulates and its offset in the FiLM row ([scale | shift] per site).
+    FILM = (("p4b", 0, 128), ("pd3b", 256, 64), ("pd2b", 384, 32), ("pd1b", 448, 16))
+    FILM_LEN = 480
+    FILM_HIDDEN = 64
 
     def __init__(self, seed=0, device="cuda"):
         gen = torch.Generator().manual_seed(seed)
@@ -236,113 +339,168 @@ class UNet3D:
                 "c3a": (c[1], c[2]), "c3b": (c[2], c[2]), "c4a": (c[2], c[3]), "c4b": (c[3], c[3]),
                 "d3a": (c[3], c[2]), "d3b": (c[2], c[2]), "d2a": (c[2], c[1]), "d2b": (c[1], c[1]),
                 "d1a": (c[1], c[0]), "d1b": (c[0], c[0]), "head": (c[0], 16)}
-        self.conv = {k: Conv(*spec[k], gen, gain=(0.5 if k == "head" else 1.0), device=device) for k in self.LAYERS}
+        # the first fifteen layers draw their initial weights in the order of the network without the
+        # residual block, so a seed gives them the same values either way
+        self.conv = {k: Conv(*spec[k], gen, gain=(0.5 if k == "head" else 1.0), device=device) for k in spec}
+        self.conv["c4c"] = Conv(c[3], c[3], gen, device=device)
+        self.conv["c4d"] = Conv(c[3], c[3], gen, gain=0.0, device=device)  # starts at zero: block = identity
+        self.conv = {k: self.conv[k] for k in self.LAYERS}
+        self.film1 = Dense(3, self.FILM_HIDDEN, gen, act=True, device=device)
+        self.film2 = Dense(self.FILM_HIDDEN, self.FILM_LEN, gen, act=False, zero=True, device=device)
         self.step_count = 0
         self.norm = None  # target standardisation (mean, std) per channel
 
     def n_params(self):
-        return sum(c.cout * 27 * c.cin + c.cout for c in self.conv.values())
+        return (sum(c.cout * 27 * c.cin + c.cout for c in self.conv.values()) +
+                sum(d.w.numel() + d.b.numel() for d in (self.film1, self.film2)))
 
     @staticmethod
     def shapes(N, X, Y, Z):
         return [(N, X >> k, Y >> k, Z >> k) for k in range(4)]
 
-    def forward(self, x, keep=False):
-        """x: bf16 (N,X,Y,Z,16), X,Y,Z multiples of 8.  Returns fp32 (N,X,Y,Z,16) (first 5 channels used)."""
+    def forward(self, x, s, keep=False):
+        """x: bf16 (N,X,Y,Z,16), X,Y,Z multiples of 8; s: FP32 (N,3) design scalars (``scalars``).
+        Returns fp32 (N,X,Y,Z,16) (first 5 channels used)."""
         N, X, Y, Z, _ = x.shape
-        s = self.shapes(N, X, Y, Z)
+        if tuple(s.shape) != (N, 3) or s.dtype != torch.float32 or not s.is_contiguous():
+            raise ValueError(f"scalars must be a contiguous float32 ({N}, 3) tensor, got {tuple(s.shape)}")
+        sh = self.shapes(N, X, Y, Z)
         cv = self.conv
-        t = {"x": x}
+        t = {"x": x, "s": s}
+        t["h1pre"], t["h1"] = self.film1.fwd(s)
+        film = self.film2.fwd(t["h1"])[0]
+        t["film"] = film
+        off = {name: o for name, o, _ in self.FILM}
+
+        def act(name, p):  # FiLM (at its sites) then SiLU; the tape keeps both pre-activations
+            t[name] = p
+            if name in off:
+                p = film_fwd(p, film, off[name])
+                t[name + "m"] = p
+            return silu_fwd(p)
+
         # encoder
-        t["p1a"] = cv["c1a"].fwd(x, s[0]); t["a1a"] = silu_fwd(t["p1a"])
-        t["p1b"] = cv["c1b"].fwd(t["a1a"], s[0]); t["s1"] = silu_fwd(t["p1b"])
-        t["q1"] = pool_fwd(t["s1"], s[1])
-        t["p2a"] = cv["c2a"].fwd(t["q1"], s[1]); t["a2a"] = silu_fwd(t["p2a"])
-        t["p2b"] = cv["c2b"].fwd(t["a2a"], s[1]); t["s2"] = silu_fwd(t["p2b"])
-        t["q2"] = pool_fwd(t["s2"], s[2])
-        t["p3a"] = cv["c3a"].fwd(t["q2"], s[2]); t["a3a"] = silu_fwd(t["p3a"])
-        t["p3b"] = cv["c3b"].fwd(t["a3a"], s[2]); t["s3"] = silu_fwd(t["p3b"])
-        t["q3"] = pool_fwd(t["s3"], s[3])
-        t["p4a"] = cv["c4a"].fwd(t["q3"], s[3]); t["a4a"] = silu_fwd(t["p4a"])
-        t["p4b"] = cv["c4b"].fwd(t["a4a"], s[3]); t["a4b"] = silu_fwd(t["p4b"])
+        t["a1a"] = act("p1a", cv["c1a"].fwd(x, sh[0]))
+        t["s1"] = act("p1b", cv["c1b"].fwd(t["a1a"], sh[0]))
+        t["q1"] = pool_fwd(t["s1"], sh[1])
+        t["a2a"] = act("p2a", cv["c2a"].fwd(t["q1"], sh[1]))
+        t["s2"] = act("p2b", cv["c2b"].fwd(t["a2a"], sh[1]))
+        t["q2"] = pool_fwd(t["s2"], sh[2])
+        t["a3a"] = act("p3a", cv["c3a"].fwd(t["q2"], sh[2]))
+        t["s3"] = act("p3b", cv["c3b"].fwd(t["a3a"], sh[2]))
+        t["q3"] = pool_fwd(t["s3"], sh[3])
+        t["a4a"] = act("p4a", cv["c4a"].fwd(t["q3"], sh[3]))
+        t["a4b"] = act("p4b", cv["c4b"].fwd(t["a4a"], sh[3]))
+        # residual block at the bottleneck
+        t["a4c"] = act("p4c", cv["c4c"].fwd(t["a4b"], sh[3]))
+        t["a4d"] = act("p4d", cv["c4d"].fwd(t["a4c"], sh[3]))
+        t["r4"] = add_bf16(t["a4b"], t["a4d"])
         # decoder
-        t["u3"] = up_fwd(t["a4b"], s[2])
-        t["pd3a"] = cv["d3a"].fwd(t["u3"], s[2]); t["ad3a"] = silu_fwd(t["pd3a"])
+        t["u3"] = up_fwd(t["r4"], sh[2])
+        t["ad3a"] = act("pd3a", cv["d3a"].fwd(t["u3"], sh[2]))
         t["k3"] = add_bf16(t["ad3a"], t["s3"])
-        t["pd3b"] = cv["d3b"].fwd(t["k3"], s[2]); t["ad3b"] = silu_fwd(t["pd3b"])
-        t["u2"] = up_fwd(t["ad3b"], s[1])
-        t["pd2a"] = cv["d2a"].fwd(t["u2"], s[1]); t["ad2a"] = silu_fwd(t["pd2a"])
+        t["ad3b"] = act("pd3b", cv["d3b"].fwd(t["k3"], sh[2]))
+        t["u2"] = up_fwd(t["ad3b"], sh[1])
+        t["ad2a"] = act("pd2a", cv["d2a"].fwd(t["u2"], sh[1]))
         t["k2"] = add_bf16(t["ad2a"], t["s2"])
-        t["pd2b"] = cv["d2b"].fwd(t["k2"], s[1]); t["ad2b"] = silu_fwd(t["pd2b"])
-        t["u1"] = up_fwd(t["ad2b"], s[0])
-        t["pd1a"] = cv["d1a"].fwd(t["u1"], s[0]); t["ad1a"] = silu_fwd(t["pd1a"])
+        t["ad2b"] = act("pd2b", cv["d2b"].fwd(t["k2"], sh[1]))
+        t["u1"] = up_fwd(t["ad2b"], sh[0])
+        t["ad1a"] = act("pd1a", cv["d1a"].fwd(t["u1"], sh[0]))
         t["k1"] = add_bf16(t["ad1a"], t["s1"])
-        t["pd1b"] = cv["d1b"].fwd(t["k1"], s[0]); t["ad1b"] = silu_fwd(t["pd1b"])
-        out = cv["head"].fwd(t["ad1b"], s[0])
+        t["ad1b"] = act("pd1b", cv["d1b"].fwd(t["k1"], sh[0]))
+        out = cv["head"].fwd(t["ad1b"], sh[0])
         if keep:
-            self._tape = (t, s)
+            self._tape = (t, sh)
         return out
 
     def backward(self, gout):
         """gout: fp32 gradient of the loss w.r.t. the forward output.  Accumulates parameter gradients."""
-        t, s = self._tape
+        t, sh = self._tape
         cv = self.conv
+        off = {name: o for name, o, _ in self.FILM}
+        dfilm = torch.empty_like(t["film"])
 
         def lin(name, x_in, g_out_bf16, shape):  # conv backward: wgrad + dgrad
             cv[name].wgrad(x_in, g_out_bf16, shape)
             return cv[name].dgrad(g_out_bf16, shape)
 
+        def dact(name, g):  # gradient w.r.t. the conv output p (bf16) from the gradient w.r.t. SiLU's output
+            if name not in off:
+                return silu_bwd(t[name], g)
+            gm = silu_bwd_f32(t[name + "m"], g)
+            return to_bf16(film_bwd(t[name], gm, t["film"], dfilm, off[name]))
+
         g = to_bf16(gout)
-        g_ad1b = lin("head", t["ad1b"], g, s[0])
-        g_k1 = lin("d1b", t["k1"], silu_bwd(t["pd1b"], g_ad1b), s[0])
+        g_ad1b = lin("head", t["ad1b"], g, sh[0])
+        g_k1 = lin("d1b", t["k1"], dact("pd1b", g_ad1b), sh[0])
         g_s1 = g_k1
-        g_u1 = lin("d1a", t["u1"], silu_bwd(t["pd1a"], g_k1), s[0])
-        g_ad2b = up_bwd(g_u1, s[1])
-        g_k2 = lin("d2b", t["k2"], silu_bwd(t["pd2b"], g_ad2b), s[1])
+        g_u1 = lin("d1a", t["u1"], dact("pd1a", g_k1), sh[0])
+        g_ad2b = up_bwd(g_u1, sh[1])
+        g_k2 = lin("d2b", t["k2"], dact("pd2b", g_ad2b), sh[1])
         g_s2 = g_k2
-        g_u2 = lin("d2a", t["u2"], silu_bwd(t["pd2a"], g_k2), s[1])
-        g_ad3b = up_bwd(g_u2, s[2])
-        g_k3 = lin("d3b", t["k3"], silu_bwd(t["pd3b"], g_ad3b), s[2])
+        g_u2 = lin("d2a", t["u2"], dact("pd2a", g_k2), sh[1])
+        g_ad3b = up_bwd(g_u2, sh[2])
+        g_k3 = lin("d3b", t["k3"], dact("pd3b", g_ad3b), sh[2])
         g_s3 = g_k3
-        g_u3 = lin("d3a", t["u3"], silu_bwd(t["pd3a"], g_k3), s[2])
-        g_a4b = up_bwd(g_u3, s[3])
-        g_a4a = lin("c4b", t["a4a"], silu_bwd(t["p4b"], g_a4b), s[3])
-        g_q3 = lin("c4a", t["q3"], silu_bwd(t["p4a"], g_a4a), s[3])
-        g_s3 = add_f32(g_s3, pool_bwd(g_q3, s[2]))
-        g_a3a = lin("c3b", t["a3a"], silu_bwd(t["p3b"], g_s3), s[2])
-        g_q2 = lin("c3a", t["q2"], silu_bwd(t["p3a"], g_a3a), s[2])
-        g_s2 = add_f32(g_s2, pool_bwd(g_q2, s[1]))
-        g_a2a = lin("c2b", t["a2a"], silu_bwd(t["p2b"], g_s2), s[1])
-        g_q1 = lin("c2a", t["q1"], silu_bwd(t["p2a"], g_a2a), s[1])
-        g_s1 = add_f32(g_s1, pool_bwd(g_q1, s[0]))
-        g_a1a = lin("c1b", t["a1a"], silu_bwd(t["p1b"], g_s1), s[0])
-        cv["c1a"].wgrad(t["x"], silu_bwd(t["p1a"], g_a1a), s[0])
+        g_u3 = lin("d3a", t["u3"], dact("pd3a", g_k3), sh[2])
+        g_r4 = up_bwd(g_u3, sh[3])
+        g_a4c = lin("c4d", t["a4c"], dact("p4d", g_r4), sh[3])
+        g_a4b = add_f32(g_r4, lin("c4c", t["a4b"], dact("p4c", g_a4c), sh[3]))
+        g_a4a = lin("c4b", t["a4a"], dact("p4b", g_a4b), sh[3])
+        g_q3 = lin("c4a", t["q3"], dact("p4a", g_a4a), sh[3])
+        g_s3 = add_f32(g_s3, pool_bwd(g_q3, sh[2]))
+        g_a3a = lin("c3b", t["a3a"], dact("p3b", g_s3), sh[2])
+        g_q2 = lin("c3a", t["q2"], dact("p3a", g_a3a), sh[2])
+        g_s2 = add_f32(g_s2, pool_bwd(g_q2, sh[1]))
+        g_a2a = lin("c2b", t["a2a"], dact("p2b", g_s2), sh[1])
+        g_q1 = lin("c2a", t["q1"], dact("p2a", g_a2a), sh[1])
+        g_s1 = add_f32(g_s1, pool_bwd(g_q1, sh[0]))
+        g_a1a = lin("c1b", t["a1a"], dact("p1b", g_s1), sh[0])
+        cv["c1a"].wgrad(t["x"], dact("p1a", g_a1a), sh[0])
+        # every FiLM row entry was written by its site's film_bwd; on through the MLP
+        g_h1 = self.film2.bwd(t["h1"], t["film"], dfilm)
+        self.film1.bwd(t["s"], t["h1pre"], g_h1, want_gx=False)
         self._tape = None
 
     def zero_grad(self):
         for c in self.conv.values():
             c.zero_grad()
+        self.film1.zero_grad()
+        self.film2.zero_grad()
 
     def adam_step(self, lr, wd=0.0):
         self.step_count += 1
         for c in self.conv.values():
             c.adam(self.step_count, lr, wd=wd)
+        self.film1.adam(self.step_count, lr)
+        self.film2.adam(self.step_count, lr)
 
     def state_dict(self):
         d = {f"{k}.{n}": v for k, c in self.conv.items() for n, v in c.state().items()}
+        for k, m in (("film1", self.film1), ("film2", self.film2)):
+            for n, v in m.state().items():
+                d[f"{k}.{n}"] = v
         if self.norm is not None:
             d["norm.mean"], d["norm.std"] = self.norm
         return d
 
     def load_state_dict(self, d):
+        """Loads every layer present in ``d``.  Weights saved before the residual block and the FiLM layers
+        existed leave those at their initial values (an identity block, zero modulation): the same function."""
         for k, c in self.conv.items():
-            c.load({"w": d[f"{k}.w"], "b": d[f"{k}.b"]})
+            if f"{k}.w" in d:
+                c.load({"w": d[f"{k}.w"], "b": d[f"{k}.b"]})
+            elif k not in ("c4c", "c4d"):
+                raise KeyError(f"weights file lacks layer {k}")
+        for k, m in (("film1", self.film1), ("film2", self.film2)):
+            if f"{k}.w" in d:
+                m.load({"w": d[f"{k}.w"], "b": d[f"{k}.b"]})
         if "norm.mean" in d:
             self.norm = (np.asarray(d["norm.mean"], np.float32), np.asarray(d["norm.std"], np.float32))
 
     def save(self, path):
-        np.savez_compressed(path, **{k: (v.astype(np.float16) if k.endswith(".w") else v)
-                                     for k, v in self.state_dict().items()})
+        np.savez_compressed(path, **{k: (v.astype(np.float16) if k.endswith(".w") and not k.startswith("film")
+                                         else v) for k, v in self.state_dict().items()})
 
     @classmethod
     def load_file(cls, path, device="cuda"):

@@ -64,42 +64,53 @@ def make_design_params(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
     return out
 
 
-def _sample(d, fields):
+def _sample(d, fields, inputs=None):
+    """One training pair on the GPU: (features bf16, raw targets, loss weights, design scalars).
+    ``inputs``: the design's device (mask, normalised heat) when the solver built it."""
     rho, u, T = fields
-    mask, heat = device_inputs(dict(d))
+    mask, heat = inputs if inputs is not None else device_inputs(dict(d))
     f = S.features_device(mask, heat, d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
     t, w = S.targets(mask.cpu().numpy(), rho, u, T, d["u_inlet_val"], d["heat_power"], PAD)
-    return f.to(torch.bfloat16), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda()
+    sc = torch.from_numpy(S.scalars(d["tau_fluid"], d["heat_power"], d["u_inlet_val"])).cuda()
+    return f.to(torch.bfloat16), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda(), sc
 
 
-def generate(designs, slots=64, tol=1e-6, max_steps=60000, log=print, confirm=CONFIRM, stop_rule="reference",
-             keep_unconverged=False):
-    """Cold batched solves; returns (samples, results) for the designs that converged (and, with
-    ``keep_unconverged``, those stopped at ``max_steps``, whose fields are close to the steady state).
+def _stack(samples, idx, m_t, s_t):
+    """Minibatch tensors (x, standardised targets, weights, scalars) of samples[idx]."""
+    x = torch.stack([samples[i][0] for i in idx])
+    t = ((torch.stack([samples[i][1] for i in idx]) - m_t) / s_t).contiguous()
+    w = torch.stack([samples[i][2] for i in idx])
+    sc = torch.stack([samples[i][3] for i in idx]).contiguous()
+    return x, t, w, sc
 
-    The targets must be the converged state the warm runs will be stopped at: the thermal field relaxes
-    with a time constant of ~15,000 steps on these lattices, so fields stopped at residual 1e-5 still
-    differ from the 1e-6 ones by 10-25 % in R_th."""
-    bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
-                       collect_fields=True, metrics=False, confirm=confirm, stop_rule=stop_rule)
+
+def generate(designs, net=None, slots=3, tol=1e-6, max_steps=120000, log=print, confirm=CONFIRM,
+             stop_rule="rise", keep_unconverged=False):
+    """Training pairs from the batched solver: returns (samples, results) for the designs that converged
+    (and, with ``keep_unconverged``, those stopped at ``max_steps``).
+
+    With ``net`` the solves start from its predictions (the warm-start path, fallback and physics gate
+    included): they reach the same steady state as cold solves, in fewer steps.  The targets must be the
+    converged state the warm runs will be stopped at, so the stop rule and tolerance are the evaluation's:
+    the thermal field relaxes with a time constant of ~15,000 steps on these lattices, and fields stopped at
+    residual 1e-5 still differ from the 1e-6 ones by 10-25 % in R_th."""
     by_key = {d["key"]: d for d in designs}
     samples, results = [], []
     t0 = time.perf_counter()
 
-    def on_result(r):
+    def keep(r):
+        fields, inputs = r.pop("fields"), r.pop("inputs")
         if r["status"] == "converged" or (keep_unconverged and r["status"] == "max_steps"):
-            samples.append(_sample(by_key[r["key"]], r.pop("fields")))
-        else:
-            r.pop("fields", None)
+            samples.append(_sample(by_key[r["key"]], fields, inputs))
         results.append(r)
         if len(results) % 32 == 0:
             log(f"[datagen] {len(results)}/{len(designs)} solved ({len(samples)} kept as samples), "
                 f"{time.perf_counter() - t0:.0f} s")
 
-    try:
-        bs.run(designs, on_result=on_result)
-    finally:
-        bs.close()
+    out = solve_designs(designs, net, slots=slots, tol=tol, max_steps=max_steps, log=None, confirm=confirm,
+                        stop_rule=stop_rule, collect_fields=True, on_final=keep)
+    log(f"[datagen] {len(designs)} designs in {out['wall']:.0f} s: {len(samples)} samples, "
+        f"{len(out['fallbacks'])} warm fallbacks")
     return samples, results
 
 
@@ -108,7 +119,7 @@ def fit_norm(samples):
     s1 = torch.zeros(S.NOUT, dtype=torch.float64, device="cuda")
     s2 = torch.zeros_like(s1)
     n = torch.zeros_like(s1)
-    for _, t, w in samples:
+    for _, t, w, _ in samples:
         tw = t.double() * w
         s1 += tw.sum(dim=(0, 1, 2))
         s2 += (tw * t.double()).sum(dim=(0, 1, 2))
@@ -135,10 +146,8 @@ def train(net, samples, epochs=40, batch=8, lr=5e-4, seed=0, val=None, log=print
         tot, cnt = 0.0, 0
         for b0 in range(0, len(order) - batch + 1, batch):
             idx = order[b0:b0 + batch]
-            x = torch.stack([samples[i][0] for i in idx])
-            t = (torch.stack([samples[i][1] for i in idx]) - m_t) / s_t
-            w = torch.stack([samples[i][2] for i in idx])
-            out = net.forward(x, keep=True)
+            x, t, w, sc = _stack(samples, idx, m_t, s_t)
+            out = net.forward(x, sc, keep=True)
             loss, g = S.mse_loss(out, t, w)
             net.zero_grad()
             net.backward(g)
@@ -165,14 +174,12 @@ def time_train_step(net, samples, batch=8, reps=3):
     mean, std = net.norm
     m_t, s_t = torch.tensor(mean, device="cuda"), torch.tensor(std, device="cuda")
     idx = list(range(min(batch, len(samples))))
-    x = torch.stack([samples[i][0] for i in idx])
-    t = (torch.stack([samples[i][1] for i in idx]) - m_t) / s_t
-    w = torch.stack([samples[i][2] for i in idx])
+    x, t, w, sc = _stack(samples, idx, m_t, s_t)
     dt = 0.0
     for _ in range(reps):
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        out = net.forward(x, keep=True)
+        out = net.forward(x, sc, keep=True)
         _, g = S.mse_loss(out, t, w)
         net.backward(g)
         torch.cuda.synchronize()
@@ -189,11 +196,8 @@ def evaluate_channels(net, samples, batch=8):
     num = torch.zeros(S.NOUT, dtype=torch.float64, device="cuda")
     den = torch.zeros_like(num)
     for b0 in range(0, len(samples), batch):
-        sl = samples[b0:b0 + batch]
-        x = torch.stack([s[0] for s in sl])
-        t = (torch.stack([s[1] for s in sl]) - m_t) / s_t
-        w = torch.stack([s[2] for s in sl])
-        out = net.forward(x)[..., :S.NOUT]
+        x, t, w, sc = _stack(samples, range(b0, min(b0 + batch, len(samples))), m_t, s_t)
+        out = net.forward(x, sc)[..., :S.NOUT]
         num += ((out - t) ** 2 * w).double().sum(dim=(0, 1, 2, 3))
         den += w.double().sum(dim=(0, 1, 2, 3))
     return (num / den).cpu().tolist()
@@ -205,13 +209,11 @@ def evaluate(net, samples, batch=8):
     m_t, s_t = torch.tensor(mean, device="cuda"), torch.tensor(std, device="cuda")
     tot, n = 0.0, 0
     for b0 in range(0, len(samples), batch):
-        sl = samples[b0:b0 + batch]
-        x = torch.stack([s[0] for s in sl])
-        t = (torch.stack([s[1] for s in sl]) - m_t) / s_t
-        w = torch.stack([s[2] for s in sl])
-        loss, _ = S.mse_loss(net.forward(x), t, w)
-        tot += loss.item() * len(sl)
-        n += len(sl)
+        idx = range(b0, min(b0 + batch, len(samples)))
+        x, t, w, sc = _stack(samples, idx, m_t, s_t)
+        loss, _ = S.mse_loss(net.forward(x, sc), t, w)
+        tot += loss.item() * len(idx)
+        n += len(idx)
     return tot / max(n, 1)
 
 
@@ -227,7 +229,8 @@ def predict_fields(net, d):
     drop, would otherwise depend on the prediction (measured: up to 13 %)."""
     mask, heat = device_inputs(dict(d))
     x = S.features_device(mask, heat, d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD).to(torch.bfloat16)[None]
-    out = net.forward(x)[0]
+    sc = torch.from_numpy(S.scalars(d["tau_fluid"], d["heat_power"], d["u_inlet_val"])).cuda()[None]
+    out = net.forward(x, sc)[0]
     nx, ny, nz = mask.shape
     mean, std = net.norm
     y = out[:nx, :ny, :nz, :S.NOUT] * torch.tensor(std, device="cuda") + torch.tensor(mean, device="cuda")
@@ -287,24 +290,28 @@ class _Stall:
 
 
 def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=print, predictor=None, timer=None,
-                  confirm=CONFIRM, stop_rule="reference", launch=None, min_steps_rule=False, gate=True):
+                  confirm=CONFIRM, stop_rule="reference", launch=None, min_steps_rule=False, gate=True,
+                  collect_fields=False, on_final=None):
     """One batched run; with ``net`` (or any ``predictor(design) -> (rho, u, T)``) designs start warm.
 
-    Fallback: a warm design that stalls or diverges is re-run cold in the same call, and so is a warm design
+    Fallback: a warm design that stalls or diverges is re-queued cold in the same run, and so is a warm design
     that converges to fields failing the physics acceptance gate (``gate``: checks.GATE, the field checks of
     analyze_dataset.py:184-325 reduced on the device; every result carries its ``checks``).  Both arms stop
     when the reference's residual test passes on ``confirm`` consecutive checks.  ``timer`` (optional):
     ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of passes; its ``stream`` is set to the solver's.
     ``launch``: launch-shape options of the fused pass (``block``, ``cells_per_thread``, ``xcd_remap``,
     ``alternate_sweep``); they change timing only.  ``min_steps_rule``: the reference's flow-through floor
-    (src/simulation.py:104; off by default here, on in the reference)."""
+    (src/simulation.py:104; off by default here, on in the reference).  ``collect_fields``: results carry
+    host copies of (rho, u, T) and the device inputs; ``on_final(r)`` sees each final result as its slot
+    retires (and may drop the fields)."""
     if predictor is None and net is not None:
         predictor = lambda d: predict_fields(net, d)  # noqa: E731
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     bs = BatchedSolver(slots, (CORE[0] + 20, CORE[1], CORE[2]), tol_ke=tol, tol_T=tol, max_steps=max_steps,
                        metrics=True, stall_check=_Stall(tol, tol), confirm=confirm, stop_rule=stop_rule,
-                       min_steps_rule=min_steps_rule, checks=gate, **(launch or {}))
+                       min_steps_rule=min_steps_rule, checks=True, collect_fields=collect_fields,
+                       **(launch or {}))
     queue = []
     t_inf = 0.0
     for d in designs:
@@ -326,36 +333,34 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
     fell_back = set()
     done = [0]
 
-    def progress(r):
+    def handle(r):
         done[0] += 1
         if log is not None and done[0] % 32 == 0:
             log(f"[solve] {'warm' if predictor is not None else 'cold'}: {done[0]} runs finished, "
                 f"{time.perf_counter() - t0:.0f} s")
+        k = r["key"]
+        why = None
+        if predictor is not None and k not in fell_back:
+            if r["status"] in ("stalled", "diverged"):
+                why = r["status"]  # the warm start did not lead to convergence
+            elif gate and r["converged"]:
+                failed = gate_failures(r["checks"])
+                if failed:  # converged to fields that fail the physics checks
+                    why = "checks:" + ",".join(failed)
+        if why is not None:  # rerun this design cold, at the end of the queue
+            fell_back.add(k)
+            fallbacks.append((k, why, r["steps"]))
+            final[k] = {"abandoned_steps": r["steps"]}
+            r.pop("fields", None)
+            return [{kk: v for kk, v in by_key[k].items() if kk not in ("init_fields", "warm")}]
+        r["abandoned_steps"] = final.get(k, {}).get("abandoned_steps", 0)
+        if on_final is not None:
+            on_final(r)
+        final[k] = r
+        return None
 
     try:
-        pending = queue
-        while pending:
-            res = bs.run(pending, timer=timer, on_result=progress)
-            pending = []
-            for r in res:
-                k = r["key"]
-                why = None
-                if predictor is not None and k not in fell_back:
-                    if r["status"] in ("stalled", "diverged"):
-                        why = r["status"]  # the warm start did not lead to convergence
-                    elif gate and r["converged"]:
-                        failed = gate_failures(r["checks"])
-                        if failed:  # converged to fields that fail the physics checks
-                            why = "checks:" + ",".join(failed)
-                if why is not None:  # rerun this design cold
-                    cold = {kk: v for kk, v in by_key[k].items() if kk not in ("init_fields", "warm")}
-                    pending.append(cold)
-                    fell_back.add(k)
-                    fallbacks.append((k, why, r["steps"]))
-                    final[k] = {"abandoned_steps": r["steps"]}
-                else:
-                    r["abandoned_steps"] = final.get(k, {}).get("abandoned_steps", 0)
-                    final[k] = r
+        bs.run(queue, timer=timer, on_result=handle)
     finally:
         lups = bs.lups
         bs.close()
@@ -459,12 +464,15 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-steps", type=int, default=30000)
     ap.add_argument("--slots", type=int, default=64)
-    ap.add_argument("--gen-tol", type=float, default=1e-6, help="residual of the cold solves that make the data")
-    ap.add_argument("--gen-max-steps", type=int, default=60000)
+    ap.add_argument("--gen-tol", type=float, default=1e-6, help="residual of the solves that make the data")
+    ap.add_argument("--gen-max-steps", type=int, default=120000)
+    ap.add_argument("--gen-slots", type=int, default=3, help="lattices per launch of the data-making solves")
+    ap.add_argument("--gen-cold", action="store_true",
+                    help="make the data with cold solves (default: warm-started from --init-weights when given)")
     ap.add_argument("--stop-rule", choices=["reference", "rise"], default="reference",
                     help="residuals of the reference, or the same on the temperature rise and absorbed heat")
-    ap.add_argument("--gen-stop-rule", choices=["reference", "rise"], default="reference",
-                    help="stop rule of the cold solves that make the training data")
+    ap.add_argument("--gen-stop-rule", choices=["reference", "rise"], default="rise",
+                    help="stop rule of the solves that make the training data")
     ap.add_argument("--keep-unconverged", action="store_true",
                     help="also train on designs stopped at --gen-max-steps (not on diverged ones)")
     ap.add_argument("--init-weights", default=None, help="continue training from these weights (and their norm)")
@@ -480,14 +488,14 @@ def main():
     log = lambda s: print(s, flush=True)
     if args.cmd == "train":
         t_start = time.perf_counter()
-        designs = make_designs(0 if args.seed0 is None else args.seed0, args.n_train + args.n_val)
-        samples, results = generate(designs, slots=args.slots, tol=args.gen_tol, max_steps=args.gen_max_steps,
-                                    log=log, stop_rule=args.gen_stop_rule, keep_unconverged=args.keep_unconverged)
+        designs = make_design_params(0 if args.seed0 is None else args.seed0, args.n_train + args.n_val)
+        net = S.UNet3D.load_file(args.init_weights) if args.init_weights else None
+        samples, results = generate(designs, None if args.gen_cold else net, slots=args.gen_slots,
+                                    tol=args.gen_tol, max_steps=args.gen_max_steps, log=log,
+                                    stop_rule=args.gen_stop_rule, keep_unconverged=args.keep_unconverged)
         nval = min(args.n_val, len(samples) // 8)
         val, tr = samples[:nval], samples[nval:]
-        if args.init_weights:
-            net = S.UNet3D.load_file(args.init_weights)
-        else:
+        if net is None:
             net = S.UNet3D(seed=0)
             net.norm = fit_norm(tr)
         log(f"[train] {len(tr)} train / {len(val)} val samples, {net.n_params()} parameters, norm {net.norm}")
