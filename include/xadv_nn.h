@@ -61,6 +61,13 @@ int xnn_film_fwd(const float* p, const float* film, float* y, int N, int64_t Vn,
 int xnn_film_bwd(const float* p, const float* gy, const float* film, float* gx, float* part, float* dfilm, int N,
                  int64_t Vn, int C, int off, int L, void* stream);
 int64_t xnn_film_part_floats(int N, int64_t Vn, int C);
+/* FiLM then SiLU in one pass, y (BF16) = silu(p * (1 + gam) + bet): the bits of xnn_film_fwd + xnn_silu_fwd. */
+int xnn_film_silu_fwd(const float* p, const float* film, void* y, int N, int64_t Vn, int C, int off, int L,
+                      void* stream);
+/* Its backward from gy (w.r.t. y, FP32): gxb (BF16, w.r.t. p) and dfilm[n][off .. off + 2C), the bits of
+ * xnn_silu_bwd_f32 (on the modulated p) + xnn_film_bwd + the BF16 cast.  part: xnn_film_part_floats floats. */
+int xnn_film_silu_bwd(const float* p, const float* gy, const float* film, void* gxb, float* part, float* dfilm,
+                      int N, int64_t Vn, int C, int off, int L, void* stream);
 /* Fully connected FP32 layer: pre = W x + b (W: O x I), post = silu(pre) if act else pre (post may be NULL). */
 int xnn_dense_fwd(const float* x, const float* W, const float* b, float* pre, float* post, int N, int I, int O,
                   int act, void* stream);

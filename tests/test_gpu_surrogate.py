@@ -308,6 +308,13 @@ def test_film_and_dense_kernels(gpu):
     assert rel(gx.cpu(), pr.grad) < 1e-6
     assert rel(dfilm.cpu()[:, off:off + 2 * C], fr.grad[:, off:off + 2 * C]) < 1e-5
     assert float(dfilm.cpu()[:, :off].abs().max()) == 0.0  # other sites' entries untouched
+    # the fused FiLM + SiLU kernels give the bits of the separate kernels
+    pc, fc, gc = p.cuda(), film.cuda(), gy.cuda()
+    assert torch.equal(S.film_silu_fwd(pc, fc, off).cpu(), S.silu_fwd(S.film_fwd(pc, fc, off)).cpu())
+    d1, d2 = torch.zeros(N, L, device="cuda"), torch.zeros(N, L, device="cuda")
+    fused = S.film_silu_bwd(pc, gc, fc, d1, off)
+    sep = S.to_bf16(S.film_bwd(pc, S.silu_bwd_f32(S.film_fwd(pc, fc, off), gc), fc, d2, off))
+    assert torch.equal(fused.cpu(), sep.cpu()) and torch.equal(d1.cpu(), d2.cpu())
     for act in (True, False):
         d = S.Dense(7, 40, gen, act=act)
         d.load({"w": torch.randn(40, 7, generator=gen), "b": torch.randn(40, generator=gen)})

@@ -1126,6 +1126,57 @@ __global__ __launch_bounds__(256) void k_film_bwd_part(const float* __restrict__
   }
 }
 
+// FiLM then SiLU in one pass: y = silu(p (1 + gam) + bet) in BF16 (the same operations, in the same order, as
+// k_film_fwd followed by k_silu_fwd, so the same bits).
+__global__ void k_film_silu_fwd(const float* __restrict__ p, const float* __restrict__ film, __bf16* __restrict__ y,
+                                int64_t Vn, int C, int off, int L, int64_t total) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int c = (int)(i % C);
+  const int n = (int)(i / ((int64_t)C * Vn));
+  const float* f = film + (int64_t)n * L + off;
+  y[i] = f2bf(silu(p[i] * (1.0f + f[c]) + f[C + c]));
+}
+
+// Backward of k_film_silu_fwd from the gradient gy w.r.t. its output: the modulated pre-activation is formed
+// again, gm = gy * silu'(pm), gx = gm * (1 + gam) in BF16, and the per-block partials of gm * p and gm as in
+// k_film_bwd_part (the same bits as k_silu_bwd_f32, k_film_bwd_part and the BF16 cast in sequence).
+__global__ __launch_bounds__(256) void k_film_silu_bwd_part(const float* __restrict__ p,
+                                                            const float* __restrict__ gy,
+                                                            const float* __restrict__ film,
+                                                            __bf16* __restrict__ gxb, float* __restrict__ part,
+                                                            int64_t Vn, int C, int off, int L, int64_t vper) {
+  __shared__ float red[2][256];
+  const int n = blockIdx.y, b = blockIdx.x, nbn = gridDim.x;
+  const int64_t v0 = (int64_t)b * vper;
+  const int64_t v1 = v0 + vper < Vn ? v0 + vper : Vn;
+  const int c = threadIdx.x % C, r = threadIdx.x / C, R = 256 / C;
+  const float* f = film + (int64_t)n * L + off;
+  const float g1 = 1.0f + f[c], bt = f[C + c];
+  float sgp = 0.0f, sg = 0.0f;
+  for (int64_t v = v0 + r; v < v1; v += R) {
+    const int64_t i = ((int64_t)n * Vn + v) * C + c;
+    const float pv = p[i];
+    const float g = gy[i] * dsilu(pv * g1 + bt);
+    sgp += g * pv;
+    sg += g;
+    gxb[i] = f2bf(g * g1);
+  }
+  red[0][threadIdx.x] = sgp;
+  red[1][threadIdx.x] = sg;
+  __syncthreads();
+  if (threadIdx.x < C) {
+    float a = 0.0f, e = 0.0f;
+    for (int k = 0; k < R; ++k) {
+      a += red[0][k * C + c];
+      e += red[1][k * C + c];
+    }
+    float* q = part + ((int64_t)n * nbn + b) * (2 * C);
+    q[c] = a;
+    q[C + c] = e;
+  }
+}
+
 // dfilm[n][off + j] = sum over the sample's blocks (in block order) of part[n][b][j], j < 2C.
 __global__ void k_film_bwd_final(const float* __restrict__ part, float* __restrict__ dfilm, int N, int nbn, int C,
                                  int off, int L) {
@@ -1263,7 +1314,6 @@ int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void*
 }
 
 int64_t xnn_bias_part_floats(int64_t V, int C) { return bias_blocks(V) * C; }
-
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream) {
   if (vec_ok(n, {x, y}))
     hipLaunchKernelGGL(k_silu_fwd4, dim3(nblk((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y,
@@ -1368,6 +1418,26 @@ int xnn_film_fwd(const float* p, const float* film, float* y, int N, int64_t Vn,
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int64_t xnn_film_part_floats(int N, int64_t Vn, int C) { return (int64_t)N * film_blocks(Vn) * 2 * C; }
+int xnn_film_silu_fwd(const float* p, const float* film, void* y, int N, int64_t Vn, int C, int off, int L,
+                      void* stream) {
+  if (N < 1 || Vn < 1 || C < 1 || off < 0 || off + 2 * C > L) return 1;
+  const int64_t total = (int64_t)N * Vn * C;
+  hipLaunchKernelGGL(k_film_silu_fwd, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, p, film, (__bf16*)y,
+                     Vn, C, off, L, total);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_film_silu_bwd(const float* p, const float* gy, const float* film, void* gxb, float* part, float* dfilm,
+                      int N, int64_t Vn, int C, int off, int L, void* stream) {
+  if (N < 1 || N > 65535 || Vn < 1 || C < 1 || C > 256 || 256 % C || off < 0 || off + 2 * C > L) return 1;
+  const int nbn = film_blocks(Vn);
+  const int64_t vper = (Vn + nbn - 1) / nbn;
+  hipStream_t s = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_film_silu_bwd_part, dim3(nbn, N), dim3(256), 0, s, p, gy, film, (__bf16*)gxb, part, Vn, C,
+                     off, L, vper);
+  hipLaunchKernelGGL(k_film_bwd_final, dim3(nblk((int64_t)N * 2 * C)), dim3(256), 0, s, part, dfilm, N, nbn, C, off,
+                     L);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
 int xnn_film_bwd(const float* p, const float* gy, const float* film, float* gx, float* part, float* dfilm, int N,
                  int64_t Vn, int C, int off, int L, void* stream) {
   if (N < 1 || N > 65535 || Vn < 1 || C < 1 || C > 256 || 256 % C || off < 0 || off + 2 * C > L) return 1;

@@ -73,6 +73,8 @@ def lib():
         L.xnn_silu_bwd_f32.argtypes = [vp, vp, vp, i64, vp]
         L.xnn_film_fwd.argtypes = [vp, vp, vp, i, i64, i, i, i, vp]
         L.xnn_film_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i, i64, i, i, i, vp]
+        L.xnn_film_silu_fwd.argtypes = [vp, vp, vp, i, i64, i, i, i, vp]
+        L.xnn_film_silu_bwd.argtypes = [vp, vp, vp, vp, vp, vp, i, i64, i, i, i, vp]
         L.xnn_film_part_floats.argtypes = [i, i64, i]
         L.xnn_film_part_floats.restype = i64
         L.xnn_dense_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, vp]
@@ -150,6 +152,28 @@ def film_bwd(p, gy, film, dfilm, off):
     gx = torch.empty_like(p)
     _ok(L.xnn_film_bwd(_p(p), _p(gy), _p(film), _p(gx), _p(part), _p(dfilm), N, Vn, C, off, film.shape[1], _st()),
         "xnn_film_bwd")
+    return gx
+
+
+def film_silu_fwd(p, film, off):
+    """silu(film_fwd(p, film, off)) in BF16, one pass (the same bits)."""
+    N, C = p.shape[0], p.shape[-1]
+    y = torch.empty(p.shape, dtype=torch.bfloat16, device=p.device)
+    _ok(lib().xnn_film_silu_fwd(_p(p), _p(film), _p(y), N, p.numel() // (N * C), C, off, film.shape[1], _st()),
+        "xnn_film_silu_fwd")
+    return y
+
+
+def film_silu_bwd(p, gy, film, dfilm, off):
+    """Backward of film_silu_fwd from gy (FP32, w.r.t. its output): BF16 gradient w.r.t. p; writes
+    dfilm[:, off:off+2C] (the bits of silu_bwd_f32 + film_bwd + to_bf16)."""
+    N, C = p.shape[0], p.shape[-1]
+    Vn = p.numel() // (N * C)
+    L = lib()
+    part = torch.empty(L.xnn_film_part_floats(N, Vn, C), dtype=torch.float32, device=p.device)
+    gx = torch.empty(p.shape, dtype=torch.bfloat16, device=p.device)
+    _ok(L.xnn_film_silu_bwd(_p(p), _p(gy), _p(film), _p(gx), _p(part), _p(dfilm), N, Vn, C, off, film.shape[1],
+                            _st()), "xnn_film_silu_bwd")
     return gx
 
 
@@ -372,12 +396,9 @@ class UNet3D:
         t["film"] = film
         off = {name: o for name, o, _ in self.FILM}
 
-        def act(name, p):  # FiLM (at its sites) then SiLU; the tape keeps both pre-activations
+        def act(name, p):  # FiLM (at its sites) then SiLU; the tape keeps the conv output
             t[name] = p
-            if name in off:
-                p = film_fwd(p, film, off[name])
-                t[name + "m"] = p
-            return silu_fwd(p)
+            return film_silu_fwd(p, film, off[name]) if name in off else silu_fwd(p)
 
         # encoder
         t["a1a"] = act("p1a", cv["c1a"].fwd(x, sh[0]))
@@ -427,8 +448,7 @@ class UNet3D:
         def dact(name, g):  # gradient w.r.t. the conv output p (bf16) from the gradient w.r.t. SiLU's output
             if name not in off:
                 return silu_bwd(t[name], g)
-            gm = silu_bwd_f32(t[name + "m"], g)
-            return to_bf16(film_bwd(t[name], gm, t["film"], dfilm, off[name]))
+            return film_silu_bwd(t[name], g, t["film"], dfilm, off[name])
 
         g = to_bf16(gout)
         g_ad1b = lin("head", t["ad1b"], g, sh[0])
@@ -581,8 +601,10 @@ def features_device(mask, heat, u_in, tau, P, pad_to=None):
     return f
 
 
-def targets(mask, rho, u, T, u_in, P, pad_to=None):
-    """(X8, Y8, Z8, 5) raw targets and (.., 5) loss weights for one solved design."""
+def targets(mask, rho, u, T, u_in, P, pad_to=None, t_weight=1.0):
+    """(X8, Y8, Z8, 5) raw targets and (.., 5) loss weights for one solved design: u and rho weighted 1 on
+    non-solid cells, T weighted ``t_weight`` on every cell (the thermal field sets how long a warm run takes
+    to converge)."""
     nx, ny, nz = mask.shape
     X, Y, Z = pad_to or (round8(nx), round8(ny), round8(nz))
     t = np.zeros((X, Y, Z, NOUT), np.float32)
@@ -592,5 +614,5 @@ def targets(mask, rho, u, T, u_in, P, pad_to=None):
     t[:nx, :ny, :nz, 3] = (rho - 1.0) / u_in
     t[:nx, :ny, :nz, 4] = (T - T_IN) / P
     w[:nx, :ny, :nz, 0:4] = fluid[..., None]
-    w[:nx, :ny, :nz, 4] = 1.0
+    w[:nx, :ny, :nz, 4] = t_weight
     return t, w

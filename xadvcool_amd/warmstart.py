@@ -64,13 +64,24 @@ def make_design_params(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
     return out
 
 
-def _sample(d, fields, inputs=None):
+def config2_design_params(seed0, n):
+    """n full-size single-solve designs (gyroid, quad_core heat map, u_in up to 0.05) as parameters only."""
+    from .geometry import config2_params
+    out = []
+    for s in range(seed0, seed0 + n):
+        p = config2_params(s)
+        out.append(dict(params=p, key=s, tau_fluid=p["tau_fluid"], u_inlet_val=p["u_inlet_val"],
+                        heat_power=p["heat_power"], family=p["family"], heat_name=p["heat_name"]))
+    return out
+
+
+def _sample(d, fields, inputs=None, t_weight=1.0):
     """One training pair on the GPU: (features bf16, raw targets, loss weights, design scalars).
     ``inputs``: the design's device (mask, normalised heat) when the solver built it."""
     rho, u, T = fields
     mask, heat = inputs if inputs is not None else device_inputs(dict(d))
     f = S.features_device(mask, heat, d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD)
-    t, w = S.targets(mask.cpu().numpy(), rho, u, T, d["u_inlet_val"], d["heat_power"], PAD)
+    t, w = S.targets(mask.cpu().numpy(), rho, u, T, d["u_inlet_val"], d["heat_power"], PAD, t_weight=t_weight)
     sc = torch.from_numpy(S.scalars(d["tau_fluid"], d["heat_power"], d["u_inlet_val"])).cuda()
     return f.to(torch.bfloat16), torch.from_numpy(t).cuda(), torch.from_numpy(w).cuda(), sc
 
@@ -85,7 +96,7 @@ def _stack(samples, idx, m_t, s_t):
 
 
 def generate(designs, net=None, slots=3, tol=1e-6, max_steps=120000, log=print, confirm=CONFIRM,
-             stop_rule="rise", keep_unconverged=False):
+             stop_rule="rise", keep_unconverged=False, t_weight=1.0):
     """Training pairs from the batched solver: returns (samples, results) for the designs that converged
     (and, with ``keep_unconverged``, those stopped at ``max_steps``).
 
@@ -101,7 +112,7 @@ def generate(designs, net=None, slots=3, tol=1e-6, max_steps=120000, log=print, 
     def keep(r):
         fields, inputs = r.pop("fields"), r.pop("inputs")
         if r["status"] == "converged" or (keep_unconverged and r["status"] == "max_steps"):
-            samples.append(_sample(by_key[r["key"]], fields, inputs))
+            samples.append(_sample(by_key[r["key"]], fields, inputs, t_weight=t_weight))
         results.append(r)
         if len(results) % 32 == 0:
             log(f"[datagen] {len(results)}/{len(designs)} solved ({len(samples)} kept as samples), "
@@ -467,6 +478,11 @@ def main():
     ap.add_argument("--gen-tol", type=float, default=1e-6, help="residual of the solves that make the data")
     ap.add_argument("--gen-max-steps", type=int, default=120000)
     ap.add_argument("--gen-slots", type=int, default=3, help="lattices per launch of the data-making solves")
+    ap.add_argument("--t-weight", type=float, default=1.0,
+                    help="train: loss weight of the temperature channel relative to u and rho")
+    ap.add_argument("--config2-frac", type=float, default=0.0,
+                    help="train: share of the designs drawn as full-size single solves (gyroid, quad_core, "
+                         "u_in up to 0.05) beside the all-family designs")
     ap.add_argument("--gen-cold", action="store_true",
                     help="make the data with cold solves (default: warm-started from --init-weights when given)")
     ap.add_argument("--stop-rule", choices=["reference", "rise"], default="reference",
@@ -488,11 +504,24 @@ def main():
     log = lambda s: print(s, flush=True)
     if args.cmd == "train":
         t_start = time.perf_counter()
-        designs = make_design_params(0 if args.seed0 is None else args.seed0, args.n_train + args.n_val)
+        seed0 = 0 if args.seed0 is None else args.seed0
+        n = args.n_train + args.n_val
+        n2 = int(round(args.config2_frac * n))
+        designs = make_design_params(seed0, n - n2)
+        if n2:  # interleaved, so the validation designs (the first samples) hold both kinds
+            c2 = config2_design_params(seed0 + 500_000, n2)
+            step = max(1, (n - n2) // n2)
+            mixed = []
+            for i, d in enumerate(designs):
+                mixed.append(d)
+                if i % step == step - 1 and c2:
+                    mixed.append(c2.pop(0))
+            designs = mixed + c2
         net = S.UNet3D.load_file(args.init_weights) if args.init_weights else None
         samples, results = generate(designs, None if args.gen_cold else net, slots=args.gen_slots,
                                     tol=args.gen_tol, max_steps=args.gen_max_steps, log=log,
-                                    stop_rule=args.gen_stop_rule, keep_unconverged=args.keep_unconverged)
+                                    stop_rule=args.gen_stop_rule, keep_unconverged=args.keep_unconverged,
+                                    t_weight=args.t_weight)
         nval = min(args.n_val, len(samples) // 8)
         val, tr = samples[:nval], samples[nval:]
         if net is None:
