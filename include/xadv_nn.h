@@ -31,6 +31,13 @@ int64_t xnn_wgrad_part_floats(int N, int X, int Y, int Z, int Cin, int Cout, int
  * floats), fixed order. */
 int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void* stream);
 int64_t xnn_bias_part_floats(int64_t V, int C);
+/* xnn_bias_grad without the workspace argument and without allocating: the per-block partials live in a
+ * static device array of the library (1 MiB per GPU, enough for every shape xnn_bias_grad takes: at most 1024
+ * blocks x 256 channels).  No counterpart in the reference; for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid it).  Calls on one GPU share the array:
+ * calls that overlap on the same GPU race.  Returns 1 for an unsupported C, 3 if the partials would not fit
+ * the array, 2 for a HIP error. */
+int xnn_bias_grad_noalloc(const void* g, float* db, int64_t V, int C, void* stream);
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream);
 int xnn_silu_bwd(const float* x, const float* gy, void* gxb, int64_t n, void* stream);
 int xnn_silu_bwd_f32(const float* x, const float* gy, float* gx, int64_t n, void* stream);

@@ -16,6 +16,7 @@
 #include <hip/hip_runtime.h>
 
 #include <initializer_list>
+#include <mutex>
 
 #include <cmath>
 #include <cstdint>
@@ -1248,6 +1249,31 @@ int64_t bias_blocks(int64_t V) {
   return nb < 1 ? 1 : (nb > 1024 ? 1024 : nb);
 }
 
+// Device scratch of the allocation-free bias gradient: the per-block partials of k_bias_part, at most
+// 1024 blocks x 256 channels (bias_blocks caps the blocks at 1024 and the kernels take C <= 256), 1 MiB per
+// GPU for the life of the process.  Every call on a GPU shares it: calls that overlap on one GPU race.
+constexpr int64_t kScratchFloats = 1024 * 256;
+static __device__ __align__(256) float scratch[kScratchFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of ``scratch`` on the current device, looked up once per device (each GPU holds its own
+// copy of the array); the host-side name of the array is not a device address.
+int scratch_on_current_device(float** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 2;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || p == nullptr) return 2;
+    g_scratch_ptr[dev] = static_cast<float*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1314,6 +1340,20 @@ int xnn_bias_grad(const void* g, float* db, float* part, int64_t V, int C, void*
 }
 
 int64_t xnn_bias_part_floats(int64_t V, int C) { return bias_blocks(V) * C; }
+int xnn_bias_grad_noalloc(const void* g, float* db, int64_t V, int C, void* stream) {
+  if (C < 8 || C > 256 || C % 8 || 256 % (C / 8)) return 1;
+  const int64_t nb = bias_blocks(V);
+  if (nb * C > kScratchFloats) return 3;
+  float* part = nullptr;
+  if (scratch_on_current_device(&part) != 0) return 2;
+  const int64_t vper = (V + nb - 1) / nb;
+  hipStream_t s = (hipStream_t)stream;
+  // k_bias_part writes all nb * C partials before k_bias_final reads them: nothing to clear between calls
+  hipLaunchKernelGGL(k_bias_part, dim3((unsigned)nb), dim3(256), 0, s, (const __bf16*)g, part, V, C, vper);
+  hipLaunchKernelGGL(k_bias_final, dim3(1), dim3(256), 0, s, part, db, (int)nb, C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+
 int xnn_silu_fwd(const float* x, void* y, int64_t n, void* stream) {
   if (vec_ok(n, {x, y}))
     hipLaunchKernelGGL(k_silu_fwd4, dim3(nblk((n + 3) / 4)), dim3(256), 0, (hipStream_t)stream, x, (__bf16*)y,
