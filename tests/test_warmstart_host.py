@@ -73,3 +73,37 @@ def test_decay_slower_than_the_stall_bound_is_abandoned():
     dT = 3e-6 * np.exp(-np.arange(400) / 2000.0)
     out = _feed(st, {"key": 6, "warm": True}, dT)
     assert True in out
+
+
+def test_compare_summary_counts_and_ratios():
+    """The benchmark's summary from synthetic results: converged counts, solves per hour, agreement only where
+    both arms converged, fallback reasons, per-family ratios."""
+    from xadvcool_amd.checks import GATE
+    ok = {k: True for k in GATE}
+
+    def res(key, steps, status, R, dP):
+        return {"key": key, "steps": steps, "status": status, "converged": status == "converged",
+                "metrics": {"R_th": R, "pressure_drop": dP, "Nu": 1.0, "P_pump": 1.0}, "checks": dict(ok)}
+
+    designs = [{"key": k, "family": f, "heat_name": "uniform", "u_inlet_val": 0.02}
+               for k, f in ((1, "gyroid"), (2, "gyroid"), (3, "straight"))]
+    cold = {"results": {1: res(1, 60000, "converged", 1.0, 0.01), 2: res(2, 120000, "max_steps", 1.0, 0.01),
+                        3: res(3, 20000, "converged", 2.0, 0.02)}, "wall": 100.0, "fallbacks": [],
+            "inference_s": 0.0}
+    w1 = dict(res(1, 20000, "converged", 1.004, 0.01), abandoned_steps=0)
+    w2 = dict(res(2, 30000, "converged", 1.0, 0.01), abandoned_steps=10000)
+    w3 = dict(res(3, 5000, "converged", 2.02, 0.02), abandoned_steps=0)
+    warm = {"results": {1: w1, 2: w2, 3: w3}, "wall": 25.0, "fallbacks": [(2, "stalled", 10000)],
+            "inference_s": 0.1}
+    rows = W.compare_rows(designs, cold, warm)
+    assert [r["warm_steps"] for r in rows] == [20000, 40000, 5000]
+    s = W.summarize(rows, cold, warm)
+    assert s["cold"]["converged"] == 2 and s["warm"]["converged"] == 3
+    assert s["cold"]["solves_per_hr"] == 72.0 and s["warm"]["solves_per_hr"] == 432.0
+    assert s["speedup_solves_per_hr"] == 6.0
+    assert s["both_converged"] == 2 and s["within_0.5pct"] == 1 and s["warm_only_converged"] == 1
+    assert abs(s["max_d_R_th"] - 0.01) < 1e-12
+    assert s["warm"]["fallback_reasons"] == {"stalled": 1}
+    g = s["by_family"]["gyroid"]
+    assert (g["cold_converged"], g["warm_converged"], g["both"], g["within_0.5pct"]) == (1, 2, 1, 1)
+    assert g["converged_per_step_ratio"] == round((2 / 60000) / (1 / 180000), 3)

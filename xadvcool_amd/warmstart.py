@@ -446,6 +446,11 @@ def summarize(rows, cold, warm):
             fam["within_0.5pct"] += r["d_R_th"] <= 5e-3 and r["d_pressure_drop"] <= 5e-3
     for fam in by_family.values():  # steps-weighted: each arm's share of the GPU time goes with its steps
         fam["steps_ratio_cold_over_warm"] = round(fam["cold_steps"] / max(fam["warm_steps"], 1), 3)
+        # converged designs per lattice step of each arm (every design of the table has the same lattice size,
+        # so this is the ratio of converged solves per hour the family would see alone)
+        if fam["cold_converged"]:
+            fam["converged_per_step_ratio"] = round((fam["warm_converged"] / max(fam["warm_steps"], 1)) /
+                                                    (fam["cold_converged"] / fam["cold_steps"]), 3)
     summ["by_family"] = by_family
     return summ
 
