@@ -390,6 +390,10 @@ def compare_rows(designs, cold, warm):
                    cold_status=c["status"], warm_status=w["status"],
                    cold_gate=gate_failures(c["checks"]) if "checks" in c else None,
                    warm_gate=gate_failures(w["checks"]) if "checks" in w else None)
+        for arm, r in (("cold", c), ("warm", w)):
+            if r.get("converged"):
+                row[arm + "_R_th"] = r["metrics"]["R_th"]
+                row[arm + "_pressure_drop"] = r["metrics"]["pressure_drop"]
         if c["converged"] and w["converged"]:
             for m in ("pressure_drop", "R_th", "Nu", "P_pump"):
                 a, b = w["metrics"][m], c["metrics"][m]
