@@ -200,16 +200,16 @@ class BatchedSolver:
                     self._start(s, queue.popleft())
         return results
 
-    def run(self, designs, on_result=None, timer=None):
+    def run(self, designs, on_result=None, timer=None, progress=None):
         """Solve every design in ``designs``; returns results in completion order.
 
         ``on_result(r)`` (optional) sees each result as its slot retires and may return designs to append to
         the queue.  ``timer`` (optional) gets ``begin()`` / ``end(n, bytes_per_pass)`` around every chunk of
         passes."""
-        return run_grouped([self], designs, on_result=on_result, timer=timer)
+        return run_grouped([self], designs, on_result=on_result, timer=timer, progress=progress)
 
 
-def run_grouped(solvers, designs, on_result=None, timer=None):
+def run_grouped(solvers, designs, on_result=None, timer=None, progress=None):
     """Solve ``designs`` on several batched solvers that share one queue, one graph chunk per solver in turn.
 
     With small batches (a few full-size lattices each), a batch's state stays partly in the 256 MB Infinity
@@ -220,10 +220,14 @@ def run_grouped(solvers, designs, on_result=None, timer=None):
     results = []
     for bs in solvers:
         bs.fill(queue)
+    t_last = time.perf_counter()
     while True:
         active = [bs for bs in solvers if bs.busy()]
         if not active:
             return results
+        if progress is not None and time.perf_counter() - t_last > 60.0:  # state of a long run, once a minute
+            t_last = time.perf_counter()
+            progress(sum(sum(st is not None for st in b.state) for b in solvers), len(queue), len(results))
         for bs in active:
             if timer is not None and len(solvers) > 1:
                 timer.stream = bs.batch.stream()  # each group's passes run on its own batch's stream

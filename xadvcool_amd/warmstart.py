@@ -370,8 +370,13 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
         final[k] = r
         return None
 
+    def progress_line(active, queued, finished):
+        if log is not None:
+            log(f"[solve] {'warm' if predictor is not None else 'cold'}: {active} running, {queued} queued, "
+                f"{len(final)} finished, {time.perf_counter() - t0:.0f} s")
+
     try:
-        bs.run(queue, timer=timer, on_result=handle)
+        bs.run(queue, timer=timer, on_result=handle, heartbeat=heartbeat)
     finally:
         lups = bs.lups
         bs.close()
