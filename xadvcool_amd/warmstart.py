@@ -376,7 +376,7 @@ def solve_designs(designs, net=None, slots=64, tol=1e-6, max_steps=30000, log=pr
                 f"{len(final)} finished, {time.perf_counter() - t0:.0f} s")
 
     try:
-        bs.run(queue, timer=timer, on_result=handle, heartbeat=heartbeat)
+        bs.run(queue, timer=timer, on_result=handle, progress=progress_line)
     finally:
         lups = bs.lups
         bs.close()
