@@ -80,12 +80,12 @@ def test_unphysical_converged_warm_result_is_rerun_cold(gpu):
 
 
 def test_trained_surrogate_cuts_steps_on_unseen_designs(gpu):
-    """Eight unseen designs (all families), residual 1e-6 on five consecutive checks, 120,000-step cap.
+    """Eight unseen designs (all families), rise residuals at 3e-7 on five consecutive checks, 150,000-step cap.
 
-    Both arms stop on the rise residuals (the reference's formulas on the temperature rise and the
-    absorbed heat): under the reference's own residuals a cold run stops with R_th still percents from
-    its steady value (T_in dilutes the relative change), so the cold result could not judge the warm
-    one at 0.5 %.  Under the rise rule both arms land within about 0.5 % of the steady state."""
+    Both arms stop on the rise residuals (the reference's formulas on the temperature rise and the absorbed
+    heat).  A rate-based rule stops an exponential approach with time constant tau about tol * tau from its
+    steady state; on these lattices tau reaches ~15,000 steps, so at 1e-6 a cold run can stop 0.7 % away and
+    could not judge the warm one at 0.5 %; at 3e-7 it stops within ~0.2 %."""
     from xadvcool_amd import surrogate as S
     from xadvcool_amd import warmstart as W
 
@@ -93,15 +93,15 @@ def test_trained_surrogate_cuts_steps_on_unseen_designs(gpu):
     net = S.UNet3D.load_file(W.DEFAULT_WEIGHTS)
     n = 8
     designs = _designs(n, 5000)
-    print("[warmstart] cold vs warm, 8 designs, rise residuals", flush=True)
-    summ, rows = W.compare(net, designs, slots=n, tol=1e-6, max_steps=120000, stop_rule="rise")
+    print("[warmstart] cold vs warm, 8 designs, rise residuals at 3e-7", flush=True)
+    summ, rows = W.compare(net, designs, slots=n, tol=3e-7, max_steps=150000, stop_rule="rise")
     print(summ, flush=True)
     both = [r for r in rows if r["cold_status"] == "converged" and r["warm_status"] == "converged"]
-    assert len(both) >= 6
+    assert len(both) >= 7
     for r in both:  # the cold LBM is the reference for the warm result
         assert r["d_pressure_drop"] <= 5e-3, r
-        assert r["d_R_th"] <= 1.5e-2, r  # each arm within ~0.5 % of the steady state
-    assert np.median([r["d_R_th"] for r in both]) <= 5e-3
+        assert r["d_R_th"] <= 5e-3, r
     cold_steps = np.mean([r["cold_steps"] for r in both])
     warm_steps = np.mean([r["warm_steps"] for r in both])
-    assert warm_steps < 0.85 * cold_steps, (warm_steps, cold_steps)
+    assert warm_steps < 0.5 * cold_steps, (warm_steps, cold_steps)
+    assert summ["speedup_solves_per_hr"] >= 2.0, summ

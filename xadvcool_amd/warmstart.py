@@ -265,12 +265,15 @@ class _Stall:
       * grows: it is more than ``growth`` times its value ``grow_span`` checks
         earlier, or
       * stalls: over the last ``patience`` checks it stayed within a factor
-        ``plateau`` (neither decaying nor growing; a decay with the thermal time
-        constant, ~15,000 steps, spans a factor ~2 over 100 checks).
+        ``plateau`` (neither decaying nor growing).  A warm run on a TPMS core can
+        hold a plateau for 10,000-15,000 steps while two thermal modes cross and
+        then decay again, still far ahead of where a cold run is (measured: a
+        run abandoned this way at 26,000 steps had the residual a cold run only
+        reaches at ~80,000 steps), so the window is 300 checks.
     Cold runs are never abandoned.
     """
 
-    def __init__(self, tol_ke, tol_T, after=10000, window=20, growth=2.0, grow_span=30, patience=100, plateau=1.25):
+    def __init__(self, tol_ke, tol_T, after=10000, window=20, growth=2.0, grow_span=30, patience=300, plateau=1.25):
         self.tol_ke, self.tol_T = tol_ke, tol_T
         self.after, self.window, self.patience = after, window, patience
         self.growth, self.grow_span, self.plateau = growth, grow_span, plateau
