@@ -54,6 +54,10 @@ int xnn_up_bwd(const float* gfine, float* gcoarse, int N, int CX, int CY, int CZ
 int xnn_mse(const float* pred, const float* target, const float* wmask, float* grad, float* part, int64_t V, int C,
             int nout, float inv_norm, void* stream);
 int xnn_mse_blocks(int64_t V, int C);
+/* The same loss with the gradient written in BF16 and the normaliser read on the device, inv_norm =
+ * 1 / max(*wsum, 1) (wsum: one FP32 value, the sum of wmask): no host round trip between forward and backward. */
+int xnn_mse_bf16(const float* pred, const float* target, const float* wmask, const float* wsum, void* grad,
+                 float* part, int64_t V, int C, int nout, void* stream);
 /* Adam with bias correction on FP32 master weights; wb (may be NULL) receives the BF16 copy. */
 int xnn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, float lr, float b1, float b2,
              float eps, int step, float wd, void* stream);
@@ -75,6 +79,15 @@ int xnn_film_silu_fwd(const float* p, const float* film, void* y, int N, int64_t
  * xnn_silu_bwd_f32 (on the modulated p) + xnn_film_bwd + the BF16 cast.  part: xnn_film_part_floats floats. */
 int xnn_film_silu_bwd(const float* p, const float* gy, const float* film, void* gxb, float* part, float* dfilm,
                       int N, int64_t Vn, int C, int off, int L, void* stream);
+/* One activation site's backward in one pass (SiLU, or FiLM then SiLU when film is not NULL), from gy (FP32,
+ * w.r.t. the site's output) and its pre-activation p (N, Vn, C) FP32: gxb (BF16, w.r.t. p; elementwise the bits
+ * of xnn_silu_bwd / xnn_film_silu_bwd), dfilm[n][off .. off + 2C) when film is given, and db[c] (NULL to skip) =
+ * sum over all voxels of gxb, the bias gradient of the convolution that produced p.  Per-block partials in part
+ * (xnn_act_part_floats floats), summed in a fixed order.  C a multiple of 4 with C / 4 a power of two, C <= 256;
+ * p and gy 16-byte aligned, gxb 8-byte aligned. */
+int xnn_act_bwd(const float* p, const float* gy, const float* film, void* gxb, float* part, float* dfilm, float* db,
+                int N, int64_t Vn, int C, int off, int L, void* stream);
+int64_t xnn_act_part_floats(int N, int64_t Vn, int C, int film);
 /* Fully connected FP32 layer: pre = W x + b (W: O x I), post = silu(pre) if act else pre (post may be NULL). */
 int xnn_dense_fwd(const float* x, const float* W, const float* b, float* pre, float* post, int N, int I, int O,
                   int act, void* stream);

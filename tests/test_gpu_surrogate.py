@@ -83,7 +83,8 @@ def test_conv_lds_paths_match_reference_and_gather_kernel(gpu, cin, cout, shape)
     Against PyTorch (rtol 1e-3), and bit for bit against the gather kernels: zero-padding z by
     two planes sends the same convolution through k_conv_fwd16 / k_conv_fwd (Z is then none of 8,
     16, 32), and the padded result's inner z planes must equal the LDS kernel's output exactly
-    (same MFMA sequence per output element)."""
+    (same MFMA sequence per output element).  (The 128 -> 64 forward takes the split-K kernel with
+    and without the padding; its padded run must give the same bits too.)"""
     from xadvcool_amd import surrogate as S
     print(f"[nn] conv {cin}->{cout} LDS-halo path {shape}", flush=True)
     gen = torch.Generator().manual_seed(1600 + 7 * cin + cout + shape[3])
@@ -331,6 +332,73 @@ def test_film_and_dense_kernels(gpu):
         gxi = d.bwd(xi.cuda(), pre, g.cuda())
         assert rel(d.dw.cpu(), wr.grad) < 1e-5 and rel(d.db.cpu(), br.grad) < 1e-5
         assert rel(gxi.cpu(), xr.grad) < 1e-5
+
+
+@pytest.mark.parametrize("C,shape", [(16, (2, 9, 8, 32)), (32, (3, 5, 7, 16)), (64, (2, 3, 4, 8)),
+                                     (128, (8, 3, 4, 4)), (128, (1, 1, 1, 1))])
+def test_activation_backward_in_one_pass(gpu, C, shape):
+    """xnn_act_bwd (SiLU, and FiLM then SiLU): the BF16 gradient is bit for bit that of the elementwise kernels,
+    dfilm that of film_silu_bwd and the bias gradient that of xnn_bias_grad on the same BF16 gradient, up to FP32
+    summation order; the voxel counts leave partial blocks."""
+    from xadvcool_amd import surrogate as S
+    print(f"[nn] activation backward C={C} {shape}", flush=True)
+    gen = torch.Generator().manual_seed(C + shape[0])
+    N, L, off = shape[0], 2 * C + 40, 24
+    p = (torch.randn(*shape, C, generator=gen) * 2).cuda()
+    gy = torch.randn(*shape, C, generator=gen).cuda()
+    film = (torch.randn(N, L, generator=gen) * 0.3).cuda()
+    gx, db = S.act_bwd(p, gy)
+    assert torch.equal(gx, S.silu_bwd(p, gy))
+    ref_db = gx.double().reshape(-1, C).sum(0).float()
+    assert rel(db.cpu(), ref_db.cpu()) < 1e-5
+    d1, d2 = torch.zeros(N, L, device="cuda"), torch.zeros(N, L, device="cuda")
+    gxf, dbf = S.act_bwd(p, gy, film, d1, off)
+    assert torch.equal(gxf, S.film_silu_bwd(p, gy, film, d2, off))
+    assert rel(d1.cpu()[:, off:off + 2 * C], d2.cpu()[:, off:off + 2 * C]) < 1e-5
+    assert float(d1[:, :off].abs().max()) == 0.0 and float(d1[:, off + 2 * C:].abs().max()) == 0.0
+    assert rel(dbf.cpu(), gxf.double().reshape(-1, C).sum(0).float().cpu()) < 1e-5
+    gx2, db2 = S.act_bwd(p, gy, want_db=False)
+    assert db2 is None and torch.equal(gx2, gx)
+
+
+@pytest.mark.parametrize("cin,cout,shape", [(64, 128, (1, 3, 5, 3)), (128, 128, (2, 5, 4, 4)),
+                                             (128, 64, (1, 3, 5, 8)), (128, 128, (8, 19, 16, 4)),
+                                             (128, 32, (1, 2, 3, 4)), (64, 16, (1, 2, 2, 2))])
+def test_conv_split_k_coarse_levels(gpu, cin, cout, shape):
+    """The split-K convolution of the coarsest level (Z <= 4) and of 128 input channels, forward and data
+    gradient, against PyTorch (rtol 1e-3), with partial 16-voxel tiles and the network's bottleneck at batch 8."""
+    from xadvcool_amd import surrogate as S
+    print(f"[nn] split-K conv {cin}->{cout} {shape}", flush=True)
+    gen = torch.Generator().manual_seed(cin + cout + sum(shape))
+    c = S.Conv(cin, cout, gen)
+    c.b.copy_(torch.randn(cout, generator=gen))
+    x = torch.randn(*shape, cin, generator=gen)
+    out = c.fwd(x.to(torch.bfloat16).cuda(), shape).cpu()
+    W = torch_weight(bf(c.w.cpu()), cin, cout)
+    xr = ncdhw(bf(x)).clone().requires_grad_(True)
+    o = F.conv3d(xr, W, bias=c.b.cpu(), padding=1)
+    assert rel(out, ndhwc(o.detach())) < 1e-3
+    g = torch.randn(*shape, cout, generator=gen)
+    gin = c.dgrad(g.to(torch.bfloat16).cuda(), shape).cpu()
+    o.backward(ncdhw(bf(g)))
+    assert rel(gin, ndhwc(xr.grad)) < 1e-3
+
+
+def test_mse_gradient_in_bf16(gpu):
+    """mse_loss: the loss and the BF16 gradient (normaliser read on the device) against torch in FP32."""
+    from xadvcool_amd import surrogate as S
+    gen = torch.Generator().manual_seed(9)
+    pred = torch.randn(2, 5, 6, 7, 16, generator=gen)
+    tgt = torch.randn(2, 5, 6, 7, S.NOUT, generator=gen)
+    w = (torch.rand(2, 5, 6, 7, S.NOUT, generator=gen) > 0.4).float()
+    loss, g = S.mse_loss(pred.cuda(), tgt.cuda(), w.cuda())
+    pr = pred.clone().requires_grad_(True)
+    lr = ((pr[..., :S.NOUT] - tgt) ** 2 * w).sum() / w.sum()
+    lr.backward()
+    assert abs(loss.item() - lr.item()) / lr.item() < 1e-5
+    assert g.dtype == torch.bfloat16
+    assert torch.equal(g.float().cpu(), bf(pr.grad)) or rel(g.float().cpu(), pr.grad) < 4e-3
+    assert float(g[..., S.NOUT:].abs().max()) == 0.0
 
 
 def test_old_weights_load_as_the_same_function(gpu, tmp_path):

@@ -17,6 +17,7 @@
 
 #include <initializer_list>
 #include <mutex>
+#include <type_traits>
 
 #include <cmath>
 #include <cstdint>
@@ -28,6 +29,8 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(__bf16 h) { return (float)h; }
 __device__ __forceinline__ __bf16 f2bf(float f) { return (__bf16)f; }
@@ -240,7 +243,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
 // x (ZL + 2) voxels, zeros outside the lattice) is staged in LDS once, as in
 // k_conv_fwd16_lds.  Each wave owns YB * ZL / 64 tiles of 16 consecutive
 // voxels (with ZL = 8 a tile spans two z columns) and walks the K steps in
-// order, loading each step's B fragments once for all of its tiles: every
+// order, loading each step's B fragments once for all of its tiles (blocks of
+// four tiles: the waves share the tiles and split the output channels): every
 // output element gets the same MFMA sequence as in k_conv_fwd<NT>, so the
 // outputs are the same bits.
 template <int CIN, int NT, int ZL, int YB>
@@ -283,17 +287,28 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
     if (e < kChunks) *reinterpret_cast<u16x8*>(halo + e * 8) = tmp[i];
   }
   __syncthreads();
-  floatx4 acc[TPW][NT];
+  // The wave's share: TW tiles x GW 16-channel output groups.  With 4 tiles per block (one per wave) the
+  // waves take all tiles and split the output groups instead, so that each B fragment (a load from global
+  // memory) serves every tile of the block; every output element still gets the same MFMA sequence.
+  constexpr int TB = YB * ZL / 16;
+  constexpr bool kShare = TB == 4 && NT >= 2;
+  constexpr int GW = kShare ? (NT >= 4 ? NT / 4 : 1) : NT;
+  constexpr int TW = kShare ? (NT >= 4 ? 4 : 2) : TPW;
+  static_assert(TW * GW * 4 == TB * NT, "the waves cover the block");
+  auto tile_of = [&](int i) { return kShare ? (NT >= 4 ? i : (wave >> 1) * 2 + i) : wave * TPW + i; };
+  auto group_of = [&](int j) { return kShare ? (NT >= 4 ? wave * GW + j : (wave & 1)) : j; };
+  floatx4 acc[TW][GW];
 #pragma unroll
-  for (int tl = 0; tl < TPW; ++tl)
+  for (int tl = 0; tl < TW; ++tl)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) acc[tl][t] = floatx4{0, 0, 0, 0};
+    for (int t = 0; t < GW; ++t) acc[tl][t] = floatx4{0, 0, 0, 0};
   for (int ks = 0; ks < KS; ++ks) {
     const int k0 = ks * 32 + kg;
-    bf16x8 Bt[NT];
+    bf16x8 Bt[GW];
 #pragma unroll
-    for (int t = 0; t < NT; ++t)
-      Bt[t] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)(t * 16 + m) * Kp + k0));
+    for (int t = 0; t < GW; ++t)
+      Bt[t] = __builtin_bit_cast(bf16x8,
+                                 *reinterpret_cast<const u16x8*>(W + (int64_t)(group_of(t) * 16 + m) * Kp + k0));
     int hoff = -1;  // LDS offset of this lane's tap relative to its voxel, or -1 (zero A)
     if (k0 < K) {
       const int tap = k0 / CIN, ci = k0 - tap * CIN;
@@ -301,27 +316,106 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
       hoff = ((dx * HY + dy) * HZ + dz) * CIN + ci;
     }
 #pragma unroll
-    for (int tl = 0; tl < TPW; ++tl) {
-      const int off = (wave * TPW + tl) * 16 + m;  // this lane's voxel in the block
+    for (int tl = 0; tl < TW; ++tl) {
+      const int off = tile_of(tl) * 16 + m;  // this lane's voxel in the block
       const int yy = off / ZL, z = off - yy * ZL;
       u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
       if (hoff >= 0) raw = *reinterpret_cast<const u16x8*>(halo + hoff + (yy * HZ + z) * CIN);
 #pragma unroll
-      for (int t = 0; t < NT; ++t)
+      for (int t = 0; t < GW; ++t)
         acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), Bt[t], acc[tl][t], 0,
                                                             0, 0);
     }
   }
   const int64_t vblock = (int64_t)blockIdx.x * (YB * ZL);
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + m;
+  for (int t = 0; t < GW; ++t) {
+    const int co = group_of(t) * 16 + m;
     const float b = bias ? bias[co] : 0.0f;
 #pragma unroll
-    for (int tl = 0; tl < TPW; ++tl) {
-      const int64_t v0 = vblock + (wave * TPW + tl) * 16;
+    for (int tl = 0; tl < TW; ++tl) {
+      const int64_t v0 = vblock + tile_of(tl) * 16;
 #pragma unroll
       for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * COUT + co] = acc[tl][t][j] + b;
+    }
+  }
+}
+
+// Split-K gather convolution for the coarse levels (few voxels, 64 or 128 input channels): there a block per
+// 64 voxels leaves most of the GPU idle while each of its waves walks 27 Cin / 32 K steps, one dependent
+// round of loads after another.  Here a block computes 16 voxels x NT * 16 output channels (channel group
+// blockIdx.y); its four waves take every fourth K step (wave w: steps w, w + 4, ...), with the operands of
+// the next step loaded during the current step's MFMAs, and the four partial sums are added in wave order
+// through LDS.  Operands as in k_conv_fwd.
+template <int NT>
+__global__ __launch_bounds__(256) void k_conv_fwd_sk(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
+                                                     const float* __restrict__ bias, float* __restrict__ out, Geo g,
+                                                     int Cin, int Cout, int Kp) {
+  __shared__ floatx4 red[3][NT][64];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int m = lane & 15, kg = (lane >> 4) * 8;
+  const int64_t V = g.V();
+  const int64_t v0 = (int64_t)blockIdx.x * 16;
+  const int co0 = (int)blockIdx.y * NT * 16;
+  const int64_t va = v0 + m;
+  int n = 0, x = 0, y = 0, z = 0;
+  const bool va_ok = va < V;
+  if (va_ok) decompose(g, va, n, x, y, z);
+  const int K = 27 * Cin, KS = Kp / 32;
+  auto load_a = [&](int ks) {
+    const int k0 = ks * 32 + kg;
+    u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    if (va_ok && k0 < K) {
+      const int tap = k0 / Cin, ci = k0 - tap * Cin;
+      const int64_t nb = tap_voxel(g, n, x, y, z, tap);
+      if (nb >= 0) raw = *reinterpret_cast<const u16x8*>(in + nb * Cin + ci);
+    }
+    return raw;
+  };
+  const __bf16* wrow = W + (int64_t)(co0 + m) * Kp + kg;
+  floatx4 acc[NT];
+#pragma unroll
+  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0, 0, 0, 0};
+  int ks = wave;
+  u16x8 a = u16x8{0, 0, 0, 0, 0, 0, 0, 0}, b[NT];
+  if (ks < KS) {
+    a = load_a(ks);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const u16x8*>(wrow + (int64_t)t * 16 * Kp + ks * 32);
+  }
+  for (; ks < KS; ks += 4) {
+    const u16x8 ac = a;
+    u16x8 bc[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bc[t] = b[t];
+    if (ks + 4 < KS) {  // the next step's operands, in flight during this step's MFMAs
+      a = load_a(ks + 4);
+#pragma unroll
+      for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const u16x8*>(wrow + (int64_t)t * 16 * Kp + (ks + 4) * 32);
+    }
+#pragma unroll
+    for (int t = 0; t < NT; ++t)
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ac), __builtin_bit_cast(bf16x8, bc[t]),
+                                                      acc[t], 0, 0, 0);
+  }
+  if (wave > 0) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) red[wave - 1][t][lane] = acc[t];
+  }
+  __syncthreads();
+  if (wave != 0) return;
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    floatx4 s = acc[t];
+    s += red[0][t][lane];
+    s += red[1][t][lane];
+    s += red[2][t][lane];
+    const int co = co0 + t * 16 + m;
+    const float bb = bias ? bias[co] : 0.0f;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int64_t v = v0 + (lane >> 4) * 4 + j;
+      if (v < V) out[v * Cout + co] = s[j] + bb;
     }
   }
 }
@@ -469,18 +563,21 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
   // tile's are issued before this tile's MFMAs; the LDS contents, and so the MFMA operands, are those
   // of element-by-element staging.
   //
-  // Wide staging (ZZ a multiple of 8): a staging element is 8 consecutive z of one column and 8
-  // channels.  Its loads are 8 voxels of 16 bytes (10 for the input: z - 1 .. z + 8, zeros outside the
-  // column), transposed in registers so that every LDS store is 16 bytes of one channel row: 8 stores for
-  // a gradient element and 24 for an input element (the three z-shifted copies), where the voxel-wise
-  // staging below stores 2 bytes at a time (64 and 192 stores for the same data).  Same LDS contents.
-  // (16 -> 16 channels at Z = 32 keeps the voxel-wise staging: the wide one needs more registers there and
-  // drops the kernel from 3 to 2 waves per SIMD, 466 -> 566 us per call at batch 8 on 152 x 128 x 32.)
-  constexpr bool kWide = ZZ % 8 == 0 && !(CIT == 1 && COT == 1 && ZZ == 32);
-  constexpr int ZO = kWide ? ZZ / 8 : 1;                  // z-octets per column
+  // Wide staging (ZZ a multiple of 8, or 4): a staging element is ZW = min(ZZ, 8) consecutive z of one
+  // column and 8 channels.  Its loads are ZW voxels of 16 bytes (ZW + 2 for the input: z - 1 .. z + ZW,
+  // zeros outside the column), transposed in registers so that every LDS store is 2 ZW bytes of one channel
+  // row: 8 stores for a gradient element and 24 for an input element (the three z-shifted copies), where
+  // the voxel-wise staging below stores 2 bytes at a time (8 ZW and 24 ZW stores for the same data).  Same
+  // LDS contents.  (16 -> 16 channels at Z = 32 keeps the voxel-wise staging: the wide one needs more
+  // registers there and drops the kernel from 3 to 2 waves per SIMD, 466 -> 566 us per call at batch 8 on
+  // 152 x 128 x 32.)
+  constexpr int ZW = ZZ >= 8 ? 8 : ZZ;
+  constexpr bool kWide = (ZZ % 8 == 0 || ZZ == 4) && !(CIT == 1 && COT == 1 && ZZ == 32);
+  using zvec = typename std::conditional<ZW == 8, u16x8, u16x4>::type;
+  constexpr int ZO = kWide ? ZZ / ZW : 1;                  // z-runs per column
   constexpr int WEG = OC * ZO * (Cout / 8), WEX = SC * ZO * (Cin / 8);
   constexpr int WNG = kWide ? (WEG + 255) / 256 : 0, WNX = kWide ? (WEX + 255) / 256 : 0;
-  u16x8 wg[WNG > 0 ? WNG : 1][8], wx[WNX > 0 ? WNX : 1][10];
+  u16x8 wg[WNG > 0 ? WNG : 1][ZW], wx[WNX > 0 ? WNX : 1][ZW + 2];
   auto fetch_wide = [&](int64_t tile) {
     const int n = (int)(tile / ((int64_t)ntx * nty));
     const int r = (int)(tile - (int64_t)n * ntx * nty);
@@ -492,14 +589,14 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
       const int zo = ro % ZO, oc = ro / ZO;
       const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
       const bool ok = e < WEG && x < g.X && y < g.Y;
-      const int64_t v0 = ok ? (((int64_t)n * g.X + x) * g.Y + y) * ZZ + zo * 8 : 0;
+      const int64_t v0 = ok ? (((int64_t)n * g.X + x) * g.Y + y) * ZZ + zo * ZW : 0;
       const __bf16* src = gout + v0 * CoutT + co0 + cg * 8;
 #pragma unroll
-      for (int j = 0; j < 8; ++j)
+      for (int j = 0; j < ZW; ++j)
         wg[q][j] = ok ? *reinterpret_cast<const u16x8*>(src + (int64_t)j * CoutT) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
     }
 #pragma unroll
-    for (int q = 0; q < WNX; ++q) {  // input of the source columns at x + dx, z - 1 .. z + 8 of each octet
+    for (int q = 0; q < WNX; ++q) {  // input of the source columns at x + dx, z - 1 .. z + ZW of each run
       const int e = threadIdx.x + q * 256;
       const int cg = e % (Cin / 8), ro = e / (Cin / 8);
       const int zo = ro % ZO, sc = ro / ZO;
@@ -507,8 +604,8 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
       const bool ok = e < WEX && x >= 0 && x < g.X && y >= 0 && y < g.Y;
       const int64_t col = ok ? (((int64_t)n * g.X + x) * g.Y + y) * ZZ : 0;
 #pragma unroll
-      for (int j = 0; j < 10; ++j) {
-        const int z = zo * 8 - 1 + j;
+      for (int j = 0; j < ZW + 2; ++j) {
+        const int z = zo * ZW - 1 + j;
         wx[q][j] = (ok && z >= 0 && z < ZZ) ? *reinterpret_cast<const u16x8*>(in + (col + z) * Cin + cg * 8)
                                             : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
       }
@@ -523,10 +620,10 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
         const int zo = ro % ZO, oc = ro / ZO;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-          u16x8 v;
+          zvec v;
 #pragma unroll
-          for (int j = 0; j < 8; ++j) v[j] = wg[q][j][c];
-          *reinterpret_cast<u16x8*>(&sg[cg * 8 + c][oc * ZZ + zo * 8]) = v;
+          for (int j = 0; j < ZW; ++j) v[j] = wg[q][j][c];
+          *reinterpret_cast<zvec*>(&sg[cg * 8 + c][oc * ZZ + zo * ZW]) = v;
         }
       }
     }
@@ -536,19 +633,19 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
       if (e < WEX) {
         const int cg = e % (Cin / 8), ro = e / (Cin / 8);
         const int zo = ro % ZO, sc = ro / ZO;
-        const int o = sc * ZZ + zo * 8;
+        const int o = sc * ZZ + zo * ZW;
 #pragma unroll
         for (int c = 0; c < 8; ++c) {
-          u16x8 lo, mid, hi;  // sx[dz + 1][ci][col][z] = in[col][z + dz]
+          zvec lo, mid, hi;  // sx[dz + 1][ci][col][z] = in[col][z + dz]
 #pragma unroll
-          for (int j = 0; j < 8; ++j) {
+          for (int j = 0; j < ZW; ++j) {
             lo[j] = wx[q][j][c];
             mid[j] = wx[q][j + 1][c];
             hi[j] = wx[q][j + 2][c];
           }
-          *reinterpret_cast<u16x8*>(&sx[0][cg * 8 + c][o]) = lo;
-          *reinterpret_cast<u16x8*>(&sx[1][cg * 8 + c][o]) = mid;
-          *reinterpret_cast<u16x8*>(&sx[2][cg * 8 + c][o]) = hi;
+          *reinterpret_cast<zvec*>(&sx[0][cg * 8 + c][o]) = lo;
+          *reinterpret_cast<zvec*>(&sx[1][cg * 8 + c][o]) = mid;
+          *reinterpret_cast<zvec*>(&sx[2][cg * 8 + c][o]) = hi;
         }
       }
     }
@@ -755,6 +852,7 @@ __global__ void k_bias_final(const float* __restrict__ part, float* __restrict__
 
 // ------------------------------------------------------------ elementwise ---
 
+
 __device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
 __device__ __forceinline__ float dsilu(float x) {
   const float s = 1.0f / (1.0f + expf(-x));
@@ -835,8 +933,6 @@ __global__ void k_up_fwd(const __bf16* __restrict__ in, __bf16* __restrict__ out
 // fp32 or 8 bf16 values, or per voxel and 8 (bf16) / 4 (fp32) channels.  The
 // host uses them when the element count fits 32-bit indices and the pointers
 // are 16-byte aligned; the scalar kernels remain for everything else.
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ void dec32(uint32_t v, int X, int Y, int Z, int& n, int& x, int& y, int& z) {
   z = (int)(v % (uint32_t)Z);
@@ -1009,6 +1105,36 @@ __global__ void k_mse(const float* __restrict__ pred, const float* __restrict__ 
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// k_mse with the gradient written in BF16 (the operand of the head's backward, so no FP32 copy is written and
+// converted) and the normaliser taken from device memory: inv_norm = 1 / max(*wsum, 1).
+__global__ void k_mse_bf16(const float* __restrict__ pred, const float* __restrict__ target,
+                           const float* __restrict__ wmask, const float* __restrict__ wsum, __bf16* __restrict__ grad,
+                           float* __restrict__ part, int64_t V, int C, int nout) {
+  __shared__ float red[256];
+  const float inv_norm = 1.0f / fmaxf(*wsum, 1.0f);
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float l = 0.0f;
+  if (i < V * C) {
+    const int64_t v = i / C;
+    const int c = (int)(i - v * C);
+    float gr = 0.0f;
+    if (c < nout) {
+      const float w = wmask[v * nout + c];
+      const float d = pred[i] - target[v * nout + c];
+      l = w * d * d * inv_norm;
+      gr = 2.0f * w * d * inv_norm;
+    }
+    grad[i] = f2bf(gr);
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy.
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, __bf16* __restrict__ wb, int64_t n, float lr, float b1, float b2,
@@ -1109,6 +1235,17 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
         hipLaunchKernelGGL((k_conv_fwd_lds<64, 8, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
+  }
+  if (Cin >= 64 && (Z <= 4 || Cin == 128)) {
+    // the coarsest level (Z <= 4) and 128 input channels: split-K over the waves of a block
+    const unsigned gv = (unsigned)((V + 15) / 16);
+    if (Cout % 64 == 0)
+      hipLaunchKernelGGL(k_conv_fwd_sk<4>, dim3(gv, Cout / 64), dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp);
+    else if (Cout % 32 == 0)
+      hipLaunchKernelGGL(k_conv_fwd_sk<2>, dim3(gv, Cout / 32), dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp);
+    else
+      hipLaunchKernelGGL(k_conv_fwd_sk<1>, dim3(gv, Cout / 16), dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
   }
   if (Cout == 16 && (Cin == 16 || Cin == 32) && Kp == (27 * Cin + 31) / 32 * 32) {
     const int tiles = 8;
@@ -1322,20 +1459,131 @@ __global__ void k_dense_bwd_w(const float* __restrict__ x, const float* __restri
     db[o] = a;
 }
 
-// gx[n][i] = sum_o W[o][i] g[n][o] (o in order).
-__global__ void k_dense_bwd_x(const float* __restrict__ W, const float* __restrict__ gy,
-                              const float* __restrict__ pre, float* __restrict__ gx, int N, int I, int O, int act) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= N * I) return;
-  const int n = t / I, i = t % I;
+// The sum of a over the 64 lanes of a wave, every lane holding the result: a butterfly of pairwise adds
+// (the same pairs in every call, so the same bits).
+__device__ __forceinline__ float wave_sum(float a) {
+#pragma unroll
+  for (int m = 32; m > 0; m >>= 1) a += __shfl_xor(a, m, 64);
+  return a;
+}
+
+// gx[n][i] = sum_o W[o][i] g[n][o]: one wave per (n, i); lane l sums o = l, l + 64, ... in order, then
+// wave_sum.  (A thread walking all O outputs alone waits on O dependent loads in turn.)
+__global__ __launch_bounds__(256) void k_dense_bwd_x(const float* __restrict__ W, const float* __restrict__ gy,
+                                                     const float* __restrict__ pre, float* __restrict__ gx, int N,
+                                                     int I, int O, int act) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= N * I) return;  // the whole wave
+  const int n = w / I, i = w % I;
   float a = 0.0f;
-  for (int o = 0; o < O; ++o) {
+  for (int o = lane; o < O; o += 64) {
     float g = gy[(int64_t)n * O + o];
     if (act) g *= dsilu(pre[(int64_t)n * O + o]);
     a += W[(int64_t)o * I + i] * g;
   }
-  gx[t] = a;
+  a = wave_sum(a);
+  if (lane == 0) gx[w] = a;
 }
+
+// ------------------------------------------------- activation backward ---
+// One activation site's backward in one pass, with the per-channel sums its parameters need:
+//   pm = p (1 + gam) + bet (FILM) or p,  gm = gy * silu'(pm),  gx = bf16(gm (1 + gam)) or bf16(gm)
+// (gx: the gradient w.r.t. the convolution output, operand of the next MFMAs; elementwise the bits of
+// k_film_silu_bwd_part / k_silu_bwd), and per (sample, block) partial sums over the block's voxels of
+// gm * p and gm (FILM; -> dgam, dbet) and of gx as BF16 (-> the convolution's bias gradient).
+// Thread t: channels 4 (t % C4) .. 4 (t % C4) + 3 as 16-byte vectors, voxel rows t / C4 + k R of the block's
+// range (R = 256 / C4, C4 = C / 4 a power of two); the block then folds its rows pairwise in LDS (a fixed
+// tree).  part[(n * nbn + b) * Q * C + q * C + c]: q = 0 sum gm p, 1 sum gm, 2 sum gx (FILM); q = 0 sum gx.
+template <bool FILM>
+__global__ __launch_bounds__(256) void k_act_bwd_part(const float* __restrict__ p, const float* __restrict__ gy,
+                                                      const float* __restrict__ film, __bf16* __restrict__ gxb,
+                                                      float* __restrict__ part, int Vn, int C, int off, int L,
+                                                      int vper) {
+  constexpr int Q = FILM ? 3 : 1;
+  __shared__ f32x4 red[Q][256];
+  const int n = blockIdx.y, b = blockIdx.x, nbn = gridDim.x;
+  const int C4 = C >> 2, R = 256 / C4;
+  const int cq = threadIdx.x % C4, r = threadIdx.x / C4;
+  f32x4 g1 = f32x4{1.0f, 1.0f, 1.0f, 1.0f}, bt = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  if constexpr (FILM) {
+    const float* f = film + (int64_t)n * L + off;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      g1[k] = 1.0f + f[4 * cq + k];
+      bt[k] = f[C + 4 * cq + k];
+    }
+  }
+  f32x4 sgp = f32x4{0.0f, 0.0f, 0.0f, 0.0f}, sg = sgp, sb = sgp;
+  const int v0 = b * vper;
+  const int v1 = v0 + vper < Vn ? v0 + vper : Vn;
+  const int64_t base = (int64_t)n * Vn * C4 + cq;
+  const f32x4* p4 = reinterpret_cast<const f32x4*>(p) + base;
+  const f32x4* g4 = reinterpret_cast<const f32x4*>(gy) + base;
+  u16x4* o4 = reinterpret_cast<u16x4*>(gxb) + base;
+#pragma unroll 4
+  for (int v = v0 + r; v < v1; v += R) {
+    const f32x4 pv = p4[(int64_t)v * C4];
+    const f32x4 gv = g4[(int64_t)v * C4];
+    u16x4 o;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      float g;
+      __bf16 h;
+      if constexpr (FILM) {
+        g = gv[k] * dsilu(pv[k] * g1[k] + bt[k]);
+        sgp[k] += g * pv[k];
+        sg[k] += g;
+        h = f2bf(g * g1[k]);
+      } else {
+        g = gv[k] * dsilu(pv[k]);
+        h = f2bf(g);
+      }
+      sb[k] += bf2f(h);
+      o[k] = __builtin_bit_cast(unsigned short, h);
+    }
+    o4[(int64_t)v * C4] = o;
+  }
+  if constexpr (FILM) {
+    red[0][threadIdx.x] = sgp;
+    red[1][threadIdx.x] = sg;
+  }
+  red[Q - 1][threadIdx.x] = sb;
+  __syncthreads();
+  for (int s = R / 2; s > 0; s >>= 1) {
+    if (r < s) {
+#pragma unroll
+      for (int q = 0; q < Q; ++q) red[q][threadIdx.x] += red[q][threadIdx.x + s * C4];
+    }
+    __syncthreads();
+  }
+  if (r == 0) {
+    float* o = part + ((int64_t)n * nbn + b) * (Q * C);
+#pragma unroll
+    for (int q = 0; q < Q; ++q)
+#pragma unroll
+      for (int k = 0; k < 4; ++k) o[q * C + 4 * cq + k] = red[q][cq][k];
+  }
+}
+
+// out[gi * ostride + ooff + j] = sum over the rows q of group gi (rows gi * rpg .. gi * rpg + rpg - 1) of
+// part[q * rowlen + col0 + j], j < ncols: one wave per (gi, j), lane l summing rows l, l + 64, ... in order,
+// then wave_sum.
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ part, float* __restrict__ out, int ngroups,
+                                                int rpg, int rowlen, int col0, int ncols, int ostride, int ooff) {
+  const int w = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (w >= ngroups * ncols) return;  // the whole wave
+  const int gi = w / ncols, j = w % ncols;
+  const float* src = part + (int64_t)gi * rpg * rowlen + col0 + j;
+  float a = 0.0f;
+#pragma unroll 4
+  for (int q = lane; q < rpg; q += 64) a += src[(int64_t)q * rowlen];
+  a = wave_sum(a);
+  if (lane == 0) out[(int64_t)gi * ostride + ooff + j] = a;
+}
+
+// Voxel rows per block of k_act_bwd_part: 16 per thread.
+inline int act_vper(int C) { return 16 * (256 / (C / 4)); }
+inline int act_blocks(int64_t Vn, int C) { return (int)((Vn + act_vper(C) - 1) / act_vper(C)); }
 
 int film_blocks(int64_t Vn) {
   int64_t nb = (Vn + 4095) / 4096;
@@ -1534,6 +1782,13 @@ int xnn_mse(const float* pred, const float* target, const float* wmask, float* g
                      V, C, nout, inv_norm);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
+int xnn_mse_bf16(const float* pred, const float* target, const float* wmask, const float* wsum, void* grad,
+                 float* part, int64_t V, int C, int nout, void* stream) {
+  if (V < 1 || C < 1 || nout < 1 || nout > C) return 1;
+  hipLaunchKernelGGL(k_mse_bf16, dim3(nblk(V * C)), dim3(256), 0, (hipStream_t)stream, pred, target, wmask, wsum,
+                     (__bf16*)grad, part, V, C, nout);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
 int xnn_mse_blocks(int64_t V, int C) { return (int)nblk(V * C); }
 int xnn_adam(float* w, const float* g, float* m, float* v, void* wb, int64_t n, float lr, float b1, float b2,
              float eps, int step, float wd, void* stream) {
@@ -1587,6 +1842,36 @@ int xnn_film_bwd(const float* p, const float* gy, const float* film, float* gx, 
                      L);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
+int64_t xnn_act_part_floats(int N, int64_t Vn, int C, int film) {
+  if (N < 1 || Vn < 1 || C < 4 || C % 4 || C > 256) return 0;
+  return (int64_t)N * act_blocks(Vn, C) * (film ? 3 : 1) * C;
+}
+int xnn_act_bwd(const float* p, const float* gy, const float* film, void* gxb, float* part, float* dfilm, float* db,
+                int N, int64_t Vn, int C, int off, int L, void* stream) {
+  const int C4 = C / 4;
+  if (N < 1 || N > 65535 || Vn < 1 || Vn >= (int64_t(1) << 31) || C < 4 || C % 4 || C > 256 || (C4 & (C4 - 1)))
+    return 1;
+  if (film && (dfilm == nullptr || off < 0 || off + 2 * C > L)) return 1;
+  if (reinterpret_cast<uintptr_t>(p) % 16 || reinterpret_cast<uintptr_t>(gy) % 16 ||
+      reinterpret_cast<uintptr_t>(gxb) % 8)
+    return 1;
+  const int nbn = act_blocks(Vn, C), vper = act_vper(C), Q = film ? 3 : 1;
+  if ((int64_t)N * nbn >= (int64_t(1) << 31) / (Q * C)) return 1;
+  hipStream_t s = (hipStream_t)stream;
+  if (film)
+    hipLaunchKernelGGL(k_act_bwd_part<true>, dim3(nbn, N), dim3(256), 0, s, p, gy, film, (__bf16*)gxb, part,
+                       (int)Vn, C, off, L, vper);
+  else
+    hipLaunchKernelGGL(k_act_bwd_part<false>, dim3(nbn, N), dim3(256), 0, s, p, gy, film, (__bf16*)gxb, part,
+                       (int)Vn, C, off, L, vper);
+  if (film)
+    hipLaunchKernelGGL(k_colsum, dim3(nblk((int64_t)N * 2 * C, 4)), dim3(256), 0, s, part, dfilm, N, nbn, Q * C, 0,
+                       2 * C, L, off);
+  if (db)
+    hipLaunchKernelGGL(k_colsum, dim3(nblk(C, 4)), dim3(256), 0, s, part, db, 1, N * nbn, Q * C, (Q - 1) * C, C, 0,
+                       0);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
 int xnn_dense_fwd(const float* x, const float* W, const float* b, float* pre, float* post, int N, int I, int O,
                   int act, void* stream) {
   if (N < 1 || I < 1 || O < 1) return 1;
@@ -1601,7 +1886,7 @@ int xnn_dense_bwd(const float* x, const float* W, const float* pre, const float*
   hipLaunchKernelGGL(k_dense_bwd_w, dim3(nblk((int64_t)O * (I + 1))), dim3(256), 0, s, x, gy, pre, dW, db, N, I, O,
                      act);
   if (gx)
-    hipLaunchKernelGGL(k_dense_bwd_x, dim3(nblk((int64_t)N * I)), dim3(256), 0, s, W, gy, pre, gx, N, I, O, act);
+    hipLaunchKernelGGL(k_dense_bwd_x, dim3(nblk((int64_t)N * I, 4)), dim3(256), 0, s, W, gy, pre, gx, N, I, O, act);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 

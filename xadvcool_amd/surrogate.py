@@ -79,6 +79,10 @@ def lib():
         L.xnn_film_part_floats.restype = i64
         L.xnn_dense_fwd.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, vp]
         L.xnn_dense_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i, i, i, vp]
+        L.xnn_act_bwd.argtypes = [vp, vp, vp, vp, vp, vp, vp, i, i64, i, i, i, vp]
+        L.xnn_act_part_floats.argtypes = [i, i64, i, i]
+        L.xnn_act_part_floats.restype = i64
+        L.xnn_mse_bf16.argtypes = [vp, vp, vp, vp, vp, vp, i64, i, i, vp]
         _lib = L
     return _lib
 
@@ -177,7 +181,59 @@ def film_silu_bwd(p, gy, film, dfilm, off):
     return gx
 
 
-class Dense:
+def act_bwd(p, gy, film=None, dfilm=None, off=0, want_db=True):
+    """Backward of an activation site (SiLU, or FiLM then SiLU with ``film``) from gy (FP32, w.r.t. its output)
+    in one pass: returns (BF16 gradient w.r.t. p, bias gradient of the convolution that produced p, or None);
+    writes dfilm[:, off:off+2C] when ``film`` is given.  Elementwise the bits of silu_bwd / film_silu_bwd."""
+    N, C = p.shape[0], p.shape[-1]
+    Vn = p.numel() // (N * C)
+    L = lib()
+    part = torch.empty(L.xnn_act_part_floats(N, Vn, C, int(film is not None)), dtype=torch.float32, device=p.device)
+    gx = torch.empty(p.shape, dtype=torch.bfloat16, device=p.device)
+    db = torch.empty(C, dtype=torch.float32, device=p.device) if want_db else None
+    Lf = film.shape[1] if film is not None else 0
+    _ok(L.xnn_act_bwd(_p(p), _p(gy), _p(film), _p(gx), _p(part), _p(dfilm), _p(db), N, Vn, C, off, Lf, _st()),
+        "xnn_act_bwd")
+    return gx, db
+
+
+class _Grads:
+    """Gradient accumulators dw, db of a layer with parameters w, b.  zero_grad() only marks them cleared: the
+    next accumulation takes the new gradients as they are (no fill kernel, no add kernel), and reading dw or db
+    while cleared zeroes them first."""
+
+    def _init_grads(self):
+        self._dw, self._db = torch.zeros_like(self.w), torch.zeros_like(self.b)
+        self._cleared = True
+
+    @property
+    def dw(self):
+        self._materialize()
+        return self._dw
+
+    @property
+    def db(self):
+        self._materialize()
+        return self._db
+
+    def _materialize(self):
+        if self._cleared:
+            self._dw.zero_()
+            self._db.zero_()
+            self._cleared = False
+
+    def _accumulate(self, dw, db):
+        if self._cleared:  # fresh tensors of the caller's: they become the accumulators
+            self._dw, self._db, self._cleared = dw, db, False
+        else:
+            self._dw.add_(dw)
+            self._db.add_(db)
+
+    def zero_grad(self):
+        self._cleared = True
+
+
+class Dense(_Grads):
     """Fully connected FP32 layer (W: O x I) with Adam moments; ``zero`` starts it at zero."""
 
     def __init__(self, fin, fout, gen, act, zero=False, device="cuda"):
@@ -187,7 +243,7 @@ class Dense:
         self.b = torch.zeros(fout, dtype=torch.float32, device=device)
         self.m_w, self.v_w = torch.zeros_like(self.w), torch.zeros_like(self.w)
         self.m_b, self.v_b = torch.zeros_like(self.b), torch.zeros_like(self.b)
-        self.dw, self.db = torch.zeros_like(self.w), torch.zeros_like(self.b)
+        self._init_grads()
 
     def fwd(self, x):
         N = x.shape[0]
@@ -203,13 +259,8 @@ class Dense:
         gx = torch.empty_like(x) if want_gx else None
         _ok(lib().xnn_dense_bwd(_p(x), _p(self.w), _p(pre), _p(gy), _p(dw), _p(db), _p(gx), N, self.fin, self.fout,
                                 self.act, _st()), "xnn_dense_bwd")
-        self.dw.add_(dw)
-        self.db.add_(db)
+        self._accumulate(dw, db)
         return gx
-
-    def zero_grad(self):
-        self.dw.zero_()
-        self.db.zero_()
 
     def adam(self, step, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
         L = lib()
@@ -270,7 +321,7 @@ def up_bwd(gf, shape_c):
     return g
 
 
-class Conv:
+class Conv(_Grads):
     """3x3x3 convolution: FP32 master weights W[cout][Kp] (k = tap*cin + ci), BF16 copy, Adam moments."""
 
     def __init__(self, cin, cout, gen, gain=1.0, device="cuda"):
@@ -285,7 +336,7 @@ class Conv:
         self.wb = self.w.to(torch.bfloat16)
         self.m_w, self.v_w = torch.zeros_like(self.w), torch.zeros_like(self.w)
         self.m_b, self.v_b = torch.zeros_like(self.b), torch.zeros_like(self.b)
-        self.dw, self.db = torch.zeros_like(self.w), torch.zeros_like(self.b)
+        self._init_grads()
         self.wt = torch.zeros(cin, self.Kpt, dtype=torch.bfloat16, device=device)
         self._parts = {}
 
@@ -298,8 +349,9 @@ class Conv:
         _ok(L.xnn_wflip(_p(self.wb), _p(self.wt), self.cin, self.cout, self.Kp, self.Kpt, _st()), "xnn_wflip")
         return conv_fwd(g_bf16, self.wt, None, shape, self.cout, self.cin, self.Kpt)
 
-    def wgrad(self, x, g_bf16, shape, vchunk=4096):
-        """Accumulate dW, db from the input x (bf16) and upstream gradient g (bf16)."""
+    def wgrad(self, x, g_bf16, shape, vchunk=4096, db=None):
+        """Accumulate dW, db from the input x (bf16) and upstream gradient g (bf16); ``db`` (the bias gradient,
+        when the activation backward already summed it) skips the bias reduction."""
         L = lib()
         N, X, Y, Z = shape
         nw = L.xnn_wgrad_part_floats(N, X, Y, Z, self.cin, self.cout, self.Kp, vchunk)
@@ -314,14 +366,10 @@ class Conv:
         dw = torch.empty_like(self.w)
         _ok(L.xnn_conv_wgrad(_p(x), _p(g_bf16), _p(part), _p(dw), N, X, Y, Z, self.cin, self.cout, self.Kp, vchunk,
                              _st()), "xnn_conv_wgrad")
-        db = torch.empty_like(self.b)
-        _ok(L.xnn_bias_grad(_p(g_bf16), _p(db), _p(bpart), N * X * Y * Z, self.cout, _st()), "xnn_bias_grad")
-        self.dw.add_(dw)
-        self.db.add_(db)
-
-    def zero_grad(self):
-        self.dw.zero_()
-        self.db.zero_()
+        if db is None:
+            db = torch.empty_like(self.b)
+            _ok(L.xnn_bias_grad(_p(g_bf16), _p(db), _p(bpart), N * X * Y * Z, self.cout, _st()), "xnn_bias_grad")
+        self._accumulate(dw, db)
 
     def adam(self, step, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
         L = lib()
@@ -441,16 +489,17 @@ class UNet3D:
         off = {name: o for name, o, _ in self.FILM}
         dfilm = torch.empty_like(t["film"])
 
-        def lin(name, x_in, g_out_bf16, shape):  # conv backward: wgrad + dgrad
-            cv[name].wgrad(x_in, g_out_bf16, shape)
-            return cv[name].dgrad(g_out_bf16, shape)
+        def lin(name, x_in, g_out, shape):  # conv backward: wgrad (+ bias) + dgrad
+            g_bf16, db = g_out if isinstance(g_out, tuple) else (g_out, None)
+            cv[name].wgrad(x_in, g_bf16, shape, db=db)
+            return cv[name].dgrad(g_bf16, shape)
 
-        def dact(name, g):  # gradient w.r.t. the conv output p (bf16) from the gradient w.r.t. SiLU's output
+        def dact(name, g):  # from the gradient w.r.t. SiLU's output: (bf16 one w.r.t. the conv output p, bias grad)
             if name not in off:
-                return silu_bwd(t[name], g)
-            return film_silu_bwd(t[name], g, t["film"], dfilm, off[name])
+                return act_bwd(t[name], g)
+            return act_bwd(t[name], g, t["film"], dfilm, off[name])
 
-        g = to_bf16(gout)
+        g = gout if gout.dtype == torch.bfloat16 else to_bf16(gout)
         g_ad1b = lin("head", t["ad1b"], g, sh[0])
         g_k1 = lin("d1b", t["k1"], dact("pd1b", g_ad1b), sh[0])
         g_s1 = g_k1
@@ -476,7 +525,8 @@ class UNet3D:
         g_q1 = lin("c2a", t["q1"], dact("p2a", g_a2a), sh[1])
         g_s1 = add_f32(g_s1, pool_bwd(g_q1, sh[0]))
         g_a1a = lin("c1b", t["a1a"], dact("p1b", g_s1), sh[0])
-        cv["c1a"].wgrad(t["x"], dact("p1a", g_a1a), sh[0])
+        g1a, db1a = dact("p1a", g_a1a)
+        cv["c1a"].wgrad(t["x"], g1a, sh[0], db=db1a)
         # every FiLM row entry was written by its site's film_bwd; on through the MLP
         g_h1 = self.film2.bwd(t["h1"], t["film"], dfilm)
         self.film1.bwd(t["s"], t["h1pre"], g_h1, want_gx=False)
@@ -531,7 +581,8 @@ class UNet3D:
 
 
 def mse_loss(pred, target, wmask):
-    """Weighted MSE on the first NOUT channels; returns (loss (device scalar), grad (fp32, pred's shape))."""
+    """Weighted MSE on the first NOUT channels; returns (loss (device scalar), grad (bf16, pred's shape)), the
+    normaliser (the sum of wmask) kept on the device."""
     V = pred.numel() // pred.shape[-1]
     C = pred.shape[-1]
     # the kernel reads target and wmask as (V, NOUT): refuse anything else before it runs
@@ -543,9 +594,10 @@ def mse_loss(pred, target, wmask):
     L = lib()
     nb = L.xnn_mse_blocks(V, C)
     part = torch.empty(nb, dtype=torch.float32, device=pred.device)
-    grad = torch.empty_like(pred)
-    inv = 1.0 / max(float(wmask.sum().item()), 1.0)  # mean over weighted entries
-    _ok(L.xnn_mse(_p(pred), _p(target), _p(wmask), _p(grad), _p(part), V, C, NOUT, inv, _st()), "xnn_mse")
+    grad = torch.empty(pred.shape, dtype=torch.bfloat16, device=pred.device)
+    wsum = wmask.sum(dtype=torch.float32).reshape(1)  # mean over weighted entries
+    _ok(L.xnn_mse_bf16(_p(pred), _p(target), _p(wmask), _p(wsum), _p(grad), _p(part), V, C, NOUT, _st()),
+        "xnn_mse_bf16")
     return part.sum(), grad
 
 
