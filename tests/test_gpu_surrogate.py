@@ -363,10 +363,12 @@ def test_activation_backward_in_one_pass(gpu, C, shape):
 
 @pytest.mark.parametrize("cin,cout,shape", [(64, 128, (1, 3, 5, 3)), (128, 128, (2, 5, 4, 4)),
                                              (128, 64, (1, 3, 5, 8)), (128, 128, (8, 19, 16, 4)),
-                                             (128, 32, (1, 2, 3, 4)), (64, 16, (1, 2, 2, 2))])
+                                             (128, 32, (1, 2, 3, 4)), (64, 16, (1, 2, 2, 2)),
+                                             (128, 64, (4, 38, 32, 8))])
 def test_conv_split_k_coarse_levels(gpu, cin, cout, shape):
     """The split-K convolution of the coarsest level (Z <= 4) and of 128 input channels, forward and data
-    gradient, against PyTorch (rtol 1e-3), with partial 16-voxel tiles and the network's bottleneck at batch 8."""
+    gradient, against PyTorch (rtol 1e-3), with partial 16-voxel tiles, the network's bottleneck at batch 8
+    (two 16-voxel tiles per block) and a 128 -> 64 layer with four tiles per block."""
     from xadvcool_amd import surrogate as S
     print(f"[nn] split-K conv {cin}->{cout} {shape}", flush=True)
     gen = torch.Generator().manual_seed(cin + cout + sum(shape))

@@ -65,7 +65,7 @@ template <int NT>
 __global__ __launch_bounds__(256) void k_conv_fwd(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
                                                   const float* __restrict__ bias, float* __restrict__ out, Geo g,
                                                   int Cin, int Kp) {
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t V = g.V();
   const int64_t v0 = ((int64_t)blockIdx.x * 4 + wave) * 16;
   if (v0 >= V) return;
@@ -123,7 +123,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const __bf16* __restrict__ i
                                                     const float* __restrict__ bias, float* __restrict__ out, Geo g,
                                                     int Kp, int tiles) {
   constexpr int K = 27 * CIN, KS = (K + 31) / 32;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = lane & 15, kg = (lane >> 4) * 8;
   const int64_t V = g.V();
   const int64_t t0 = ((int64_t)blockIdx.x * 4 + wave) * tiles;
@@ -160,6 +160,37 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const __bf16* __restrict__ i
   }
 }
 
+// Loads a block's input halo into registers: 3 x HY x HZ voxels of CH * 8 BF16 channels around (n, x,
+// y0 .., z = 0 ..), voxel hv = (ix * HY + iy) * HZ + iz from (x + ix - 1, y0 + iy - 1, iz - 1), zero outside
+// the lattice; chunk e = threadIdx.x + 256 i is 16 bytes (part e % CH) of voxel e / CH.  The coordinates
+// of a thread's chunks are stepped from one to the next (256 / CH voxels apart) instead of divided out, and
+// every load is issued (a position outside the lattice reads the block's own voxel (x, y0, 0) and is
+// replaced by zeros), as a 32-bit offset from that voxel's address.
+template <int CH, int HY, int HZ, int PER>
+__device__ __forceinline__ void load_halo(const __bf16* __restrict__ in, u16x8 (&tmp)[PER], const Geo& g, int n,
+                                          int x, int y0) {
+  constexpr int NCH = 3 * HY * HZ * CH, DV = 256 / CH;
+  constexpr int DZ = DV % HZ, DY = (DV / HZ) % HY, DX = DV / (HY * HZ);
+  static_assert(256 % CH == 0, "chunks per voxel");
+  const int part = threadIdx.x % CH, hv = threadIdx.x / CH;
+  int ix = hv / (HY * HZ), iy = (hv / HZ) % HY, iz = hv % HZ;
+  const int sz = CH * 8, sy = g.Z * sz, sx = g.Y * sy;  // element strides of z, y and x
+  const __bf16* blk = in + (((int64_t)n * g.X + x) * g.Y + y0) * g.Z * sz;  // voxel (n, x, y0, 0)
+#pragma unroll
+  for (int i = 0; i < PER; ++i) {
+    const bool ok = ((int)threadIdx.x + i * 256 < NCH) & ((unsigned)(x + ix - 1) < (unsigned)g.X) &
+                    ((unsigned)(y0 + iy - 1) < (unsigned)g.Y) & ((unsigned)(iz - 1) < (unsigned)g.Z);
+    const int off = (ok ? (ix - 1) * sx + (iy - 1) * sy + (iz - 1) * sz : 0) + part * 8;
+    const u16x8 v = *reinterpret_cast<const u16x8*>(blk + off);
+    tmp[i] = ok ? v : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    iz += DZ;
+    iy += DY;
+    ix += DX;
+    if (iz >= HZ) iz -= HZ, ++iy;
+    if (iy >= HY) iy -= HY, ++ix;
+  }
+}
+
 // CIN -> 16 convolution for lattices with Z = 32 and Y a multiple of YB (the
 // full-resolution layers): a block owns the YB * 32 voxels of one (n, x) and
 // YB consecutive y columns, all z.  It stages their input halo (x - 1 .. x + 1,
@@ -183,7 +214,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
   static_assert(HV * CIN * 2 <= 65536, "halo must fit the static LDS limit");
   static_assert(YB % 2 == 0 && YB * 2 == 4 * TPW, "a block is 4 waves of whole tiles");
   __shared__ __attribute__((aligned(16))) __bf16 halo[HV * CIN];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = lane & 15, kg = (lane >> 4) * 8;
   const int ybl = g.Y / YB;  // blocks per (n, x)
   const int nx = blockIdx.x / ybl;
@@ -191,20 +222,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
   const int n = nx / g.X, x = nx - n * g.X;
   // Stage the halo: every load of this thread is issued before any store.
   u16x8 tmp[kPer];
-#pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const int e = threadIdx.x + i * 256;
-    tmp[i] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    if (e < kChunks) {
-      const int hv = e / CH, part = e - hv * CH;
-      const int ix = hv / (HY * HZ), r = hv - ix * (HY * HZ);
-      const int iy = r / HZ, iz = r - iy * HZ;
-      const int xx = x + ix - 1, yy = y0 + iy - 1, zz = iz - 1;
-      if (xx >= 0 && xx < g.X && yy >= 0 && yy < g.Y && zz >= 0 && zz < g.Z)
-        tmp[i] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + xx) * g.Y + yy) * g.Z + zz) * CIN +
-                                                 part * 8);
-    }
-  }
+  load_halo<CH, HY, HZ>(in, tmp, g, n, x, y0);
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int e = threadIdx.x + i * 256;
@@ -217,24 +235,53 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
   const float b = bias ? bias[m] : 0.0f;
   __syncthreads();
   const int64_t vblock = (int64_t)blockIdx.x * (YB * 32);
-  for (int t = 0; t < TPW; ++t) {
-    const int tt = wave * TPW + t;  // tile of the block: column tt / 2, z from (tt % 2) * 16
-    const int yy = tt >> 1, z0 = (tt & 1) * 16;
-    floatx4 acc = floatx4{0, 0, 0, 0};
+  // The wave's tiles are computed G at a time with independent accumulators, and the A operands of
+  // the next K-step are read from LDS before this step's MFMAs, so neither the LDS latency nor the
+  // accumulator dependency of one tile's MFMA chain stalls the wave.  Each tile still gets the MFMA
+  // sequence above (K-steps in order), so the outputs are the same bits.
+  constexpr int G = TPW < 4 ? TPW : 4;
+  static_assert(TPW % G == 0 && TPW % 2 == 0, "tile groups");
+  // LDS offset (elements) of this lane's A operand for K-step ks in the wave's first tile; the other tiles
+  // add a constant (tile t: column t / 2, z from (t % 2) * 16).  Lanes past K in the last step read zeros.
+  int lo[KS];
+#pragma unroll
+  for (int ks = 0; ks < KS; ++ks) {
+    const int k0 = ks * 32 + kg;
+    const int tap = k0 / CIN, ci = k0 - tap * CIN;
+    const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
+    lo[ks] = ((dx * HY + wave * (TPW / 2) + dy) * HZ + m + dz) * CIN + ci;
+  }
+  auto a_op = [&](int ks, int t) {
+    if (ks == KS - 1 && (KS - 1) * 32 + kg >= K) return u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    return *reinterpret_cast<const u16x8*>(halo + lo[ks] + ((t >> 1) * HZ + (t & 1) * 16) * CIN);
+  };
+#pragma unroll
+  for (int t0 = 0; t0 < TPW; t0 += G) {
+    floatx4 acc[G];
+    u16x8 raw[2][G];
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      acc[q] = floatx4{0, 0, 0, 0};
+      raw[0][q] = a_op(0, t0 + q);
+    }
 #pragma unroll
     for (int ks = 0; ks < KS; ++ks) {
-      const int k0 = ks * 32 + kg;
-      u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (k0 < K) {
-        const int tap = k0 / CIN, ci = k0 - tap * CIN;
-        const int dx = tap / 9, dy = (tap / 3) % 3, dz = tap % 3;  // offsets + 1
-        raw = *reinterpret_cast<const u16x8*>(halo + ((dx * HY + yy + dy) * HZ + z0 + m + dz) * CIN + ci);
-      }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), B[ks], acc, 0, 0, 0);
-    }
-    const int64_t v0 = vblock + tt * 16;
+      const int cur = ks & 1;
+      if (ks + 1 < KS) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * 16 + m] = acc[j] + b;
+        for (int q = 0; q < G; ++q) raw[cur ^ 1][q] = a_op(ks + 1, t0 + q);
+      }
+#pragma unroll
+      for (int q = 0; q < G; ++q)
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw[cur][q]), B[ks], acc[q],
+                                                         0, 0, 0);
+    }
+#pragma unroll
+    for (int q = 0; q < G; ++q) {
+      const int64_t v0 = vblock + (wave * TPW + t0 + q) * 16;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * 16 + m] = acc[q][j] + b;
+    }
   }
 }
 
@@ -260,27 +307,14 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
   static_assert(HV * CIN * 2 <= 65536, "halo must fit the static LDS limit");
   static_assert((ZL == 8 || ZL % 16 == 0) && TPW * 64 == YB * ZL, "a block is 4 waves of whole tiles");
   __shared__ __attribute__((aligned(16))) __bf16 halo[HV * CIN];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = lane & 15, kg = (lane >> 4) * 8;
   const int ybl = g.Y / YB;
   const int nx = blockIdx.x / ybl;
   const int y0 = (blockIdx.x - nx * ybl) * YB;
   const int n = nx / g.X, x = nx - n * g.X;
   u16x8 tmp[kPer];
-#pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const int e = threadIdx.x + i * 256;
-    tmp[i] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    if (e < kChunks) {
-      const int hv = e / CH, part = e - hv * CH;
-      const int ix = hv / (HY * HZ), r = hv - ix * (HY * HZ);
-      const int iy = r / HZ, iz = r - iy * HZ;
-      const int xx = x + ix - 1, yy = y0 + iy - 1, zz = iz - 1;
-      if (xx >= 0 && xx < g.X && yy >= 0 && yy < g.Y && zz >= 0 && zz < g.Z)
-        tmp[i] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + xx) * g.Y + yy) * g.Z + zz) * CIN +
-                                                 part * 8);
-    }
-  }
+  load_halo<CH, HY, HZ>(in, tmp, g, n, x, y0);
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int e = threadIdx.x + i * 256;
@@ -302,9 +336,10 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
   for (int tl = 0; tl < TW; ++tl)
 #pragma unroll
     for (int t = 0; t < GW; ++t) acc[tl][t] = floatx4{0, 0, 0, 0};
-  for (int ks = 0; ks < KS; ++ks) {
+  // Operands of K-step ks: the B fragments (global memory) and each tile's A operand (LDS); the next step's
+  // are loaded during this step's MFMAs.  Same MFMA sequence per output element.
+  auto load_ops = [&](int ks, bf16x8 (&Bt)[GW], u16x8 (&raw)[TW]) {
     const int k0 = ks * 32 + kg;
-    bf16x8 Bt[GW];
 #pragma unroll
     for (int t = 0; t < GW; ++t)
       Bt[t] = __builtin_bit_cast(bf16x8,
@@ -319,12 +354,27 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
     for (int tl = 0; tl < TW; ++tl) {
       const int off = tile_of(tl) * 16 + m;  // this lane's voxel in the block
       const int yy = off / ZL, z = off - yy * ZL;
-      u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (hoff >= 0) raw = *reinterpret_cast<const u16x8*>(halo + hoff + (yy * HZ + z) * CIN);
+      raw[tl] = hoff >= 0 ? *reinterpret_cast<const u16x8*>(halo + hoff + (yy * HZ + z) * CIN)
+                          : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
+    }
+  };
+  auto mfmas = [&](const bf16x8 (&Bt)[GW], const u16x8 (&raw)[TW]) {
+#pragma unroll
+    for (int tl = 0; tl < TW; ++tl)
 #pragma unroll
       for (int t = 0; t < GW; ++t)
-        acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), Bt[t], acc[tl][t], 0,
-                                                            0, 0);
+        acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw[tl]), Bt[t],
+                                                            acc[tl][t], 0, 0, 0);
+  };
+  bf16x8 B0[GW], B1[GW];
+  u16x8 R0[TW], R1[TW];
+  load_ops(0, B0, R0);
+  for (int ks = 0; ks < KS; ks += 2) {
+    if (ks + 1 < KS) load_ops(ks + 1, B1, R1);
+    mfmas(B0, R0);
+    if (ks + 1 < KS) {
+      if (ks + 2 < KS) load_ops(ks + 2, B0, R0);
+      mfmas(B1, R1);
     }
   }
   const int64_t vblock = (int64_t)blockIdx.x * (YB * ZL);
@@ -343,81 +393,99 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
 
 // Split-K gather convolution for the coarse levels (few voxels, 64 or 128 input channels): there a block per
 // 64 voxels leaves most of the GPU idle while each of its waves walks 27 Cin / 32 K steps, one dependent
-// round of loads after another.  Here a block computes 16 voxels x NT * 16 output channels (channel group
-// blockIdx.y); its four waves take every fourth K step (wave w: steps w, w + 4, ...), with the operands of
-// the next step loaded during the current step's MFMAs, and the four partial sums are added in wave order
-// through LDS.  Operands as in k_conv_fwd.
-template <int NT>
+// round of loads after another.  Here a block computes MT tiles of 16 voxels x NT * 16 output channels
+// (channel group blockIdx.y); its four waves take every fourth K step (wave w: steps w, w + 4, ...), with
+// the operands of the next two of its steps in flight during the current step's MFMAs, and the four partial
+// sums are added in wave order through LDS.  The MT tiles share each B fragment (weights: with one tile per
+// block the weights are fetched again for every 16 voxels).  Operands as in k_conv_fwd; every output element
+// gets the same MFMA sequence and sum whatever MT.
+template <int NT, int MT>
 __global__ __launch_bounds__(256) void k_conv_fwd_sk(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
                                                      const float* __restrict__ bias, float* __restrict__ out, Geo g,
                                                      int Cin, int Cout, int Kp) {
-  __shared__ floatx4 red[3][NT][64];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  constexpr int D = 3;  // K steps of this wave in registers: the current one and the next two
+  __shared__ floatx4 red[3][MT][NT][64];
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
   const int m = lane & 15, kg = (lane >> 4) * 8;
   const int64_t V = g.V();
-  const int64_t v0 = (int64_t)blockIdx.x * 16;
+  const int64_t v0 = (int64_t)blockIdx.x * (MT * 16);
   const int co0 = (int)blockIdx.y * NT * 16;
-  const int64_t va = v0 + m;
-  int n = 0, x = 0, y = 0, z = 0;
-  const bool va_ok = va < V;
-  if (va_ok) decompose(g, va, n, x, y, z);
+  int n[MT], x[MT], y[MT], z[MT];
+  bool ok[MT];
+#pragma unroll
+  for (int t = 0; t < MT; ++t) {
+    const int64_t va = v0 + t * 16 + m;
+    ok[t] = va < V;
+    n[t] = x[t] = y[t] = z[t] = 0;
+    if (ok[t]) decompose(g, va, n[t], x[t], y[t], z[t]);
+  }
   const int K = 27 * Cin, KS = Kp / 32;
-  auto load_a = [&](int ks) {
+  auto load_a = [&](int ks, int t) {
     const int k0 = ks * 32 + kg;
     u16x8 raw = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    if (va_ok && k0 < K) {
+    if (ok[t] && k0 < K) {
       const int tap = k0 / Cin, ci = k0 - tap * Cin;
-      const int64_t nb = tap_voxel(g, n, x, y, z, tap);
+      const int64_t nb = tap_voxel(g, n[t], x[t], y[t], z[t], tap);
       if (nb >= 0) raw = *reinterpret_cast<const u16x8*>(in + nb * Cin + ci);
     }
     return raw;
   };
   const __bf16* wrow = W + (int64_t)(co0 + m) * Kp + kg;
-  floatx4 acc[NT];
+  floatx4 acc[MT][NT];
 #pragma unroll
-  for (int t = 0; t < NT; ++t) acc[t] = floatx4{0, 0, 0, 0};
-  int ks = wave;
-  u16x8 a = u16x8{0, 0, 0, 0, 0, 0, 0, 0}, b[NT];
-  if (ks < KS) {
-    a = load_a(ks);
+  for (int t = 0; t < MT; ++t)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const u16x8*>(wrow + (int64_t)t * 16 * Kp + ks * 32);
-  }
-  for (; ks < KS; ks += 4) {
-    const u16x8 ac = a;
-    u16x8 bc[NT];
+    for (int u = 0; u < NT; ++u) acc[t][u] = floatx4{0, 0, 0, 0};
+  const int cnt = KS > wave ? (KS - wave + 3) / 4 : 0;  // this wave's K steps: wave + 4 i, i < cnt
+  u16x8 a[D][MT], b[D][NT];
+  auto load = [&](int i, u16x8 (&as)[MT], u16x8 (&bs)[NT]) {
+    const int ks = wave + 4 * i;
 #pragma unroll
-    for (int t = 0; t < NT; ++t) bc[t] = b[t];
-    if (ks + 4 < KS) {  // the next step's operands, in flight during this step's MFMAs
-      a = load_a(ks + 4);
+    for (int t = 0; t < MT; ++t) as[t] = load_a(ks, t);
 #pragma unroll
-      for (int t = 0; t < NT; ++t) b[t] = *reinterpret_cast<const u16x8*>(wrow + (int64_t)t * 16 * Kp + (ks + 4) * 32);
+    for (int u = 0; u < NT; ++u) bs[u] = *reinterpret_cast<const u16x8*>(wrow + (int64_t)u * 16 * Kp + ks * 32);
+  };
+#pragma unroll
+  for (int d = 0; d < D - 1; ++d)
+    if (d < cnt) load(d, a[d], b[d]);
+  for (int i = 0; i < cnt; i += D) {
+#pragma unroll
+    for (int s = 0; s < D; ++s) {
+      if (i + s < cnt) {
+        if (i + s + D - 1 < cnt) load(i + s + D - 1, a[(s + D - 1) % D], b[(s + D - 1) % D]);
+#pragma unroll
+        for (int t = 0; t < MT; ++t)
+#pragma unroll
+          for (int u = 0; u < NT; ++u)
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[s][t]),
+                                                                __builtin_bit_cast(bf16x8, b[s][u]), acc[t][u], 0, 0, 0);
+      }
     }
-#pragma unroll
-    for (int t = 0; t < NT; ++t)
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, ac), __builtin_bit_cast(bf16x8, bc[t]),
-                                                      acc[t], 0, 0, 0);
   }
   if (wave > 0) {
 #pragma unroll
-    for (int t = 0; t < NT; ++t) red[wave - 1][t][lane] = acc[t];
+    for (int t = 0; t < MT; ++t)
+#pragma unroll
+      for (int u = 0; u < NT; ++u) red[wave - 1][t][u][lane] = acc[t][u];
   }
   __syncthreads();
   if (wave != 0) return;
 #pragma unroll
-  for (int t = 0; t < NT; ++t) {
-    floatx4 s = acc[t];
-    s += red[0][t][lane];
-    s += red[1][t][lane];
-    s += red[2][t][lane];
-    const int co = co0 + t * 16 + m;
-    const float bb = bias ? bias[co] : 0.0f;
+  for (int t = 0; t < MT; ++t)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t v = v0 + (lane >> 4) * 4 + j;
-      if (v < V) out[v * Cout + co] = s[j] + bb;
+    for (int u = 0; u < NT; ++u) {
+      floatx4 sm = acc[t][u];
+      sm += red[0][t][u][lane];
+      sm += red[1][t][u][lane];
+      sm += red[2][t][u][lane];
+      const int co = co0 + u * 16 + m;
+      const float bb = bias ? bias[co] : 0.0f;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int64_t v = v0 + t * 16 + (lane >> 4) * 4 + j;
+        if (v < V) out[v * Cout + co] = sm[j] + bb;
+      }
     }
-  }
 }
 
 // W'[ci][tap' * Cout + co] = W[co][(26 - tap') * Cin + ci], K' padded with zeros.
@@ -449,7 +517,7 @@ __global__ __launch_bounds__(256) void k_conv_wgrad(const __bf16* __restrict__ i
   const int tap = blockIdx.x;
   const int cot = blockIdx.y;
   const int chunk = blockIdx.z;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int64_t V = g.V();
   const int64_t vb = (int64_t)chunk * vchunk;
   const int64_t ve = vb + vchunk < V ? vb + vchunk : V;
@@ -515,41 +583,57 @@ __global__ void k_wgrad_reduce(const float* __restrict__ part, float* __restrict
 
 // ------------------------------------------- weight grad, z-column tiles ---
 // For Z in {4, 8, 16, 32}: a block owns tiles of TX x TY (x, y) columns over the
-// full z range and the 9 taps that share one dx (grid.y = dx + 1), and loops
-// over tiles (grid.x blocks), accumulating in registers.  Per tile it stages,
-// in LDS and transposed to [channel][column][z]:
-//   sg          the upstream gradient of the tile's columns (A operand, co x voxel);
-//   sx[dz + 1]  the input of the TX x (TY + 2) source columns at x + dx,
-//               shifted by dz (sx[dz + 1][ci][col][z] = in[col][z + dz], zero
-//               outside), so every B operand (voxel x ci) is an aligned
-//               16-byte LDS read whatever the tap.
-// A K-step is 32 consecutive voxels of the tile in (column, z) order; lane
-// group l / 16 takes 8 of them (never straddling a column: Z % 8 == 0).
-// Each wave owns taps p = wave, wave + 4, wave + 8 of the block's nine
-// (p = (dy + 1) * 3 + dz + 1).  Partials: part[block][co][tap * Cin + ci].
+// full z range and the 9 taps that share one dx (grid.y = dx + 1; grid.z splits
+// wide layers' output channels), and loops over tiles (grid.x blocks),
+// accumulating in registers.  Per tile it stages in LDS the upstream gradient
+// of the tile's columns (A operand, co x voxel) and the input of the TX x
+// (TY + 2) source columns at x + dx (B operand, voxel x ci), with the next
+// tile's loads in flight during the current tile's MFMAs.  A K-step is 32
+// consecutive voxels of the tile in (column, z) order; lane group l / 16 takes
+// 8 of them.  Partials: part[dx + 1][block][co][p * Cin + ci], p = (dy + 1) * 3
+// + dz + 1, summed over the blocks in a fixed order by k_wgrad_tile_reduce.
 constexpr int kWTX = 2, kWTY = 4;
 
+// Weight gradient with both operands kept in LDS in their natural channels-last
+// layout and read transposed by ds_read_b64_tr_b16 (gfx950).  Within each
+// 16-lane group that instruction returns, to lane i, element i % 4 of the 8-byte
+// rows addressed by lanes i / 4, i / 4 + 4, i / 4 + 8 and i / 4 + 12 (checked
+// on the device).  With lane r of the group addressing voxel k0 + r / 4 at
+// channel 4 (r % 4), lane i receives channel i of voxels k0 .. k0 + 3: two such
+// reads give the 8 consecutive voxels of channel i that an MFMA operand lane
+// holds (A: rows = output channels, B: columns = input channels, K = voxels).
+// Staging is then a plain 16-byte copy per 8 channels of a voxel, the input is
+// stored once (the tap's z shift is an address offset into its column, which
+// carries zero rows at z = -1 and z = ZZ), and rows of 32 or more channels are
+// padded by 16 channels so that the four rows of a read fall in distinct banks.
+// Wave w takes taps w and w + 4 of the block's nine over every K-step; the
+// ninth tap's K-steps go to wave ks % 4 when there are a multiple of four of
+// them (their four sums folded in wave order at the end), else all to wave 0.
+typedef short s16x4 __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(3))) s16x4 lds_s16x4;
+
+__device__ __forceinline__ s16x4 lds_read_tr16(const __bf16* p) {
+  return __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_s16x4*)(p));
+}
+
 template <int CIT, int COT, int ZZ>
-__global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restrict__ in,
-                                                         const __bf16* __restrict__ gout, float* __restrict__ part,
-                                                         Geo g, int ntx, int nty, int CoutT) {
-  // This block: COT 16-channel output tiles starting at tile blockIdx.z * COT of the CoutT channels.
+__global__ __launch_bounds__(256) void k_conv_wgrad_tr(const __bf16* __restrict__ in, const __bf16* __restrict__ gout,
+                                                       float* __restrict__ part, Geo g, int ntx, int nty, int CoutT) {
   constexpr int Cin = CIT * 16, Cout = COT * 16;
   constexpr int OC = kWTX * kWTY;         // output columns per tile
   constexpr int SY = kWTY + 2;            // source columns per x row (y halo)
   constexpr int SC = kWTX * SY;           // source columns per tile
+  constexpr int ZH = ZZ + 2;              // rows per source column: z = -1 .. ZZ
   constexpr int KS = OC * ZZ / 32;        // K-steps per tile
-  // Rows padded by 8 values (16 bytes): consecutive rows start 16 bytes apart
-  // modulo 256, so the 16-byte reads of 8 lanes in different rows hit
-  // different LDS banks.
-  constexpr int RG = OC * ZZ + 8, RX = SC * ZZ + 8;
-  // A lane group's 8 voxels are 8 / ZZ consecutive columns of one x row when ZZ < 8.
-  static_assert((ZZ % 8 == 0 || (ZZ == 4 && kWTY % 2 == 0)) && (OC * ZZ) % 32 == 0, "tile shape");
+  constexpr int GR = Cout + (Cout >= 32 ? 16 : 0), XR = Cin + (Cin >= 32 ? 16 : 0);  // row lengths (elements)
+  static_assert(ZZ % 4 == 0 && (OC * ZZ) % 32 == 0, "tile shape");
+  constexpr bool kBal = KS % 4 == 0;
   const int co0 = (int)blockIdx.z * Cout;
-  __shared__ __attribute__((aligned(16))) __bf16 sg[Cout][RG];
-  __shared__ __attribute__((aligned(16))) __bf16 sx[3][Cin][RX];
+  __shared__ __attribute__((aligned(16))) __bf16 sg[OC * ZZ * GR];
+  __shared__ __attribute__((aligned(16))) __bf16 sx[SC * ZH * XR];
+  __shared__ __attribute__((aligned(16))) floatx4 red[kBal ? 4 * COT * CIT * 64 : 1];
   const int dx = (int)blockIdx.y - 1;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane((threadIdx.x >> 6) & 3);
   const int64_t ntiles = (int64_t)g.N * ntx * nty;
   floatx4 acc[3][COT][CIT];
 #pragma unroll
@@ -559,209 +643,147 @@ __global__ __launch_bounds__(256) void k_conv_wgrad_tile(const __bf16* __restric
 #pragma unroll
       for (int w = 0; w < CIT; ++w) acc[a][u][w] = floatx4{0, 0, 0, 0};
 
-  // Staging loads of a tile go to registers first (all of them in flight together), and the next
-  // tile's are issued before this tile's MFMAs; the LDS contents, and so the MFMA operands, are those
-  // of element-by-element staging.
-  //
-  // Wide staging (ZZ a multiple of 8, or 4): a staging element is ZW = min(ZZ, 8) consecutive z of one
-  // column and 8 channels.  Its loads are ZW voxels of 16 bytes (ZW + 2 for the input: z - 1 .. z + ZW,
-  // zeros outside the column), transposed in registers so that every LDS store is 2 ZW bytes of one channel
-  // row: 8 stores for a gradient element and 24 for an input element (the three z-shifted copies), where
-  // the voxel-wise staging below stores 2 bytes at a time (8 ZW and 24 ZW stores for the same data).  Same
-  // LDS contents.  (16 -> 16 channels at Z = 32 keeps the voxel-wise staging: the wide one needs more
-  // registers there and drops the kernel from 3 to 2 waves per SIMD, 466 -> 566 us per call at batch 8 on
-  // 152 x 128 x 32.)
-  constexpr int ZW = ZZ >= 8 ? 8 : ZZ;
-  constexpr bool kWide = (ZZ % 8 == 0 || ZZ == 4) && !(CIT == 1 && COT == 1 && ZZ == 32);
-  using zvec = typename std::conditional<ZW == 8, u16x8, u16x4>::type;
-  constexpr int ZO = kWide ? ZZ / ZW : 1;                  // z-runs per column
-  constexpr int WEG = OC * ZO * (Cout / 8), WEX = SC * ZO * (Cin / 8);
-  constexpr int WNG = kWide ? (WEG + 255) / 256 : 0, WNX = kWide ? (WEX + 255) / 256 : 0;
-  u16x8 wg[WNG > 0 ? WNG : 1][ZW], wx[WNX > 0 ? WNX : 1][ZW + 2];
-  auto fetch_wide = [&](int64_t tile) {
-    const int n = (int)(tile / ((int64_t)ntx * nty));
-    const int r = (int)(tile - (int64_t)n * ntx * nty);
-    const int x0 = (r / nty) * kWTX, y0 = (r % nty) * kWTY;
-#pragma unroll
-    for (int q = 0; q < WNG; ++q) {  // upstream gradient: OC columns x ZO octets x Cout / 8 channel groups
-      const int e = threadIdx.x + q * 256;
-      const int cg = e % (Cout / 8), ro = e / (Cout / 8);
-      const int zo = ro % ZO, oc = ro / ZO;
-      const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
-      const bool ok = e < WEG && x < g.X && y < g.Y;
-      const int64_t v0 = ok ? (((int64_t)n * g.X + x) * g.Y + y) * ZZ + zo * ZW : 0;
-      const __bf16* src = gout + v0 * CoutT + co0 + cg * 8;
-#pragma unroll
-      for (int j = 0; j < ZW; ++j)
-        wg[q][j] = ok ? *reinterpret_cast<const u16x8*>(src + (int64_t)j * CoutT) : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-    }
-#pragma unroll
-    for (int q = 0; q < WNX; ++q) {  // input of the source columns at x + dx, z - 1 .. z + ZW of each run
-      const int e = threadIdx.x + q * 256;
-      const int cg = e % (Cin / 8), ro = e / (Cin / 8);
-      const int zo = ro % ZO, sc = ro / ZO;
-      const int x = x0 + sc / SY + dx, y = y0 + sc % SY - 1;
-      const bool ok = e < WEX && x >= 0 && x < g.X && y >= 0 && y < g.Y;
-      const int64_t col = ok ? (((int64_t)n * g.X + x) * g.Y + y) * ZZ : 0;
-#pragma unroll
-      for (int j = 0; j < ZW + 2; ++j) {
-        const int z = zo * ZW - 1 + j;
-        wx[q][j] = (ok && z >= 0 && z < ZZ) ? *reinterpret_cast<const u16x8*>(in + (col + z) * Cin + cg * 8)
-                                            : u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      }
-    }
-  };
-  auto store_wide = [&]() {
-#pragma unroll
-    for (int q = 0; q < WNG; ++q) {
-      const int e = threadIdx.x + q * 256;
-      if (e < WEG) {
-        const int cg = e % (Cout / 8), ro = e / (Cout / 8);
-        const int zo = ro % ZO, oc = ro / ZO;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          zvec v;
-#pragma unroll
-          for (int j = 0; j < ZW; ++j) v[j] = wg[q][j][c];
-          *reinterpret_cast<zvec*>(&sg[cg * 8 + c][oc * ZZ + zo * ZW]) = v;
-        }
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < WNX; ++q) {
-      const int e = threadIdx.x + q * 256;
-      if (e < WEX) {
-        const int cg = e % (Cin / 8), ro = e / (Cin / 8);
-        const int zo = ro % ZO, sc = ro / ZO;
-        const int o = sc * ZZ + zo * ZW;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          zvec lo, mid, hi;  // sx[dz + 1][ci][col][z] = in[col][z + dz]
-#pragma unroll
-          for (int j = 0; j < ZW; ++j) {
-            lo[j] = wx[q][j][c];
-            mid[j] = wx[q][j + 1][c];
-            hi[j] = wx[q][j + 2][c];
-          }
-          *reinterpret_cast<zvec*>(&sx[0][cg * 8 + c][o]) = lo;
-          *reinterpret_cast<zvec*>(&sx[1][cg * 8 + c][o]) = mid;
-          *reinterpret_cast<zvec*>(&sx[2][cg * 8 + c][o]) = hi;
-        }
-      }
-    }
-  };
-  constexpr int NEG = kWide ? 0 : OC * ZZ * (Cout / 8), NEX = kWide ? 0 : SC * ZZ * (Cin / 8);
+  // Staging: 16-byte chunks (8 channels of one voxel), loaded to registers (all in flight together, the
+  // next tile's during this tile's MFMAs), then stored to LDS.
+  constexpr int NEG = OC * ZZ * (Cout / 8), NEX = SC * ZH * (Cin / 8);
   constexpr int NG = (NEG + 255) / 256, NX = (NEX + 255) / 256;
-  u16x8 rg[NG > 0 ? NG : 1], rx[NX > 0 ? NX : 1];
+  u16x8 rg[NG], rx[NX];
   auto fetch = [&](int64_t tile) {
     const int n = (int)(tile / ((int64_t)ntx * nty));
     const int r = (int)(tile - (int64_t)n * ntx * nty);
     const int x0 = (r / nty) * kWTX, y0 = (r % nty) * kWTY;
 #pragma unroll
-    for (int q = 0; q < NG; ++q) {  // upstream gradient: OC columns x ZZ voxels x Cout / 8 groups of 8 channels
+    for (int q = 0; q < NG; ++q) {  // upstream gradient of the tile's columns
       const int e = threadIdx.x + q * 256;
+      const int cg = e % (Cout / 8), v = e / (Cout / 8);
+      const int z = v % ZZ, oc = v / ZZ;
+      const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
       rg[q] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (e < NEG) {
-        const int cg = e % (Cout / 8), vz = e / (Cout / 8);
-        const int z = vz % ZZ, oc = vz / ZZ;
-        const int x = x0 + oc / kWTY, y = y0 + oc % kWTY;
-        if (x < g.X && y < g.Y)
-          rg[q] = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * CoutT +
-                                                  co0 + cg * 8);
-      }
+      if (e < NEG && x < g.X && y < g.Y)
+        rg[q] = *reinterpret_cast<const u16x8*>(gout + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * CoutT + co0 +
+                                                cg * 8);
     }
 #pragma unroll
-    for (int q = 0; q < NX; ++q) {  // input of the source columns at x + dx
+    for (int q = 0; q < NX; ++q) {  // input of the source columns at x + dx, with the zero z rows
       const int e = threadIdx.x + q * 256;
+      const int cg = e % (Cin / 8), v = e / (Cin / 8);
+      const int zh = v % ZH, sc = v / ZH;
+      const int x = x0 + sc / SY + dx, y = y0 + sc % SY - 1, z = zh - 1;
       rx[q] = u16x8{0, 0, 0, 0, 0, 0, 0, 0};
-      if (e < NEX) {
-        const int cg = e % (Cin / 8), vz = e / (Cin / 8);
-        const int z = vz % ZZ, sc = vz / ZZ;
-        const int x = x0 + sc / SY + dx, y = y0 + sc % SY - 1;
-        if (x >= 0 && x < g.X && y >= 0 && y < g.Y)
-          rx[q] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cin +
-                                                  cg * 8);
-      }
+      if (e < NEX && x >= 0 && x < g.X && y >= 0 && y < g.Y && z >= 0 && z < ZZ)
+        rx[q] = *reinterpret_cast<const u16x8*>(in + ((((int64_t)n * g.X + x) * g.Y + y) * ZZ + z) * Cin + cg * 8);
     }
   };
-  if (blockIdx.x < ntiles) {
-    if constexpr (kWide)
-      fetch_wide(blockIdx.x);
-    else
-      fetch(blockIdx.x);
-  }
+  const int a = lane >> 4, r4 = lane & 15;
+  const int rrow = r4 >> 2, rcol = (r4 & 3) * 4;  // this lane's row and channel offset in a transposed read
+  // One half (4 voxels) of a K-step's operands: the upstream gradient A[u] and the input B[w] of tap p.
+  auto put = [](bf16x8& d, int h, s16x4 v) {
+    u16x8 t = __builtin_bit_cast(u16x8, d);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) t[h * 4 + j] = (unsigned short)v[j];
+    d = __builtin_bit_cast(bf16x8, t);
+  };
+  auto voxel = [&](int ks, int h, int& oc, int& z) {  // the voxel this lane addresses, (column, z) order
+    const int L = ks * 32 + a * 8 + h * 4 + rrow;
+    oc = L / ZZ;
+    z = L % ZZ;
+  };
+  auto load_a = [&](int ks, bf16x8 (&A)[COT]) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      int oc, z;
+      voxel(ks, h, oc, z);
+#pragma unroll
+      for (int u = 0; u < COT; ++u) put(A[u], h, lds_read_tr16(&sg[(oc * ZZ + z) * GR + u * 16 + rcol]));
+    }
+  };
+  auto load_b = [&](int ks, int p, bf16x8 (&B)[CIT]) {
+    const int dy = p / 3 - 1, dzi = p % 3;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      int oc, z;
+      voxel(ks, h, oc, z);
+      const int row = ((oc / kWTY) * SY + oc % kWTY + dy + 1) * ZH + z + dzi;
+#pragma unroll
+      for (int w = 0; w < CIT; ++w) put(B[w], h, lds_read_tr16(&sx[row * XR + w * 16 + rcol]));
+    }
+  };
+  auto mfma = [&](floatx4 (&c)[COT][CIT], const bf16x8 (&A)[COT], const bf16x8 (&B)[CIT]) {
+#pragma unroll
+    for (int w = 0; w < CIT; ++w)
+#pragma unroll
+      for (int u = 0; u < COT; ++u) c[u][w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[u], B[w], c[u][w], 0, 0, 0);
+  };
+  if (blockIdx.x < ntiles) fetch(blockIdx.x);
   for (int64_t tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
-    if constexpr (kWide) store_wide();
 #pragma unroll
     for (int q = 0; q < NG; ++q) {
       const int e = threadIdx.x + q * 256;
-      if (e < NEG) {
-        const int cg = e % (Cout / 8), vz = e / (Cout / 8);
-        const int z = vz % ZZ, oc = vz / ZZ;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) reinterpret_cast<unsigned short*>(&sg[cg * 8 + c][0])[oc * ZZ + z] = rg[q][c];
-      }
+      if (e < NEG) *reinterpret_cast<u16x8*>(&sg[(e / (Cout / 8)) * GR + (e % (Cout / 8)) * 8]) = rg[q];
     }
-    // three z-shifted copies of the input
 #pragma unroll
     for (int q = 0; q < NX; ++q) {
       const int e = threadIdx.x + q * 256;
-      if (e < NEX) {
-        const int cg = e % (Cin / 8), vz = e / (Cin / 8);
-        const int z = vz % ZZ, sc = vz / ZZ;
-        const int o = sc * ZZ + z;
-#pragma unroll
-        for (int c = 0; c < 8; ++c) {
-          if (z + 1 < ZZ) reinterpret_cast<unsigned short*>(&sx[0][cg * 8 + c][0])[o + 1] = rx[q][c];  // dz = -1
-          reinterpret_cast<unsigned short*>(&sx[1][cg * 8 + c][0])[o] = rx[q][c];                      // dz =  0
-          if (z > 0) reinterpret_cast<unsigned short*>(&sx[2][cg * 8 + c][0])[o - 1] = rx[q][c];       // dz = +1
-        }
-      }
+      if (e < NEX) *reinterpret_cast<u16x8*>(&sx[(e / (Cin / 8)) * XR + (e % (Cin / 8)) * 8]) = rx[q];
     }
-    // positions whose source z lies outside [0, ZZ): zero (never written above; the wide staging stores them)
-    if constexpr (!kWide) {
-      for (int e = threadIdx.x; e < Cin * SC; e += 256) {
-        const int ci = e / SC, sc = e % SC;
-        reinterpret_cast<unsigned short*>(&sx[0][ci][0])[sc * ZZ] = 0;
-        reinterpret_cast<unsigned short*>(&sx[2][ci][0])[sc * ZZ + ZZ - 1] = 0;
-      }
-    }
-    if (tile + gridDim.x < ntiles) {  // in flight during this tile's MFMAs
-      if constexpr (kWide)
-        fetch_wide(tile + gridDim.x);
-      else
-        fetch(tile + gridDim.x);
-    }
+    if (tile + gridDim.x < ntiles) fetch(tile + gridDim.x);  // in flight during this tile's MFMAs
     __syncthreads();
-    const int a = lane >> 4, m = lane & 15;
+    // Taps wave and wave + 4 over every K-step, the next step's operands read during this step's MFMAs.
+    {
+      bf16x8 A[2][COT], B0[2][CIT], B1[2][CIT];
+      load_a(0, A[0]);
+      load_b(0, wave, B0[0]);
+      load_b(0, wave + 4, B1[0]);
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) {
-      const int L = ks * 32 + a * 8;  // the lane group's first voxel, (column, z) order
-      const int oc = L / ZZ, z0 = L % ZZ;
-      const int ox = oc / kWTY, oy = oc % kWTY;
-      bf16x8 A[COT];
-#pragma unroll
-      for (int u = 0; u < COT; ++u) A[u] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sg[u * 16 + m][L]));
-#pragma unroll
-      for (int pi = 0; pi < 3; ++pi) {
-        const int p = wave + 4 * pi;
-        if (p >= 9) continue;
-        const int dy = p / 3 - 1, dzi = p % 3;
-        const int so = (ox * SY + oy + dy + 1) * ZZ + z0;
-#pragma unroll
-        for (int w = 0; w < CIT; ++w) {
-          const bf16x8 B = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(&sx[dzi][w * 16 + m][so]));
-#pragma unroll
-          for (int u = 0; u < COT; ++u)
-            acc[pi][u][w] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[u], B, acc[pi][u][w], 0, 0, 0);
+      for (int ks = 0; ks < KS; ++ks) {
+        const int cur = ks & 1;
+        if (ks + 1 < KS) {
+          load_a(ks + 1, A[cur ^ 1]);
+          load_b(ks + 1, wave, B0[cur ^ 1]);
+          load_b(ks + 1, wave + 4, B1[cur ^ 1]);
         }
+        mfma(acc[0], A[cur], B0[cur]);
+        mfma(acc[1], A[cur], B1[cur]);
+      }
+    }
+    // The ninth tap: K-steps dealt out over the waves (balanced), or all of them on wave 0.
+    if (kBal || wave == 0) {
+      constexpr int NK = kBal ? KS / 4 : KS;
+      bf16x8 A[2][COT], B[2][CIT];
+      const int k0 = kBal ? wave : 0, kstep = kBal ? 4 : 1;
+      load_a(k0, A[0]);
+      load_b(k0, 8, B[0]);
+#pragma unroll
+      for (int j = 0; j < NK; ++j) {
+        const int cur = j & 1;
+        if (j + 1 < NK) {
+          load_a(k0 + (j + 1) * kstep, A[cur ^ 1]);
+          load_b(k0 + (j + 1) * kstep, 8, B[cur ^ 1]);
+        }
+        mfma(acc[2], A[cur], B[cur]);
       }
     }
     __syncthreads();
   }
-  // D[m = co][n = ci]: lane holds co = 4 * (l / 16) + j, ci = l % 16 of each 16 x 16 tile.
-  // part[dx + 1][block][co][p * Cin + ci], p = the tap's index among the group's nine.
   const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  if constexpr (kBal) {  // the ninth tap's four per-wave sums, folded in wave order
+#pragma unroll
+    for (int u = 0; u < COT; ++u)
+#pragma unroll
+      for (int w = 0; w < CIT; ++w) red[((wave * COT + u) * CIT + w) * 64 + lane] = acc[2][u][w];
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int u = 0; u < COT; ++u)
+#pragma unroll
+        for (int w = 0; w < CIT; ++w) {
+          floatx4 s = acc[2][u][w];
+#pragma unroll
+          for (int wv = 1; wv < 4; ++wv) s += red[((wv * COT + u) * CIT + w) * 64 + lane];
+          acc[2][u][w] = s;
+        }
+    }
+  }
 #pragma unroll
   for (int pi = 0; pi < 3; ++pi) {
     const int p = wave + 4 * pi;
@@ -1135,6 +1157,50 @@ __global__ void k_mse_bf16(const float* __restrict__ pred, const float* __restri
   if (threadIdx.x == 0) part[blockIdx.x] = red[0];
 }
 
+// k_mse_bf16 for C = 16 with one voxel per thread: the prediction row in four 16-byte loads, the gradient
+// row in two 16-byte stores.  The gradient has the same bits; the loss partials sum a voxel's channels first.
+// Partials nblk(V) .. npart - 1 (the scalar kernel's count, which the caller allocates) are set to zero.
+__global__ void k_mse16_bf16(const float* __restrict__ pred, const float* __restrict__ target,
+                             const float* __restrict__ wmask, const float* __restrict__ wsum,
+                             __bf16* __restrict__ grad, float* __restrict__ part, int64_t V, int nout,
+                             int64_t npart) {
+  __shared__ float red[256];
+  const float inv_norm = 1.0f / fmaxf(*wsum, 1.0f);
+  const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  for (int64_t k = gridDim.x + v; k < npart; k += (int64_t)gridDim.x * blockDim.x) part[k] = 0.0f;
+  float l = 0.0f;
+  if (v < V) {
+    float pr[16];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const f32x4 a = reinterpret_cast<const f32x4*>(pred + v * 16)[q];
+#pragma unroll
+      for (int k = 0; k < 4; ++k) pr[q * 4 + k] = a[k];
+    }
+    u16x8 o[2];
+#pragma unroll
+    for (int c = 0; c < 16; ++c) {
+      float gr = 0.0f;
+      if (c < nout) {
+        const float w = wmask[v * nout + c];
+        const float d = pr[c] - target[v * nout + c];
+        l += w * d * d * inv_norm;
+        gr = 2.0f * w * d * inv_norm;
+      }
+      o[c / 8][c % 8] = __builtin_bit_cast(unsigned short, f2bf(gr));
+    }
+    reinterpret_cast<u16x8*>(grad + v * 16)[0] = o[0];
+    reinterpret_cast<u16x8*>(grad + v * 16)[1] = o[1];
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
 // Adam (Kingma & Ba 2015) on FP32 master weights; refreshes the BF16 copy.
 __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float* __restrict__ m,
                        float* __restrict__ v, __bf16* __restrict__ wb, int64_t n, float lr, float b1, float b2,
@@ -1238,13 +1304,22 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
   }
   if (Cin >= 64 && (Z <= 4 || Cin == 128)) {
     // the coarsest level (Z <= 4) and 128 input channels: split-K over the waves of a block
-    const unsigned gv = (unsigned)((V + 15) / 16);
-    if (Cout % 64 == 0)
-      hipLaunchKernelGGL(k_conv_fwd_sk<4>, dim3(gv, Cout / 64), dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp);
-    else if (Cout % 32 == 0)
-      hipLaunchKernelGGL(k_conv_fwd_sk<2>, dim3(gv, Cout / 32), dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp);
-    else
-      hipLaunchKernelGGL(k_conv_fwd_sk<1>, dim3(gv, Cout / 16), dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp);
+    // tiles per block: as many as keep at least 512 blocks (two per CU) in the launch
+    const int nt = Cout % 64 == 0 ? 4 : Cout % 32 == 0 ? 2 : 1;
+    const int64_t groups = Cout / (16 * nt);
+    const int mt = (V + 63) / 64 * groups >= 512 ? 4 : (V + 31) / 32 * groups >= 512 ? 2 : 1;
+    const dim3 grid((unsigned)((V + 16 * mt - 1) / (16 * mt)), (unsigned)groups);
+    switch (nt * 10 + mt) {
+      case 44: hipLaunchKernelGGL((k_conv_fwd_sk<4, 4>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 42: hipLaunchKernelGGL((k_conv_fwd_sk<4, 2>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 41: hipLaunchKernelGGL((k_conv_fwd_sk<4, 1>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 24: hipLaunchKernelGGL((k_conv_fwd_sk<2, 4>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 22: hipLaunchKernelGGL((k_conv_fwd_sk<2, 2>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 21: hipLaunchKernelGGL((k_conv_fwd_sk<2, 1>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 14: hipLaunchKernelGGL((k_conv_fwd_sk<1, 4>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      case 12: hipLaunchKernelGGL((k_conv_fwd_sk<1, 2>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+      default: hipLaunchKernelGGL((k_conv_fwd_sk<1, 1>), grid, dim3(256), 0, s, i, w, bias, out, g, Cin, Cout, Kp); break;
+    }
     return hipGetLastError() == hipSuccess ? 0 : 2;
   }
   if (Cout == 16 && (Cin == 16 || Cin == 32) && Kp == (27 * Cin + 31) / 32 * 32) {
@@ -1304,12 +1379,13 @@ int wgrad_tile_blocks(int N, int X, int Y, int Z, int Cin, int Cout) {
 }
 
 template <int CIT, int COT, int ZZ>
-void launch_wgrad_tile(const __bf16* i, const __bf16* go, float* part, Geo g, int nb, int split, int CoutT,
-                       hipStream_t s) {
+void launch_wgrad_tr(const __bf16* i, const __bf16* go, float* part, Geo g, int nb, int split, int CoutT,
+                     hipStream_t s) {
   const int ntx = (g.X + kWTX - 1) / kWTX, nty = (g.Y + kWTY - 1) / kWTY;
-  hipLaunchKernelGGL((k_conv_wgrad_tile<CIT, COT, ZZ>), dim3(nb, 3, split), dim3(256), 0, s, i, go, part, g, ntx,
-                     nty, CoutT);
+  hipLaunchKernelGGL((k_conv_wgrad_tr<CIT, COT, ZZ>), dim3(nb, 3, split), dim3(256), 0, s, i, go, part, g, ntx, nty,
+                     CoutT);
 }
+
 
 // ------------------------------------------------------------------ FiLM ---
 // Feature-wise affine modulation of a layer's pre-activations by per-sample
@@ -1372,6 +1448,23 @@ __global__ void k_film_silu_fwd(const float* __restrict__ p, const float* __rest
   const int n = (int)(i / ((int64_t)C * Vn));
   const float* f = film + (int64_t)n * L + off;
   y[i] = f2bf(silu(p[i] * (1.0f + f[c]) + f[C + c]));
+}
+
+// k_film_silu_fwd with 4 values per thread (16-byte loads, 8-byte stores), C a power of two >= 4 and
+// Vn * C / 4 < 2^31: sample blockIdx.y, so the index math is 32-bit with no division.  Same bits.
+__global__ void k_film_silu_fwd4(const float* __restrict__ p, const float* __restrict__ film, __bf16* __restrict__ y,
+                                 uint32_t n4, int C, int off, int L) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n4) return;
+  const int n = blockIdx.y;
+  const int64_t e = (int64_t)n * n4 + i;  // 4-value group in the whole tensor
+  const int c = (int)((i * 4) & (uint32_t)(C - 1));
+  const float* f = film + (int64_t)n * L + off;
+  const f32x4 v = reinterpret_cast<const f32x4*>(p)[e];
+  u16x4 o;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) o[k] = __builtin_bit_cast(unsigned short, f2bf(silu(v[k] * (1.0f + f[c + k]) + f[C + c + k])));
+  reinterpret_cast<u16x4*>(y)[e] = o;
 }
 
 // Backward of k_film_silu_fwd from the gradient gy w.r.t. its output: the modulated pre-activation is formed
@@ -1638,16 +1731,16 @@ int xnn_conv_wgrad(const void* in, const void* gout, float* part, float* dW, int
     wgrad_tile_plan(Z, Cin, Cout, cotb, split);
     const int key = Z * 100 + (Cin / 16) * 10 + cotb;
     switch (key) {
-      case 3211: launch_wgrad_tile<1, 1, 32>(i, go, part, g, nb, split, Cout, s); break;
-      case 3221: launch_wgrad_tile<2, 1, 32>(i, go, part, g, nb, split, Cout, s); break;
-      case 1612: launch_wgrad_tile<1, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
-      case 1622: launch_wgrad_tile<2, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
-      case 1642: launch_wgrad_tile<4, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
-      case 824: launch_wgrad_tile<2, 4, 8>(i, go, part, g, nb, split, Cout, s); break;
-      case 844: launch_wgrad_tile<4, 4, 8>(i, go, part, g, nb, split, Cout, s); break;
-      case 881: launch_wgrad_tile<8, 1, 8>(i, go, part, g, nb, split, Cout, s); break;
-      case 442: launch_wgrad_tile<4, 2, 4>(i, go, part, g, nb, split, Cout, s); break;
-      case 481: launch_wgrad_tile<8, 1, 4>(i, go, part, g, nb, split, Cout, s); break;
+      case 3211: launch_wgrad_tr<1, 1, 32>(i, go, part, g, nb, split, Cout, s); break;
+      case 3221: launch_wgrad_tr<2, 1, 32>(i, go, part, g, nb, split, Cout, s); break;
+      case 1612: launch_wgrad_tr<1, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
+      case 1622: launch_wgrad_tr<2, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
+      case 1642: launch_wgrad_tr<4, 2, 16>(i, go, part, g, nb, split, Cout, s); break;
+      case 824: launch_wgrad_tr<2, 4, 8>(i, go, part, g, nb, split, Cout, s); break;
+      case 844: launch_wgrad_tr<4, 4, 8>(i, go, part, g, nb, split, Cout, s); break;
+      case 881: launch_wgrad_tr<8, 1, 8>(i, go, part, g, nb, split, Cout, s); break;
+      case 442: launch_wgrad_tr<4, 2, 4>(i, go, part, g, nb, split, Cout, s); break;
+      case 481: launch_wgrad_tr<8, 1, 4>(i, go, part, g, nb, split, Cout, s); break;
       default: return 1;
     }
     hipLaunchKernelGGL(k_wgrad_tile_reduce, dim3(nblk(n)), dim3(256), 0, s, part, dW, nb, Cout, Cin, Kp);
@@ -1776,6 +1869,14 @@ int xnn_up_bwd(const float* gfine, float* gcoarse, int N, int CX, int CY, int CZ
     hipLaunchKernelGGL(k_up_bwd, dim3(nblk(gc.V() * C)), dim3(256), 0, (hipStream_t)stream, gfine, gcoarse, gc, C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
+}  // extern "C"
+namespace {
+// The one-voxel-per-thread MSE: 16 channels and 16-byte aligned rows.
+bool mse16(const float* pred, const void* grad, int C) {
+  return C == 16 && reinterpret_cast<uintptr_t>(pred) % 16 == 0 && reinterpret_cast<uintptr_t>(grad) % 16 == 0;
+}
+}  // namespace
+extern "C" {
 int xnn_mse(const float* pred, const float* target, const float* wmask, float* grad, float* part, int64_t V, int C,
             int nout, float inv_norm, void* stream) {
   hipLaunchKernelGGL(k_mse, dim3(nblk(V * C)), dim3(256), 0, (hipStream_t)stream, pred, target, wmask, grad, part,
@@ -1785,6 +1886,11 @@ int xnn_mse(const float* pred, const float* target, const float* wmask, float* g
 int xnn_mse_bf16(const float* pred, const float* target, const float* wmask, const float* wsum, void* grad,
                  float* part, int64_t V, int C, int nout, void* stream) {
   if (V < 1 || C < 1 || nout < 1 || nout > C) return 1;
+  if (mse16(pred, grad, C)) {
+    hipLaunchKernelGGL(k_mse16_bf16, dim3(nblk(V)), dim3(256), 0, (hipStream_t)stream, pred, target, wmask, wsum,
+                       (__bf16*)grad, part, V, nout, (int64_t)nblk(V * C));
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
   hipLaunchKernelGGL(k_mse_bf16, dim3(nblk(V * C)), dim3(256), 0, (hipStream_t)stream, pred, target, wmask, wsum,
                      (__bf16*)grad, part, V, C, nout);
   return hipGetLastError() == hipSuccess ? 0 : 2;
@@ -1815,6 +1921,13 @@ int xnn_film_silu_fwd(const float* p, const float* film, void* y, int N, int64_t
                       void* stream) {
   if (N < 1 || Vn < 1 || C < 1 || off < 0 || off + 2 * C > L) return 1;
   const int64_t total = (int64_t)N * Vn * C;
+  const int64_t n4 = Vn * C / 4;
+  if (C >= 4 && (C & (C - 1)) == 0 && n4 < ((int64_t)1 << 31) && N <= 65535 &&
+      reinterpret_cast<uintptr_t>(p) % 16 == 0 && reinterpret_cast<uintptr_t>(y) % 8 == 0) {
+    hipLaunchKernelGGL(k_film_silu_fwd4, dim3(nblk(n4), N), dim3(256), 0, (hipStream_t)stream, p, film, (__bf16*)y,
+                       (uint32_t)n4, C, off, L);
+    return hipGetLastError() == hipSuccess ? 0 : 2;
+  }
   hipLaunchKernelGGL(k_film_silu_fwd, dim3(nblk(total)), dim3(256), 0, (hipStream_t)stream, p, film, (__bf16*)y,
                      Vn, C, off, L, total);
   return hipGetLastError() == hipSuccess ? 0 : 2;
