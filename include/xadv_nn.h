@@ -18,6 +18,10 @@ extern "C" {
  * Cin, Cout multiples of 16, Cout <= 128, Kp = K padded to 32 (K = 27*Cin). BF16 MFMA, FP32 out. */
 int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, int N, int X, int Y, int Z, int Cin,
                  int Cout, int Kp, void* stream);
+/* xnn_conv_fwd that also writes act = bf16(SiLU(out)) (the bits of xnn_silu_fwd on out): in the epilogue of the
+ * LDS-halo kernels (the full- and half-resolution layers), by a separate pass otherwise.  act must not be NULL. */
+int xnn_conv_fwd_silu(const void* in, const void* W, const float* bias, float* out, void* act, int N, int X, int Y,
+                      int Z, int Cin, int Cout, int Kp, void* stream);
 /* Weights of the data-gradient convolution: Wt[ci][tap'*Cout+co] = W[co][(26-tap')*Cin+ci]. */
 int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void* stream);
 /* dW[co][tap*Cin+ci] = sum_v gout[v][co] * in[v+tap][ci] (zero in the K padding).  Z in {4, 8, 16, 32} with

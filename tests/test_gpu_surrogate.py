@@ -386,6 +386,26 @@ def test_conv_split_k_coarse_levels(gpu, cin, cout, shape):
     assert rel(gin, ndhwc(xr.grad)) < 1e-3
 
 
+@pytest.mark.parametrize("cin,cout,shape", [(16, 16, (2, 3, 16, 32)), (32, 16, (2, 3, 8, 32)),
+                                             (16, 32, (2, 3, 16, 16)), (32, 32, (1, 3, 16, 16)),
+                                             (64, 64, (1, 3, 8, 8)), (64, 128, (1, 3, 5, 4)),
+                                             (16, 16, (1, 3, 5, 6))])
+def test_conv_with_silu_epilogue_matches_conv_then_silu(gpu, cin, cout, shape):
+    """xnn_conv_fwd_silu gives the bits of xnn_conv_fwd followed by xnn_silu_fwd: in the LDS-halo kernels'
+    epilogue (the first five shapes) and through the separate pass (split-K and gather kernels)."""
+    from xadvcool_amd import surrogate as S
+    print(f"[nn] conv+SiLU {cin}->{cout} {shape}", flush=True)
+    gen = torch.Generator().manual_seed(77 + cin + cout + shape[2])
+    c = S.Conv(cin, cout, gen)
+    c.b.copy_(torch.randn(cout, generator=gen))
+    x = torch.randn(*shape, cin, generator=gen).to(torch.bfloat16).cuda()
+    p_ref = c.fwd(x, shape)
+    a_ref = S.silu_fwd(p_ref)
+    p, a = c.fwd_silu(x, shape)
+    assert torch.equal(p, p_ref)
+    assert torch.equal(a, a_ref)
+
+
 def test_mse_gradient_in_bf16(gpu):
     """mse_loss: the loss and the BF16 gradient (normaliser read on the device) against torch in FP32."""
     from xadvcool_amd import surrogate as S

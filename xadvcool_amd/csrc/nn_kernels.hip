@@ -10,7 +10,9 @@
 //   weight grad dW[co][k] = sum_v g[v][co] * in[v + tap(k)][ci(k)], split over
 //             voxel chunks (partials reduced in a fixed order), db[co] = sum_v g.
 // MFMA operand layout (checked on the device): lane l holds A[l%16][8*(l/16)+j],
-// B[8*(l/16)+j][l%16] (j = 0..7) and D[4*(l/16)+j][l%16] (j = 0..3).
+// B[8*(l/16)+j][l%16] (j = 0..7) and D[4*(l/16)+j][l%16] (j = 0..3).  The forward
+// kernels pass the weights as A and the input as B, so D is (channel x voxel) and
+// a lane ends with four consecutive channels of one voxel: one 16-byte store.
 // Activations use SiLU, x * sigmoid(x) with HIP's expf.  Adam follows Kingma &
 // Ba (2015) with bias correction; master weights and moments are FP32.
 #include <hip/hip_runtime.h>
@@ -34,6 +36,11 @@ typedef unsigned short u16x4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ float bf2f(__bf16 h) { return (float)h; }
 __device__ __forceinline__ __bf16 f2bf(float f) { return (__bf16)f; }
+__device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
+__device__ __forceinline__ float dsilu(float x) {
+  const float s = 1.0f / (1.0f + expf(-x));
+  return s * (1.0f + x * (1.0f - s));
+}
 
 struct Geo {
   int X, Y, Z;  // spatial dims of this level
@@ -98,18 +105,17 @@ __global__ __launch_bounds__(256) void k_conv_fwd(const __bf16* __restrict__ in,
     for (int t = 0; t < NT; ++t) {
       const int co = t * 16 + (lane & 15);
       const u16x8 rb = *reinterpret_cast<const u16x8*>(W + (int64_t)co * Kp + k0);
-      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, __builtin_bit_cast(bf16x8, rb), acc[t], 0, 0, 0);
+      acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, rb), a, acc[t], 0, 0, 0);
     }
   }
+  if (!va_ok) return;
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
-    const int co = t * 16 + (lane & 15);
-    const float b = bias ? bias[co] : 0.0f;
+    const int c0 = t * 16 + (lane >> 4) * 4;  // this lane's four channels of voxel va
+    f32x4 o;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t v = v0 + (lane >> 4) * 4 + j;
-      if (v < V) out[v * Cout + co] = acc[t][j] + b;
-    }
+    for (int j = 0; j < 4; ++j) o[j] = acc[t][j] + (bias ? bias[c0 + j] : 0.0f);
+    *reinterpret_cast<f32x4*>(out + va * Cout + c0) = o;
   }
 }
 
@@ -132,7 +138,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const __bf16* __restrict__ i
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks)
     B[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)m * Kp + ks * 32 + kg));
-  const float b = bias ? bias[m] : 0.0f;
+  f32x4 b;  // this lane's four output channels 4 (l / 16) + j
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = bias ? bias[kg / 2 + j] : 0.0f;
   for (int t = 0; t < tiles; ++t) {
     const int64_t v0 = (t0 + t) * 16;
     if (v0 >= V) break;
@@ -150,13 +158,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd16(const __bf16* __restrict__ i
         const int64_t nb = tap_voxel(g, n, x, y, z, tap);
         if (nb >= 0) raw = *reinterpret_cast<const u16x8*>(in + nb * CIN + ci);
       }
-      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw), B[ks], acc, 0, 0, 0);
+      acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[ks], __builtin_bit_cast(bf16x8, raw), acc, 0, 0, 0);
     }
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int64_t v = v0 + (lane >> 4) * 4 + j;
-      if (v < V) out[v * 16 + m] = acc[j] + b;
-    }
+    if (va_ok) *reinterpret_cast<f32x4*>(out + va * 16 + kg / 2) = acc + b;
   }
 }
 
@@ -204,7 +208,7 @@ __device__ __forceinline__ void load_halo(const __bf16* __restrict__ in, u16x8 (
 template <int CIN, int YB>
 __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
                                                         const float* __restrict__ bias, float* __restrict__ out,
-                                                        Geo g, int Kp) {
+                                                        __bf16* __restrict__ act, Geo g, int Kp) {
   constexpr int K = 27 * CIN, KS = (K + 31) / 32;
   constexpr int HY = YB + 2, HZ = 34, HV = 3 * HY * HZ;
   constexpr int CH = CIN / 8;                 // 16-byte chunks per voxel
@@ -232,7 +236,9 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks)
     B[ks] = __builtin_bit_cast(bf16x8, *reinterpret_cast<const u16x8*>(W + (int64_t)m * Kp + ks * 32 + kg));
-  const float b = bias ? bias[m] : 0.0f;
+  f32x4 b;  // this lane's four output channels 4 (l / 16) + j
+#pragma unroll
+  for (int j = 0; j < 4; ++j) b[j] = bias ? bias[kg / 2 + j] : 0.0f;
   __syncthreads();
   const int64_t vblock = (int64_t)blockIdx.x * (YB * 32);
   // The wave's tiles are computed G at a time with independent accumulators, and the A operands of
@@ -273,14 +279,20 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
       }
 #pragma unroll
       for (int q = 0; q < G; ++q)
-        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw[cur][q]), B[ks], acc[q],
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(B[ks], __builtin_bit_cast(bf16x8, raw[cur][q]), acc[q],
                                                          0, 0, 0);
     }
 #pragma unroll
     for (int q = 0; q < G; ++q) {
-      const int64_t v0 = vblock + (wave * TPW + t0 + q) * 16;
+      const int64_t o = (vblock + (wave * TPW + t0 + q) * 16 + m) * 16 + kg / 2;
+      const f32x4 pv = acc[q] + b;
+      *reinterpret_cast<f32x4*>(out + o) = pv;
+      if (act) {
+        u16x4 a;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * 16 + m] = acc[q][j] + b;
+        for (int j = 0; j < 4; ++j) a[j] = __builtin_bit_cast(unsigned short, f2bf(silu(pv[j])));
+        *reinterpret_cast<u16x4*>(act + o) = a;
+      }
     }
   }
 }
@@ -296,8 +308,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
 // outputs are the same bits.
 template <int CIN, int NT, int ZL, int YB>
 __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
-                                                      const float* __restrict__ bias, float* __restrict__ out, Geo g,
-                                                      int Kp) {
+                                                      const float* __restrict__ bias, float* __restrict__ out,
+                                                      __bf16* __restrict__ act, Geo g, int Kp) {
   constexpr int K = 27 * CIN, KS = (K + 31) / 32, COUT = NT * 16;
   constexpr int HY = YB + 2, HZ = ZL + 2, HV = 3 * HY * HZ;
   constexpr int CH = CIN / 8;
@@ -363,7 +375,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
     for (int tl = 0; tl < TW; ++tl)
 #pragma unroll
       for (int t = 0; t < GW; ++t)
-        acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, raw[tl]), Bt[t],
+        acc[tl][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(Bt[t], __builtin_bit_cast(bf16x8, raw[tl]),
                                                             acc[tl][t], 0, 0, 0);
   };
   bf16x8 B0[GW], B1[GW];
@@ -380,13 +392,21 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
   const int64_t vblock = (int64_t)blockIdx.x * (YB * ZL);
 #pragma unroll
   for (int t = 0; t < GW; ++t) {
-    const int co = group_of(t) * 16 + m;
-    const float b = bias ? bias[co] : 0.0f;
+    const int c0 = group_of(t) * 16 + kg / 2;  // this lane's four output channels
+    f32x4 b;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) b[j] = bias ? bias[c0 + j] : 0.0f;
 #pragma unroll
     for (int tl = 0; tl < TW; ++tl) {
-      const int64_t v0 = vblock + tile_of(tl) * 16;
+      const int64_t o = (vblock + tile_of(tl) * 16 + m) * COUT + c0;
+      const f32x4 pv = acc[tl][t] + b;
+      *reinterpret_cast<f32x4*>(out + o) = pv;
+      if (act) {
+        u16x4 a;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) out[(v0 + (lane >> 4) * 4 + j) * COUT + co] = acc[tl][t][j] + b;
+        for (int j = 0; j < 4; ++j) a[j] = __builtin_bit_cast(unsigned short, f2bf(silu(pv[j])));
+        *reinterpret_cast<u16x4*>(act + o) = a;
+      }
     }
   }
 }
@@ -457,8 +477,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_sk(const __bf16* __restrict__ 
         for (int t = 0; t < MT; ++t)
 #pragma unroll
           for (int u = 0; u < NT; ++u)
-            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, a[s][t]),
-                                                                __builtin_bit_cast(bf16x8, b[s][u]), acc[t][u], 0, 0, 0);
+            acc[t][u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(bf16x8, b[s][u]),
+                                                                __builtin_bit_cast(bf16x8, a[s][t]), acc[t][u], 0, 0, 0);
       }
     }
   }
@@ -478,13 +498,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd_sk(const __bf16* __restrict__ 
       sm += red[0][t][u][lane];
       sm += red[1][t][u][lane];
       sm += red[2][t][u][lane];
-      const int co = co0 + u * 16 + m;
-      const float bb = bias ? bias[co] : 0.0f;
+      const int c0 = co0 + u * 16 + kg / 2;  // this lane's four channels of voxel v
+      const int64_t v = v0 + t * 16 + m;
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int64_t v = v0 + t * 16 + (lane >> 4) * 4 + j;
-        if (v < V) out[v * Cout + co] = sm[j] + bb;
-      }
+      for (int j = 0; j < 4; ++j) sm[j] += bias ? bias[c0 + j] : 0.0f;
+      if (v < V) *reinterpret_cast<f32x4*>(out + v * Cout + c0) = sm;
     }
 }
 
@@ -875,11 +893,6 @@ __global__ void k_bias_final(const float* __restrict__ part, float* __restrict__
 // ------------------------------------------------------------ elementwise ---
 
 
-__device__ __forceinline__ float silu(float x) { return x / (1.0f + expf(-x)); }
-__device__ __forceinline__ float dsilu(float x) {
-  const float s = 1.0f / (1.0f + expf(-x));
-  return s * (1.0f + x * (1.0f - s));
-}
 
 __global__ void k_silu_fwd(const float* __restrict__ x, __bf16* __restrict__ y, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1233,24 +1246,43 @@ inline bool vec_ok(int64_t nelem, std::initializer_list<const void*> ptrs) {
 
 extern "C" {
 
+}  // extern "C"
+namespace {
+// The convolution entry points' body; act (optional) receives bf16(SiLU(out)) from the LDS-halo kernels'
+// epilogue, or from a separate pass after the other kernels.
+int conv_fwd(const void* in, const void* W, const float* bias, float* out, __bf16* act, int N, int X, int Y, int Z,
+             int Cin, int Cout, int Kp, hipStream_t s);
+}  // namespace
+extern "C" {
 int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, int N, int X, int Y, int Z, int Cin,
                  int Cout, int Kp, void* stream) {
+  return conv_fwd(in, W, bias, out, nullptr, N, X, Y, Z, Cin, Cout, Kp, (hipStream_t)stream);
+}
+int xnn_conv_fwd_silu(const void* in, const void* W, const float* bias, float* out, void* act, int N, int X, int Y,
+                      int Z, int Cin, int Cout, int Kp, void* stream) {
+  if (!act) return 1;
+  return conv_fwd(in, W, bias, out, (__bf16*)act, N, X, Y, Z, Cin, Cout, Kp, (hipStream_t)stream);
+}
+}  // extern "C"
+namespace {
+int conv_fwd_kernels(const void* in, const void* W, const float* bias, float* out, __bf16* act, bool& fused, int N,
+                     int X, int Y, int Z, int Cin, int Cout, int Kp, hipStream_t s) {
+  fused = true;
   if (Cin % 16 || Cout % 16 || Cout > 128 || Kp % 32 || Kp < 27 * Cin) return 1;
   Geo g{X, Y, Z, N};
   const int64_t V = g.V();
   const unsigned grid = (unsigned)((V + 63) / 64);
-  hipStream_t s = (hipStream_t)stream;
   const __bf16* i = (const __bf16*)in;
   const __bf16* w = (const __bf16*)W;
   if (Cout == 16 && Cin == 16 && Kp == (27 * Cin + 31) / 32 * 32 && Z == 32 && Y % 16 == 0) {
     // one block per (n, x) and 16 y columns: V / 512 blocks
-    hipLaunchKernelGGL((k_conv_fwd16_lds<16, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias, out, g,
+    hipLaunchKernelGGL((k_conv_fwd16_lds<16, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias, out, act, g,
                        Kp);
     return hipGetLastError() == hipSuccess ? 0 : 2;
   }
   if (Cout == 16 && Cin == 32 && Kp == (27 * Cin + 31) / 32 * 32 && Z == 32 && Y % 8 == 0) {
     // one block per (n, x) and 8 y columns: V / 256 blocks
-    hipLaunchKernelGGL((k_conv_fwd16_lds<32, 8>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias, out, g,
+    hipLaunchKernelGGL((k_conv_fwd16_lds<32, 8>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias, out, act, g,
                        Kp);
     return hipGetLastError() == hipSuccess ? 0 : 2;
   }
@@ -1258,50 +1290,51 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
     // LDS-halo variants of k_conv_fwd<2> and k_conv_fwd16<32> for the half- and full-resolution layers
     if (Cout == 32 && Cin == 16 && Z == 32) {
       hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 32, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 32 && Cin == 16 && Z == 16) {
       hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 32 && Cin == 32 && Z == 16) {
       hipLaunchKernelGGL((k_conv_fwd_lds<32, 2, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 16 && Cin == 32 && Z == 16) {
       hipLaunchKernelGGL((k_conv_fwd_lds<32, 1, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 64 && Cin == 32 && Z == 16) {
       hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 32 && Cin == 64 && Z == 16) {  // 4 y columns: the 64-channel halo fits the LDS limit
       hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 16, 4>), dim3((unsigned)(V / 64)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Z == 8 && Cin == 32 && Cout == 64) {
       hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 8, 16>), dim3((unsigned)(V / 128)), dim3(256), 0, s, i, w, bias,
-                         out, g, Kp);
+                         out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Z == 8 && Cin == 64 && (Cout == 32 || Cout == 64 || Cout == 128)) {  // 8 y columns per block
       const dim3 grid((unsigned)(V / 64));
       if (Cout == 32)
-        hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
       else if (Cout == 64)
-        hipLaunchKernelGGL((k_conv_fwd_lds<64, 4, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 4, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
       else
-        hipLaunchKernelGGL((k_conv_fwd_lds<64, 8, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, g, Kp);
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 8, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
   }
+  fused = false;  // the kernels below have no SiLU epilogue
   if (Cin >= 64 && (Z <= 4 || Cin == 128)) {
     // the coarsest level (Z <= 4) and 128 input channels: split-K over the waves of a block
     // tiles per block: as many as keep at least 512 blocks (two per CU) in the launch
@@ -1340,6 +1373,15 @@ int xnn_conv_fwd(const void* in, const void* W, const float* bias, float* out, i
   }
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
+int conv_fwd(const void* in, const void* W, const float* bias, float* out, __bf16* act, int N, int X, int Y, int Z,
+             int Cin, int Cout, int Kp, hipStream_t s) {
+  bool fused = false;
+  const int rc = conv_fwd_kernels(in, W, bias, out, act, fused, N, X, Y, Z, Cin, Cout, Kp, s);
+  if (rc != 0 || !act || fused) return rc;
+  return xnn_silu_fwd(out, act, (int64_t)N * X * Y * Z * Cout, s);
+}
+}  // namespace
+extern "C" {
 
 int xnn_wflip(const void* W, void* Wt, int Cin, int Cout, int Kp, int Kpt, void* stream) {
   if (Kpt % 32 || Kpt < 27 * Cout) return 1;

@@ -51,6 +51,7 @@ def lib():
         L = ctypes.CDLL(NN_LIB)
         vp, i, i64, f, fp = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_float, ctypes.c_void_p
         L.xnn_conv_fwd.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
+        L.xnn_conv_fwd_silu.argtypes = [vp, vp, vp, vp, vp, i, i, i, i, i, i, i, vp]
         L.xnn_wflip.argtypes = [vp, vp, i, i, i, i, vp]
         L.xnn_conv_wgrad.argtypes = [vp, vp, vp, vp, i, i, i, i, i, i, i, i, vp]
         L.xnn_wgrad_part_floats.argtypes = [i, i, i, i, i, i, i, i]
@@ -112,6 +113,16 @@ def conv_fwd(x, W, b, shape, cin, cout, Kp):
     out = torch.empty((N, X, Y, Z, cout), dtype=torch.float32, device=x.device)
     _ok(lib().xnn_conv_fwd(_p(x), _p(W), _p(b), _p(out), N, X, Y, Z, cin, cout, Kp, _st()), "xnn_conv_fwd")
     return out
+
+
+def conv_fwd_silu(x, W, b, shape, cin, cout, Kp):
+    """conv_fwd and bf16(SiLU) of its output in one call: returns (fp32 output, bf16 activation)."""
+    N, X, Y, Z = shape
+    out = torch.empty((N, X, Y, Z, cout), dtype=torch.float32, device=x.device)
+    act = torch.empty((N, X, Y, Z, cout), dtype=torch.bfloat16, device=x.device)
+    _ok(lib().xnn_conv_fwd_silu(_p(x), _p(W), _p(b), _p(out), _p(act), N, X, Y, Z, cin, cout, Kp, _st()),
+        "xnn_conv_fwd_silu")
+    return out, act
 
 
 def to_bf16(x):
@@ -343,6 +354,10 @@ class Conv(_Grads):
     def fwd(self, x, shape):
         return conv_fwd(x, self.wb, self.b, shape, self.cin, self.cout, self.Kp)
 
+    def fwd_silu(self, x, shape):
+        """(conv output, bf16 SiLU of it): the same values as fwd then silu_fwd."""
+        return conv_fwd_silu(x, self.wb, self.b, shape, self.cin, self.cout, self.Kp)
+
     def dgrad(self, g_bf16, shape):
         """Gradient w.r.t. the input (fp32) of an upstream gradient g (bf16, w.r.t. this conv's output)."""
         L = lib()
@@ -444,39 +459,42 @@ class UNet3D:
         t["film"] = film
         off = {name: o for name, o, _ in self.FILM}
 
-        def act(name, p):  # FiLM (at its sites) then SiLU; the tape keeps the conv output
-            t[name] = p
-            return film_silu_fwd(p, film, off[name]) if name in off else silu_fwd(p)
+        def act(name, conv, x_in, shape):  # conv, then FiLM (at its sites) and SiLU; the tape keeps the conv output
+            if name in off:
+                t[name] = cv[conv].fwd(x_in, shape)
+                return film_silu_fwd(t[name], film, off[name])
+            t[name], a = cv[conv].fwd_silu(x_in, shape)
+            return a
 
         # encoder
-        t["a1a"] = act("p1a", cv["c1a"].fwd(x, sh[0]))
-        t["s1"] = act("p1b", cv["c1b"].fwd(t["a1a"], sh[0]))
+        t["a1a"] = act("p1a", "c1a", x, sh[0])
+        t["s1"] = act("p1b", "c1b", t["a1a"], sh[0])
         t["q1"] = pool_fwd(t["s1"], sh[1])
-        t["a2a"] = act("p2a", cv["c2a"].fwd(t["q1"], sh[1]))
-        t["s2"] = act("p2b", cv["c2b"].fwd(t["a2a"], sh[1]))
+        t["a2a"] = act("p2a", "c2a", t["q1"], sh[1])
+        t["s2"] = act("p2b", "c2b", t["a2a"], sh[1])
         t["q2"] = pool_fwd(t["s2"], sh[2])
-        t["a3a"] = act("p3a", cv["c3a"].fwd(t["q2"], sh[2]))
-        t["s3"] = act("p3b", cv["c3b"].fwd(t["a3a"], sh[2]))
+        t["a3a"] = act("p3a", "c3a", t["q2"], sh[2])
+        t["s3"] = act("p3b", "c3b", t["a3a"], sh[2])
         t["q3"] = pool_fwd(t["s3"], sh[3])
-        t["a4a"] = act("p4a", cv["c4a"].fwd(t["q3"], sh[3]))
-        t["a4b"] = act("p4b", cv["c4b"].fwd(t["a4a"], sh[3]))
+        t["a4a"] = act("p4a", "c4a", t["q3"], sh[3])
+        t["a4b"] = act("p4b", "c4b", t["a4a"], sh[3])
         # residual block at the bottleneck
-        t["a4c"] = act("p4c", cv["c4c"].fwd(t["a4b"], sh[3]))
-        t["a4d"] = act("p4d", cv["c4d"].fwd(t["a4c"], sh[3]))
+        t["a4c"] = act("p4c", "c4c", t["a4b"], sh[3])
+        t["a4d"] = act("p4d", "c4d", t["a4c"], sh[3])
         t["r4"] = add_bf16(t["a4b"], t["a4d"])
         # decoder
         t["u3"] = up_fwd(t["r4"], sh[2])
-        t["ad3a"] = act("pd3a", cv["d3a"].fwd(t["u3"], sh[2]))
+        t["ad3a"] = act("pd3a", "d3a", t["u3"], sh[2])
         t["k3"] = add_bf16(t["ad3a"], t["s3"])
-        t["ad3b"] = act("pd3b", cv["d3b"].fwd(t["k3"], sh[2]))
+        t["ad3b"] = act("pd3b", "d3b", t["k3"], sh[2])
         t["u2"] = up_fwd(t["ad3b"], sh[1])
-        t["ad2a"] = act("pd2a", cv["d2a"].fwd(t["u2"], sh[1]))
+        t["ad2a"] = act("pd2a", "d2a", t["u2"], sh[1])
         t["k2"] = add_bf16(t["ad2a"], t["s2"])
-        t["ad2b"] = act("pd2b", cv["d2b"].fwd(t["k2"], sh[1]))
+        t["ad2b"] = act("pd2b", "d2b", t["k2"], sh[1])
         t["u1"] = up_fwd(t["ad2b"], sh[0])
-        t["ad1a"] = act("pd1a", cv["d1a"].fwd(t["u1"], sh[0]))
+        t["ad1a"] = act("pd1a", "d1a", t["u1"], sh[0])
         t["k1"] = add_bf16(t["ad1a"], t["s1"])
-        t["ad1b"] = act("pd1b", cv["d1b"].fwd(t["k1"], sh[0]))
+        t["ad1b"] = act("pd1b", "d1b", t["k1"], sh[0])
         out = cv["head"].fwd(t["ad1b"], sh[0])
         if keep:
             self._tape = (t, sh)
