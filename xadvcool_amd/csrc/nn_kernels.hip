@@ -205,7 +205,7 @@ __device__ __forceinline__ void load_halo(const __bf16* __restrict__ in, u16x8 (
 // fragments, exactly the MFMA sequence of k_conv_fwd16<CIN> per tile, so the
 // outputs are the same bits.  Halo: 3 * (YB + 2) * 34 voxels of CIN BF16
 // channels (CIN 16, YB 16: 58,752 bytes; CIN 32, YB 8: 65,280 bytes).
-template <int CIN, int YB>
+template <int CIN, int YB, bool PERSIST>
 __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
                                                         const float* __restrict__ bias, float* __restrict__ out,
                                                         __bf16* __restrict__ act, Geo g, int Kp) {
@@ -220,18 +220,25 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
   __shared__ __attribute__((aligned(16))) __bf16 halo[HV * CIN];
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = lane & 15, kg = (lane >> 4) * 8;
-  const int ybl = g.Y / YB;  // blocks per (n, x)
-  const int nx = blockIdx.x / ybl;
-  const int y0 = (blockIdx.x - nx * ybl) * YB;
-  const int n = nx / g.X, x = nx - n * g.X;
-  // Stage the halo: every load of this thread is issued before any store.
+  const int ybl = g.Y / YB;  // work items per (n, x)
+  const int nitems = g.N * g.X * ybl;
+  // Work item `it`: one (n, x) and YB y columns from y0.  PERSIST: the grid is persistent and a block's
+  // next item's halo is loaded into registers while it computes the current one (more registers).
   u16x8 tmp[kPer];
-  load_halo<CH, HY, HZ>(in, tmp, g, n, x, y0);
+  auto fetch = [&](int it) {
+    const int nx = it / ybl, y0 = (it - nx * ybl) * YB;
+    const int n = nx / g.X, x = nx - n * g.X;
+    load_halo<CH, HY, HZ>(in, tmp, g, n, x, y0);
+  };
+  auto stage = [&]() {
 #pragma unroll
-  for (int i = 0; i < kPer; ++i) {
-    const int e = threadIdx.x + i * 256;
-    if (e < kChunks) *reinterpret_cast<u16x8*>(halo + e * 8) = tmp[i];
-  }
+    for (int i = 0; i < kPer; ++i) {
+      const int e = threadIdx.x + i * 256;
+      if (e < kChunks) *reinterpret_cast<u16x8*>(halo + e * 8) = tmp[i];
+    }
+  };
+  if ((int)blockIdx.x < nitems) fetch(blockIdx.x);
+  if (!PERSIST) stage();  // before the weights are loaded: the staging registers are free again
   bf16x8 B[KS];
 #pragma unroll
   for (int ks = 0; ks < KS; ++ks)
@@ -239,8 +246,11 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
   f32x4 b;  // this lane's four output channels 4 (l / 16) + j
 #pragma unroll
   for (int j = 0; j < 4; ++j) b[j] = bias ? bias[kg / 2 + j] : 0.0f;
+  for (int it = blockIdx.x; it < nitems; it = PERSIST ? it + (int)gridDim.x : nitems) {  // else one item per block
+  if (PERSIST) stage();
   __syncthreads();
-  const int64_t vblock = (int64_t)blockIdx.x * (YB * 32);
+  if (PERSIST && it + (int)gridDim.x < nitems) fetch(it + gridDim.x);
+  const int64_t vblock = (int64_t)it * (YB * 32);
   // The wave's tiles are computed G at a time with independent accumulators, and the A operands of
   // the next K-step are read from LDS before this step's MFMAs, so neither the LDS latency nor the
   // accumulator dependency of one tile's MFMA chain stalls the wave.  Each tile still gets the MFMA
@@ -295,6 +305,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
       }
     }
   }
+  __syncthreads();
+  }
 }
 
 // CIN -> NT * 16 convolution for lattices with Z = ZL and Y a multiple of YB:
@@ -306,7 +318,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd16_lds(const __bf16* __restrict
 // four tiles: the waves share the tiles and split the output channels): every
 // output element gets the same MFMA sequence as in k_conv_fwd<NT>, so the
 // outputs are the same bits.
-template <int CIN, int NT, int ZL, int YB>
+template <int CIN, int NT, int ZL, int YB, bool PERSIST>
 __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__ in, const __bf16* __restrict__ W,
                                                       const float* __restrict__ bias, float* __restrict__ out,
                                                       __bf16* __restrict__ act, Geo g, int Kp) {
@@ -322,17 +334,23 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
   const int lane = threadIdx.x & 63, wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int m = lane & 15, kg = (lane >> 4) * 8;
   const int ybl = g.Y / YB;
-  const int nx = blockIdx.x / ybl;
-  const int y0 = (blockIdx.x - nx * ybl) * YB;
-  const int n = nx / g.X, x = nx - n * g.X;
+  const int nitems = g.N * g.X * ybl;
+  // Work items and PERSIST as in k_conv_fwd16_lds.
   u16x8 tmp[kPer];
-  load_halo<CH, HY, HZ>(in, tmp, g, n, x, y0);
+  auto fetch = [&](int it) {
+    const int nx = it / ybl, y0 = (it - nx * ybl) * YB;
+    const int n = nx / g.X, x = nx - n * g.X;
+    load_halo<CH, HY, HZ>(in, tmp, g, n, x, y0);
+  };
+  if ((int)blockIdx.x < nitems) fetch(blockIdx.x);
+  for (int it = blockIdx.x; it < nitems; it = PERSIST ? it + (int)gridDim.x : nitems) {  // else one item per block
 #pragma unroll
   for (int i = 0; i < kPer; ++i) {
     const int e = threadIdx.x + i * 256;
     if (e < kChunks) *reinterpret_cast<u16x8*>(halo + e * 8) = tmp[i];
   }
   __syncthreads();
+  if (PERSIST && it + (int)gridDim.x < nitems) fetch(it + gridDim.x);
   // The wave's share: TW tiles x GW 16-channel output groups.  With 4 tiles per block (one per wave) the
   // waves take all tiles and split the output groups instead, so that each B fragment (a load from global
   // memory) serves every tile of the block; every output element still gets the same MFMA sequence.
@@ -389,7 +407,7 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
       mfmas(B1, R1);
     }
   }
-  const int64_t vblock = (int64_t)blockIdx.x * (YB * ZL);
+  const int64_t vblock = (int64_t)it * (YB * ZL);
 #pragma unroll
   for (int t = 0; t < GW; ++t) {
     const int c0 = group_of(t) * 16 + kg / 2;  // this lane's four output channels
@@ -408,6 +426,8 @@ __global__ __launch_bounds__(256) void k_conv_fwd_lds(const __bf16* __restrict__
         *reinterpret_cast<u16x4*>(act + o) = a;
       }
     }
+  }
+  __syncthreads();
   }
 }
 
@@ -1232,6 +1252,21 @@ __global__ void k_adam(float* __restrict__ w, const float* __restrict__ g, float
 
 inline unsigned nblk(int64_t n, int bs = 256) { return (unsigned)((n + bs - 1) / bs); }
 
+// Blocks of a persistent launch over `items` work items: per_cu blocks on every CU of the current device
+// (the number its LDS allows), at most one per item.
+unsigned persistent_grid(int64_t items, int per_cu) {
+  static int cus[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) dev = 0;
+  if (cus[dev] <= 0) {
+    int c = 0;
+    if (hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || c <= 0) c = 256;
+    cus[dev] = c;
+  }
+  const int64_t g = (int64_t)cus[dev] * per_cu;
+  return (unsigned)(items < g ? items : g);
+}
+
 // The vectorised elementwise kernels index elements with 32 bits and access 16 bytes at a time.
 inline bool vec_ok(int64_t nelem, std::initializer_list<const void*> ptrs) {
   if (nelem <= 0 || nelem >= (int64_t(1) << 31)) return false;
@@ -1276,61 +1311,61 @@ int conv_fwd_kernels(const void* in, const void* W, const float* bias, float* ou
   const __bf16* w = (const __bf16*)W;
   if (Cout == 16 && Cin == 16 && Kp == (27 * Cin + 31) / 32 * 32 && Z == 32 && Y % 16 == 0) {
     // one block per (n, x) and 16 y columns: V / 512 blocks
-    hipLaunchKernelGGL((k_conv_fwd16_lds<16, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias, out, act, g,
-                       Kp);
+    hipLaunchKernelGGL((k_conv_fwd16_lds<16, 16, true>), dim3(persistent_grid(V / 512, 2)), dim3(256), 0, s, i, w,
+                       bias, out, act, g, Kp);
     return hipGetLastError() == hipSuccess ? 0 : 2;
   }
   if (Cout == 16 && Cin == 32 && Kp == (27 * Cin + 31) / 32 * 32 && Z == 32 && Y % 8 == 0) {
     // one block per (n, x) and 8 y columns: V / 256 blocks
-    hipLaunchKernelGGL((k_conv_fwd16_lds<32, 8>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias, out, act, g,
-                       Kp);
+    hipLaunchKernelGGL((k_conv_fwd16_lds<32, 8, false>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+                       out, act, g, Kp);
     return hipGetLastError() == hipSuccess ? 0 : 2;
   }
   if (Kp == (27 * Cin + 31) / 32 * 32 && Y % 16 == 0) {
     // LDS-halo variants of k_conv_fwd<2> and k_conv_fwd16<32> for the half- and full-resolution layers
     if (Cout == 32 && Cin == 16 && Z == 32) {
-      hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 32, 16>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 32, 16, false>), dim3((unsigned)(V / 512)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 32 && Cin == 16 && Z == 16) {
-      hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<16, 2, 16, 16, false>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 32 && Cin == 32 && Z == 16) {
-      hipLaunchKernelGGL((k_conv_fwd_lds<32, 2, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 2, 16, 16, false>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 16 && Cin == 32 && Z == 16) {
-      hipLaunchKernelGGL((k_conv_fwd_lds<32, 1, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 1, 16, 16, false>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 64 && Cin == 32 && Z == 16) {
-      hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 16, 16>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 16, 16, false>), dim3((unsigned)(V / 256)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Cout == 32 && Cin == 64 && Z == 16) {  // 4 y columns: the 64-channel halo fits the LDS limit
-      hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 16, 4>), dim3((unsigned)(V / 64)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 16, 4, false>), dim3((unsigned)(V / 64)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Z == 8 && Cin == 32 && Cout == 64) {
-      hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 8, 16>), dim3((unsigned)(V / 128)), dim3(256), 0, s, i, w, bias,
+      hipLaunchKernelGGL((k_conv_fwd_lds<32, 4, 8, 16, false>), dim3((unsigned)(V / 128)), dim3(256), 0, s, i, w, bias,
                          out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
     if (Z == 8 && Cin == 64 && (Cout == 32 || Cout == 64 || Cout == 128)) {  // 8 y columns per block
       const dim3 grid((unsigned)(V / 64));
       if (Cout == 32)
-        hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 2, 8, 8, false>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
       else if (Cout == 64)
-        hipLaunchKernelGGL((k_conv_fwd_lds<64, 4, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 4, 8, 8, false>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
       else
-        hipLaunchKernelGGL((k_conv_fwd_lds<64, 8, 8, 8>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
+        hipLaunchKernelGGL((k_conv_fwd_lds<64, 8, 8, 8, false>), grid, dim3(256), 0, s, i, w, bias, out, act, g, Kp);
       return hipGetLastError() == hipSuccess ? 0 : 2;
     }
   }
