@@ -62,6 +62,12 @@ typedef struct {
   int alternate_sweep;  /* 1: even passes visit the lattice blocks in reverse order, so a pass
                            starts on the lines the previous pass touched last (still in the
                            Infinity Cache when the batch's state is larger than it).  Same bits. */
+  int thermal_q;        /* thermal lattice: 19 (or 0), the reference's D3Q19; 7, the D3Q7 variant
+                           (no reference counterpart): seven velocities, weights 1/4 and 1/8,
+                           g_eq = w T (1 + 4 e.u), relaxed at the diffusivity of the instance's
+                           tau_th and heated with the reference step's factor 1 - 1/(2 tau_th);
+                           needs the fluid lattice and general = 0.  A step is the fused fluid
+                           pass (storing u) followed by one D3Q7 kernel. */
 } xadv_batch_desc;
 
 #define XADV_STORAGE_AUTO 0

@@ -16,6 +16,11 @@
 //   validate_analytical.py  (the six test-only kernels)
 // and the six-launch step order of src/simulation.py:94-99.
 //
+// It also holds the D3Q7 thermal variant, which has no reference kernel (the
+// reference's thermal lattice is D3Q19): the same collide / stream / moments
+// on the seven velocities 0..6 of E_CONST with weights 1/4 and 1/8, pinned by
+// the closed forms of tests/test_oracle_d3q7.py.
+//
 // Layout is the reference's: populations (19, nx, ny, nz) C-order, fields
 // (nx, ny, nz), velocity (nx, ny, nz, 3), mask int32 (nx, ny, nz).
 // Every kernel reads one buffer and writes another (no in-place update), as
@@ -413,6 +418,130 @@ void thermal_macroscopic(const Dims& d, const R* g, R* T) {
   }
 }
 
+// ------------------------------------------------------ D3Q7 thermal variant ----
+// Not a reference kernel: the reference's thermal lattice is D3Q19.  The same
+// advection-diffusion problem on the rest and six face velocities (directions
+// 0..6 of E_CONST, with their OPPOSITE_CONST pairs), weights 1/4 and 1/8
+// (c_s^2 = 1/4), the linear equilibrium g_eq_i = w_i T (1 + 4 e_i.u) (the
+// quadratic terms of the D3Q19 equilibrium are not isotropic on seven
+// velocities), and the streaming rules of thermal_engine3d.py:147-211 for those
+// seven directions.  The relaxation times passed in are the reference's (D3Q19,
+// diffusivity (tau - 1/2) / 3): the D3Q7 step relaxes with
+// tau7 = 1/2 + 4/3 (tau - 1/2), the same diffusivity, and injects the source as
+// w_i (1 - 1/(2 tau)) q, the heat the D3Q19 step adds per step (the factor
+// depends on tau, so with (1 - 1/(2 tau7)) the two lattices would heat
+// differently).  Planted mistakes: 1 drops (1 - omega/2) from the
+// source, 2 bounces back the same direction at walls, 4 takes 1 + 3 e.u (the
+// D3Q19 factor), 5 takes the weights 1/3 and 1/9.
+
+template <class R> struct Wt7 {
+  R w[7];
+  Wt7() {
+    const bool m5 = g_mutation == 5;
+    w[0] = m5 ? (R)(1.0 / 3.0) : (R)0.25;
+    for (int i = 1; i < 7; ++i) w[i] = m5 ? (R)(1.0 / 9.0) : (R)0.125;
+  }
+};
+
+template <class R>
+inline R geq7(const Wt7<R>& W, int i, R T, R ux, R uy, R uz) {
+  const R eu = (R)EX[i] * ux + (R)EY[i] * uy + (R)EZ[i] * uz;
+  return W.w[i] * T * ((R)1.0 + (g_mutation == 4 ? (R)3.0 : (R)4.0) * eu);
+}
+
+template <class R>
+void thermal_equilibrium7(const Dims& d, const R* u, const R* T, R* geq, const int* mask) {
+  const Wt7<R> W;
+  const int64_t N = d.n();
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < N; ++c) {
+    R ux = u[3 * c], uy = u[3 * c + 1], uz = u[3 * c + 2];
+    if (mask[c] == BC_SOLID) ux = uy = uz = 0;
+    for (int i = 0; i < 7; ++i) geq[i * N + c] = geq7(W, i, T[c], ux, uy, uz);
+  }
+}
+
+template <class R>
+void fused_thermal_collision7(const Dims& d, const R* go, const R* u, const R* T, R* gn, const int* mask, R tau_f,
+                              R tau_s, const R* hs, R power) {
+  const Wt7<R> W;
+  const int64_t N = d.n();
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < N; ++c) {
+    const int m = mask[c];
+    R tau = (m == BC_SOLID) ? tau_s : tau_f;
+    R omega = (R)1.0 / ((R)0.5 + (tau - (R)0.5) * (R)(4.0 / 3.0));
+    R sf = (R)1.0 - (R)0.5 * ((R)1.0 / tau);
+    R q = hs[c] * power;
+    R ux = u[3 * c], uy = u[3 * c + 1], uz = u[3 * c + 2];
+    if (m == BC_SOLID) ux = uy = uz = 0;
+    for (int i = 0; i < 7; ++i) {
+      R v = go[i * N + c] - omega * (go[i * N + c] - geq7(W, i, T[c], ux, uy, uz));
+      if (q > (R)0.0) v += (g_mutation == 1) ? W.w[i] * q : W.w[i] * sf * q;
+      gn[i * N + c] = v;
+    }
+  }
+}
+
+template <class R>
+void thermal_streaming7(const Dims& d, const R* gpc, R* gs, const int* mask, R T_in) {
+  const Wt7<R> W;
+  const int64_t N = d.n();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int x = 0; x < d.nx; ++x)
+    for (int y = 0; y < d.ny; ++y)
+      for (int z = 0; z < d.nz; ++z) {
+        const int64_t c = d.c(x, y, z);
+        const int m = mask[c];
+        if (m == BC_INLET) {
+          for (int i = 0; i < 7; ++i) gs[i * N + c] = W.w[i] * T_in;
+          continue;
+        }
+        if (m == BC_OUTLET) {
+          const int64_t s = d.c(x > 0 ? x - 1 : 0, y, z);
+          for (int i = 0; i < 7; ++i) gs[i * N + c] = gpc[i * N + s];
+          continue;
+        }
+        for (int i = 0; i < 7; ++i) {
+          int sx = x - EX[i], sy = y - EY[i], sz = z - EZ[i];
+          if (sx < 0 || sx >= d.nx || sy < 0 || sy >= d.ny || sz < 0 || sz >= d.nz) {
+            if (sx < 0)
+              gs[i * N + c] = W.w[i] * T_in;
+            else if (sx >= d.nx)
+              gs[i * N + c] = gpc[i * N + c];
+            else
+              gs[i * N + c] = gpc[(g_mutation == 2 ? i : OPP[i]) * N + c];
+          } else {
+            gs[i * N + c] = gpc[i * N + d.c(sx, sy, sz)];
+          }
+        }
+      }
+}
+
+template <class R>
+void periodic_thermal_streaming7(const Dims& d, const R* gpc, R* gs) {
+  const int64_t N = d.n();
+#pragma omp parallel for collapse(2) schedule(static)
+  for (int x = 0; x < d.nx; ++x)
+    for (int y = 0; y < d.ny; ++y)
+      for (int z = 0; z < d.nz; ++z)
+        for (int i = 0; i < 7; ++i) {
+          const int sx = (x - EX[i] + d.nx) % d.nx, sy = (y - EY[i] + d.ny) % d.ny, sz = (z - EZ[i] + d.nz) % d.nz;
+          gs[i * N + d.c(x, y, z)] = gpc[i * N + d.c(sx, sy, sz)];
+        }
+}
+
+template <class R>
+void thermal_macroscopic7(const Dims& d, const R* g, R* T) {
+  const int64_t N = d.n();
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < N; ++c) {
+    R s = 0;
+    for (int i = 0; i < 7; ++i) s += g[i * N + c];
+    T[c] = s;
+  }
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------ C ABI ----
@@ -435,7 +564,9 @@ typedef struct {
   // thermal: 0 off; 1 = simulation.py schedule (fused thermal collide with the
   //          velocity just computed, stream, macro); 2 = validate_analytical.py
   //          schedule (thermal equilibrium, thermal BGK, stream, [dirichlet], macro)
-  //          with a frozen velocity field.
+  //          with a frozen velocity field; 3 = schedule 1 on the D3Q7 variant
+  //          (populations in planes 0..6 of g); 4 = the D3Q7 collide, stream
+  //          (thermal_stream 1: periodic) and macro with the velocity field as given.
   int thermal;
   int thermal_stream;  // 0 thermal_streaming_kernel, 1 periodic_thermal_streaming_kernel
   int dirichlet_x;     // -1: none; else dirichlet_thermal_bc_kernel plane after streaming
@@ -444,7 +575,8 @@ typedef struct {
   // Planted mistakes, for tests that show the closed-form cases catch them:
   // 1 drops the (1 - omega/2) factor of the thermal source, 2 bounces back
   // the same direction instead of the opposite one, 3 flips the sign of the
-  // moving-wall momentum.  0 = the reference.
+  // moving-wall momentum; D3Q7: 4 takes 1 + 3 e.u in the equilibrium, 5 the
+  // weights 1/3 and 1/9.  0 = the reference.
   int mutation;
 } orc_cfg;
 }  // extern "C"
@@ -493,6 +625,13 @@ static int run_steps(const orc_cfg* cfg, int nsteps, const int* mask, const R* h
         thermal_streaming<R>(d, g_new, g_old, mask, Tin);
       if (cfg->dirichlet_x >= 0) dirichlet_plane<R>(d, g_old, (R)cfg->dirichlet_T, cfg->dirichlet_x);
       thermal_macroscopic<R>(d, g_old, T);
+    } else if (cfg->thermal == 3 || cfg->thermal == 4) {
+      fused_thermal_collision7<R>(d, g_old, u, T, g_new, mask, ttf, tts, hs, P);
+      if (cfg->thermal == 4 && cfg->thermal_stream == 1)
+        periodic_thermal_streaming7<R>(d, g_new, g_old);
+      else
+        thermal_streaming7<R>(d, g_new, g_old, mask, Tin);
+      thermal_macroscopic7<R>(d, g_old, T);
     }
   }
   return 0;
@@ -522,6 +661,14 @@ void orc_thermal_equilibrium_f32(int nx, int ny, int nz, const float* u, const f
 void orc_thermal_equilibrium_f64(int nx, int ny, int nz, const double* u, const double* T, double* geq,
                                  const int* mask) {
   thermal_equilibrium<double>(Dims{nx, ny, nz}, u, T, geq, mask);
+}
+void orc_thermal_equilibrium7_f32(int nx, int ny, int nz, const float* u, const float* T, float* geq,
+                                  const int* mask) {
+  thermal_equilibrium7<float>(Dims{nx, ny, nz}, u, T, geq, mask);
+}
+void orc_thermal_equilibrium7_f64(int nx, int ny, int nz, const double* u, const double* T, double* geq,
+                                  const int* mask) {
+  thermal_equilibrium7<double>(Dims{nx, ny, nz}, u, T, geq, mask);
 }
 void orc_macroscopic_f32(int nx, int ny, int nz, const float* f, float* rho, float* u) {
   macroscopic<float>(Dims{nx, ny, nz}, f, rho, u);

@@ -114,6 +114,18 @@ class State:
         self.g_old[...] = self.g_eq
 
 
+def init_thermal7(state: State, T=None, mask=None):
+    """D3Q7 variant: g_eq7(T, u) into planes 0..6 of g_old (the initial state of the D3Q7 schedules,
+    ``thermal=3`` and ``4`` of ``run``)."""
+    if T is not None:
+        state.T[...] = T
+    nx, ny, nz = state.shape
+    m = np.ascontiguousarray(mask if mask is not None else np.zeros(state.shape, np.int32), dtype=np.int32)
+    getattr(lib(), "orc_thermal_equilibrium7_" + _sfx(state.dtype))(
+        nx, ny, nz, _p(state.u), _p(state.T), _p(state.g_eq), _p(m))
+    state.g_old[...] = state.g_eq
+
+
 def run(state: State, mask, nsteps, *, fluid=0, fluid_stream=0, forced=0, thermal=0, thermal_stream=0,
         dirichlet_x=-1, tau=0.6, force_x=0.0, u_wall=0.0, u_in=0.0, rho_out=1.0,
         tau_th_fluid=0.6, tau_th_solid=0.6, heat_power=0.0, T_in=0.0, dirichlet_T=0.0, heat=None, mutation=0):
@@ -203,8 +215,10 @@ def pump_update(u_in, sum_rho_in, sum_rho_out, c_in, c_out, target_power, mutati
 def run_simulation(mask, max_steps=40000, tau_fluid=0.52, heat_power=0.1, t_inlet=25.0, u_inlet_val=0.03,
                    stats_every=100, tol_ke=1e-7, tol_T=1e-7, pump_mode="constant_flow", target_power=1.0,
                    device="cpu", silent=True, heat_source=None, Pr=7.0, k_ratio=628.0, dtype=np.float32,
-                   min_steps_rule=True, pump_mutation=0):
+                   min_steps_rule=True, pump_mutation=0, thermal_lattice="D3Q19"):
     """src/simulation.py:14-162 on the oracle kernels.
+
+    ``thermal_lattice="D3Q7"`` runs the oracle's D3Q7 thermal variant (schedule 3 of ``run``) in the same loop.
 
     ``min_steps_rule=False`` drops the flow-through floor (simulation.py:104);
     everything else is the reference loop.  ``pump_mutation`` plants a pump
@@ -219,7 +233,11 @@ def run_simulation(mask, max_steps=40000, tau_fluid=0.52, heat_power=0.1, t_inle
     st.T[...] = t_inlet
     ttf, tts = thermal_taus(tau_fluid, Pr, k_ratio)
     hm = normalized_heat(heat_source, (nx, ny, nz))
-    st.init_thermal(mask=mask)
+    th = 3 if thermal_lattice == "D3Q7" else 1
+    if th == 3:
+        init_thermal7(st, mask=mask)
+    else:
+        st.init_thermal(mask=mask)
     st.init_fluid()
     min_steps = min(int(0.6 * max_steps), max(5000, int(2.0 * nx / max(u_inlet_val, 1e-6))))
     if not min_steps_rule:
@@ -232,7 +250,7 @@ def run_simulation(mask, max_steps=40000, tau_fluid=0.52, heat_power=0.1, t_inle
     while step < max_steps:
         n = stats_every - (step % stats_every)
         n = min(n, max_steps - step)
-        run(st, mask, n, fluid=1, thermal=1, tau=tau_fluid, u_in=float(u_in), rho_out=1.0,
+        run(st, mask, n, fluid=1, thermal=th, tau=tau_fluid, u_in=float(u_in), rho_out=1.0,
             tau_th_fluid=ttf, tau_th_solid=tts, heat_power=heat_power, T_in=t_inlet, heat=hm)
         step += n
         if step % stats_every == 0:

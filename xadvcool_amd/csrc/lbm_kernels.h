@@ -69,6 +69,7 @@ struct Slot {
   R* rho;
   R* u;  // (ncell, 3)
   R* T;
+  R* g7;  // D3Q7 thermal variant: [2][7][npad], the two buffers a step reads and writes (null otherwise)
 };
 
 // Lattice block of this launch block (block order bits: lattice.h,
@@ -877,6 +878,140 @@ __global__ void k_fill(R* __restrict__ p, int64_t n, R v) {
   if (i < n) p[i] = v;
 }
 
+// ---------------------------------------------------------- D3Q7 thermal ----
+// D3Q7 variant of the thermal lattice.  Not a reference kernel: the
+// reference's thermal lattice is D3Q19.  It solves the same advection-diffusion
+// problem on directions 0..6 of the D3Q19 set (rest and the six faces, with
+// their opposite pairs), weights 1/4 and 1/8 (c_s^2 = 1/4), the linear
+// equilibrium g_eq_i = w_i T (1 + 4 e_i.u) (the quadratic terms are not
+// isotropic on seven velocities).  A slot's relaxation times are the
+// reference's: omega_tf / omega_ts hold 1/tau7 with tau7 = 1/2 + 4/3 (tau - 1/2)
+// (the same diffusivity) and sf_tf / sf_ts the reference step's source factor
+// 1 - 1/(2 tau), so the heat injected per step is the reference's.  Streaming
+// follows thermal_engine3d.py:147-211 for these directions.  Two buffers of 7
+// planes per slot (g7, g7 + 7 npad): odd steps read the first and write the
+// second, even steps the reverse; a step runs after the fluid pass of the same
+// step and collides with the velocity that pass stored (u = 0 in solids).  The
+// first odd step after an init takes the initial g and T as given (raw), as the
+// fused pass's thermal half does; the next even step clears the flag.
+template <class R>
+__device__ __forceinline__ R w7(int i) {
+  return i == 0 ? (R)0.25 : (R)0.125;
+}
+
+// Streamed D3Q7 populations of cell c from the post-collision buffer src.
+template <class R>
+__device__ __forceinline__ void gather7(const Slot<R>& S, const R* __restrict__ src, uint32_t c, uint32_t fl,
+                                        R* g) {
+  const int type = (int)(fl & F_TYPE);
+  const uint32_t N = (uint32_t)S.npad;
+  if (type == BC_INLET) {  // thermal_engine3d.py:160-165
+#pragma unroll
+    for (int i = 0; i < 7; ++i) g[i] = w7<R>(i) * S.T_in;
+    return;
+  }
+  if (type == BC_OUTLET) {  // :170-177 (x - 1, or the cell itself on x = 0)
+    const uint32_t s = (fl & F_FX0) ? c : c - (uint32_t)S.P;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) g[i] = ld(src, i * N + s);
+    return;
+  }
+  // :180-211: the source c - e_i; off the lattice: x < 0 -> w_i T_in, x >= nx -> own value, y/z -> opposite.
+  // Every load's index is selected first (the neighbour, or the cell's own slot), so no load leaves the slot.
+  const uint32_t P = (uint32_t)S.P, nz = (uint32_t)S.nz;
+  g[0] = ld(src, c);
+  const R v1 = ld(src, N + ((fl & F_FX0) ? c : c - P));
+  g[1] = (fl & F_FX0) ? w7<R>(1) * S.T_in : v1;
+  g[2] = ld(src, 2 * N + ((fl & F_FXN) ? c : c + P));
+  g[3] = ld(src, (fl & F_FY0) ? 4 * N + c : 3 * N + c - nz);
+  g[4] = ld(src, (fl & F_FYN) ? 3 * N + c : 4 * N + c + nz);
+  g[5] = ld(src, (fl & F_FZ0) ? 6 * N + c : 5 * N + c - 1);
+  g[6] = ld(src, (fl & F_FZN) ? 5 * N + c : 6 * N + c + 1);
+}
+
+template <class R>
+__global__ __launch_bounds__(256) void k_thermal7(Slot<R>* __restrict__ slots, int bps, int G, int order, int nodd) {
+  const int b = map_block(bps, order);
+  if (b >= G) return;
+  const int si = b / bps;
+  const int lb = b - si * bps;
+  Slot<R>& S = slots[si];
+  if (!S.active) return;
+  const bool raw = nodd && S.raw;
+  if (!nodd && lb == 0 && threadIdx.x == 0) S.raw = 0;  // read by odd steps only
+  const uint32_t c = (uint32_t)lb * blockDim.x + threadIdx.x;
+  if (c >= (uint32_t)S.ncell) return;
+  const uint32_t N = (uint32_t)S.npad;
+  const R* __restrict__ src = nodd ? S.g7 : S.g7 + 7 * (size_t)N;
+  R* __restrict__ dst = nodd ? S.g7 + 7 * (size_t)N : S.g7;
+  const uint32_t fl = ld(S.flags, c);
+  const int type = (int)(fl & F_TYPE);
+  R g[7], T;
+  if (raw) {
+#pragma unroll
+    for (int i = 0; i < 7; ++i) g[i] = ld(src, i * N + c);
+    T = ld(S.T, c);
+  } else {
+    gather7<R>(S, src, c, fl, g);
+    T = 0;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) T += g[i];
+  }
+  R ux = 0, uy = 0, uz = 0;
+  if (type != BC_SOLID) {
+    ux = ld(S.u, 3 * c);
+    uy = ld(S.u, 3 * c + 1);
+    uz = ld(S.u, 3 * c + 2);
+  }
+  const R om = type == BC_SOLID ? S.omega_ts : S.omega_tf;
+  const R sf = type == BC_SOLID ? S.sf_ts : S.sf_tf;
+  const R q = (fl & F_HEAT) ? ld(S.q, c) : (R)0;
+  const R eu[7] = {(R)0, ux, -ux, uy, -uy, uz, -uz};
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    const R geq = w7<R>(i) * T * ((R)1.0 + (R)4.0 * eu[i]);
+    R v = g[i] - om * (g[i] - geq);
+    if (q > (R)0.0) v += w7<R>(i) * sf * q;
+    st(dst, i * N + c, v);
+  }
+}
+
+// Initial D3Q7 populations g_eq(T, u) (u = 0 in solids) into the buffer the first odd step reads.
+template <class R>
+__global__ __launch_bounds__(256) void k_init7(Slot<R>* __restrict__ slots, int slot) {
+  Slot<R>& S = slots[slot];
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= S.ncell) return;
+  const int type = (int)(S.flags[c] & F_TYPE);
+  R ux = S.u[3 * c], uy = S.u[3 * c + 1], uz = S.u[3 * c + 2];
+  if (type == BC_SOLID) ux = uy = uz = 0;
+  const R T = S.T[c];
+  const R eu[7] = {(R)0, ux, -ux, uy, -uy, uz, -uz};
+#pragma unroll
+  for (int i = 0; i < 7; ++i) S.g7[(int64_t)i * S.npad + c] = w7<R>(i) * T * ((R)1.0 + (R)4.0 * eu[i]);
+}
+
+// T of the current step: streaming of the buffer the last step wrote (the second after an odd step) + moments.
+template <class R>
+__global__ __launch_bounds__(256) void k_readout7(Slot<R>* __restrict__ slots, int bps, int G, int order,
+                                                  int last_odd) {
+  const int b = map_block(bps, order);
+  if (b >= G) return;
+  const int si = b / bps;
+  const int lb = b - si * bps;
+  Slot<R>& S = slots[si];
+  if (!S.active) return;
+  const uint32_t c = (uint32_t)lb * blockDim.x + threadIdx.x;
+  if (c >= (uint32_t)S.ncell) return;
+  const R* src = last_odd ? S.g7 + 7 * (size_t)S.npad : S.g7;
+  R g[7];
+  gather7<R>(S, src, c, ld(S.flags, c), g);
+  R T = 0;
+#pragma unroll
+  for (int i = 0; i < 7; ++i) T += g[i];
+  st(S.T, c, T);
+}
+
 // -------------------------------------------------------------- readout ----
 
 // Wave (64 lanes) then block reduction of NV doubles; op[k] = 0 sum, 1 max, 2 min.
@@ -912,7 +1047,7 @@ __constant__ int kStatOp[16] = {1, 0, 0, 0, 0, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0, 0};
 // of src/fluid_engine3d.py:283-312 and src/thermal_engine3d.py:234-243.
 template <class R, int FL, int TH, int GEN, int NODD>
 __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, int bps, int G, int order,
-                                                 int with_stats, double* __restrict__ partials) {
+                                                 int with_stats, double* __restrict__ partials, int t_field) {
   const int b = map_block(bps, order);
   if (b >= G) return;
   const int si = b / bps;
@@ -940,6 +1075,8 @@ __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, in
 #pragma unroll
       for (int i = 0; i < 19; ++i) Tl += g[i];
       st(S.T, c, Tl);
+    } else if (t_field) {  // T formed by another kernel (the D3Q7 readout)
+      Tl = S.T[c];
     }
     if (with_stats) {
       const float ux = (float)S.u[3 * c], uy = (float)S.u[3 * c + 1], uz = (float)S.u[3 * c + 2];
@@ -965,7 +1102,7 @@ __global__ __launch_bounds__(256) void k_readout(Slot<R>* __restrict__ slots, in
         v[7] = ux;
         v[10] = 1.0;
       }
-      if (TH) {
+      if (TH || t_field) {
         const float t = (float)Tl;
         v[11] = t > 0.0f ? t : 0.0f;
         v[12] = t;

@@ -53,10 +53,11 @@ struct xadv_batch {
   void* slots_host = nullptr;  // pinned copy of the slot table
   size_t slot_bytes = 0;
   int compact = 0;  // fluid lattice stored for non-solid cells only
+  int q7 = 0;       // thermal lattice is the D3Q7 variant (k_thermal7 after each fluid pass; d.thermal is 0)
   int64_t cols_per_slot = 0;  // padded column-table entries per slot
   // arenas (bytes)
   char *f = nullptr, *g = nullptr, *flags = nullptr, *q = nullptr, *fin = nullptr, *fsnap = nullptr, *gsnap = nullptr,
-       *gxh = nullptr, *rho = nullptr, *u = nullptr, *T = nullptr, *cols = nullptr;
+       *gxh = nullptr, *rho = nullptr, *u = nullptr, *T = nullptr, *cols = nullptr, *g7 = nullptr;
   double* partials = nullptr;
   double* stats_dev = nullptr;
   double* met_partials = nullptr;
@@ -142,10 +143,17 @@ void launch_readout_t(xadv_batch* b, int nodd, int with_stats) {
   dim3 grid(b->G_r_launch), block(b->d.block);
   if (nodd)
     hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps_r, b->G_r,
-                       b->d.xcd_remap, with_stats, b->partials);
+                       b->d.xcd_remap, with_stats, b->partials, b->q7);
   else
     hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps_r, b->G_r,
-                       b->d.xcd_remap, with_stats, b->partials);
+                       b->d.xcd_remap, with_stats, b->partials, b->q7);
+}
+
+// D3Q7 thermal step of every slot (after the fluid pass of the same step), one cell per thread.
+template <class R>
+void launch_thermal7(xadv_batch* b, int nodd) {
+  hipLaunchKernelGGL((xadv::k_thermal7<R>), dim3(b->G_r_launch), dim3(b->d.block), 0, b->stream,
+                     reinterpret_cast<Slot<R>*>(b->slots_dev), b->bps_r, b->G_r, b->d.xcd_remap, nodd);
 }
 
 template <class R, int FL, int TH, int GEN>
@@ -174,10 +182,18 @@ void launch_init_t(xadv_batch* b, int slot, int64_t ncell, int frozen) {
   } while (0)
 
 void launch_pass(xadv_batch* b, int nodd, int want_fields) {
+  // With the D3Q7 thermal variant every fluid pass stores u, which its thermal step collides with.
+  if (b->q7) want_fields = 1;
   if (b->d.precision == 64)
     XADV_DISPATCH(launch_pass_t, double, b, nodd, want_fields);
   else
     XADV_DISPATCH(launch_pass_t, float, b, nodd, want_fields);
+  if (b->q7) {
+    if (b->d.precision == 64)
+      launch_thermal7<double>(b, nodd);
+    else
+      launch_thermal7<float>(b, nodd);
+  }
 }
 
 void launch_readout(xadv_batch* b, int nodd, int with_stats) {
@@ -381,6 +397,12 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d, bool device_inputs
   S.omega_ts = (R)1.0 / tts;
   S.sf_tf = (R)1.0 - (R)0.5 * S.omega_tf;
   S.sf_ts = (R)1.0 - (R)0.5 * S.omega_ts;
+  if (b->q7) {
+    // D3Q7 thermal variant: 1/tau7 with tau7 = 1/2 + 4/3 (tau - 1/2), the diffusivity of the reference's tau;
+    // sf_tf / sf_ts keep the reference step's source factor 1 - 1/(2 tau).
+    S.omega_tf = (R)1.0 / ((R)0.5 + (ttf - (R)0.5) * (R)(4.0 / 3.0));
+    S.omega_ts = (R)1.0 / ((R)0.5 + (tts - (R)0.5) * (R)(4.0 / 3.0));
+  }
   S.T_in = (R)d->T_in;
   S.dir_T = (R)d->dirichlet_T;
   S.nf = (uint32_t)(b->compact ? round_up(nfluid, 64) : S.npad);
@@ -400,6 +422,7 @@ int load_t(xadv_batch* b, int s, const xadv_instance_desc* d, bool device_inputs
   S.rho = reinterpret_cast<R*>(b->rho + (size_t)s * cap * rs);
   S.u = reinterpret_cast<R*>(b->u + (size_t)s * 3 * cap * rs);
   S.T = reinterpret_cast<R*>(b->T + (size_t)s * cap * rs);
+  S.g7 = b->q7 ? reinterpret_cast<R*>(b->g7 + (size_t)s * 2 * 7 * capn * rs) : nullptr;
   if (!device_inputs) {
     HIPCHK(hipMemcpyAsync((void*)S.flags, fl.data(), n * 4, hipMemcpyHostToDevice, b->stream));
     HIPCHK(hipMemcpyAsync((void*)S.q, q.data(), n * rs, hipMemcpyHostToDevice, b->stream));
@@ -436,8 +459,13 @@ int init_t(xadv_batch* b, int s, const void* rho, const void* u, const void* T) 
   HIPCHK(hipGetLastError());
   XADV_DISPATCH(launch_init_t, R, b, s, n, b->frozen[s]);
   HIPCHK(hipGetLastError());
+  if (b->q7) {
+    hipLaunchKernelGGL((xadv::k_init7<R>), dim3(gb), dim3(bs), 0, b->stream, reinterpret_cast<Slot<R>*>(b->slots_dev),
+                       s);
+    HIPCHK(hipGetLastError());
+  }
   S.active = 1;
-  S.raw = b->d.thermal ? 1 : 0;
+  S.raw = (b->d.thermal || b->q7) ? 1 : 0;
   b->inited[s] = 1;
   return push_slot<R>(b, s);
 }
@@ -462,7 +490,7 @@ hipError_t release_batch(xadv_batch* b) {
   if (b->stream) keep(hipStreamSynchronize(b->stream));
   for (auto& kv : b->graphs) keep(hipGraphExecDestroy(kv.second));
   char** ps[] = {&b->f, &b->g, &b->flags, &b->q, &b->fin, &b->fsnap, &b->gsnap, &b->gxh, &b->rho, &b->u, &b->T,
-                 &b->cols};
+                 &b->cols, &b->g7};
   for (char** p : ps)
     if (*p) keep(hipFree(*p));
   if (b->partials) keep(hipFree(b->partials));
@@ -509,11 +537,18 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   if (d.alternate_sweep != 0 && d.alternate_sweep != 1) return fail(XADV_E_ARG, "alternate_sweep must be 0 or 1");
   if (d.fluid_storage < XADV_STORAGE_AUTO || d.fluid_storage > XADV_STORAGE_COMPACT)
     return fail(XADV_E_ARG, "fluid_storage must be XADV_STORAGE_AUTO, _DENSE or _COMPACT");
+  if (d.thermal_q != 0 && d.thermal_q != 19 && d.thermal_q != 7)
+    return fail(XADV_E_ARG, "thermal_q must be 19 (or 0) or 7");
+  const bool q7 = d.thermal && d.thermal_q == 7;
+  if (q7 && (!d.fluid || d.general))
+    return fail(XADV_E_ARG, "the D3Q7 thermal lattice runs coupled to the fluid lattice, channel boundaries only");
+  if (q7) d.thermal = 0;  // the fused passes run the fluid lattice only; k_thermal7 does the thermal step
   const bool can_compact = d.fluid && !d.general;
   if (d.fluid_storage == XADV_STORAGE_COMPACT && !can_compact)
     return fail(XADV_E_ARG, "compact fluid storage needs the fluid lattice on and general = 0");
   auto* b = new xadv_batch();
   b->d = d;
+  b->q7 = q7 ? 1 : 0;
   b->compact = can_compact && (d.fluid_storage == XADV_STORAGE_COMPACT ||
                                (d.fluid_storage == XADV_STORAGE_AUTO && d.cap_nz <= xadv::kCompactMaxNz));
   b->rsize = d.precision == 64 ? 8 : 4;
@@ -562,6 +597,7 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
       {&b->u, S * 3 * cap * rs},
       {&b->T, S * cap * rs},
       {&b->cols, b->compact ? S * (size_t)b->cols_per_slot * sizeof(xadv::ColInfo) : 0},
+      {&b->g7, q7 ? S * 2 * 7 * capn * rs : 0},
   };
   for (auto& a : allocs) {
     if (!a.bytes) continue;
@@ -810,6 +846,15 @@ int xadv_readout(xadv_batch* b, int with_stats) {
   if (!b) return fail(XADV_E_ARG, "null batch");
   if (b->npass < 1) return fail(XADV_E_STATE, "readout needs at least one step");
   if (!b->fields_fresh) return fail(XADV_E_STATE, "readout needs the last xadv_step with want_fields = 1");
+  if (b->q7) {
+    const int last_odd = (int)(b->npass & 1);
+    if (b->d.precision == 64)
+      hipLaunchKernelGGL((xadv::k_readout7<double>), dim3(b->G_r_launch), dim3(b->d.block), 0, b->stream,
+                         reinterpret_cast<Slot<double>*>(b->slots_dev), b->bps_r, b->G_r, b->d.xcd_remap, last_odd);
+    else
+      hipLaunchKernelGGL((xadv::k_readout7<float>), dim3(b->G_r_launch), dim3(b->d.block), 0, b->stream,
+                         reinterpret_cast<Slot<float>*>(b->slots_dev), b->bps_r, b->G_r, b->d.xcd_remap, last_odd);
+  }
   launch_readout(b, (int)((b->npass + 1) & 1), with_stats);
   HIPCHK(hipGetLastError());
   if (with_stats) {

@@ -40,7 +40,8 @@ class LatticeBatch:
     """One batch of lattice slots advanced together by the fused HIP pass."""
 
     def __init__(self, *, slots=1, cap_shape, fluid=True, thermal=True, general=False, precision=32, block=256,
-                 xcd_remap=False, graph_chunk=100, cells_per_thread=1, fluid_storage="auto", alternate_sweep=True):
+                 xcd_remap=False, graph_chunk=100, cells_per_thread=1, fluid_storage="auto", alternate_sweep=True,
+                 thermal_lattice="D3Q19"):
         _require_gpu()
         self._lib = _lib.load()
         if graph_chunk and graph_chunk % 2:
@@ -48,15 +49,19 @@ class LatticeBatch:
         storage = {"auto": _lib.STORAGE_AUTO, "dense": _lib.STORAGE_DENSE, "compact": _lib.STORAGE_COMPACT}
         if fluid_storage not in storage:
             raise ValueError(f"fluid_storage must be one of {sorted(storage)}")
+        if thermal_lattice not in ("D3Q19", "D3Q7"):
+            raise ValueError("thermal_lattice must be 'D3Q19' (the reference's) or 'D3Q7'")
         d = _lib.BatchDesc(int(precision), int(bool(fluid)), int(bool(thermal)), int(bool(general)), int(slots),
                            int(cap_shape[0]), int(cap_shape[1]), int(cap_shape[2]), int(block), int(bool(xcd_remap)),
-                           int(graph_chunk), int(cells_per_thread), storage[fluid_storage], int(bool(alternate_sweep)))
+                           int(graph_chunk), int(cells_per_thread), storage[fluid_storage], int(bool(alternate_sweep)),
+                           7 if thermal_lattice == "D3Q7" else 19)
         h = ctypes.c_void_p()
         torch.cuda.init()
         _lib.check(self._lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)), "xadv_batch_create")
         self._h = h
         self.slots = int(slots)
         self.fluid, self.thermal, self.general = bool(fluid), bool(thermal), bool(general)
+        self.thermal_lattice = thermal_lattice
         self.precision = int(precision)
         self.dtype = torch.float64 if precision == 64 else torch.float32
         self.np_dtype = np.float64 if precision == 64 else np.float32
@@ -236,7 +241,7 @@ class LatticeSolver:
     def __init__(self, mask, *, fluid=True, thermal=True, heat=None, tau=0.6, tau_th_fluid=0.6, tau_th_solid=0.6,
                  heat_power=0.0, T_in=0.0, rho_out=1.0, u_in=0.0, periodic=(0, 0, 0), force_x=0.0, u_wall=0.0,
                  dirichlet=None, frozen_velocity=False, precision=32, block=256, xcd_remap=False, graph_chunk=100,
-                 cells_per_thread=1, fluid_storage="auto", alternate_sweep=True):
+                 cells_per_thread=1, fluid_storage="auto", alternate_sweep=True, thermal_lattice="D3Q19"):
         mask = np.asarray(mask)
         general = any(periodic) or force_x != 0.0 or u_wall != 0.0 or dirichlet is not None
         self.mask = mask
@@ -244,7 +249,7 @@ class LatticeSolver:
         self.batch = LatticeBatch(slots=1, cap_shape=self.shape, fluid=fluid, thermal=thermal, general=general,
                                   precision=precision, block=block, xcd_remap=xcd_remap, graph_chunk=graph_chunk,
                                   cells_per_thread=cells_per_thread, fluid_storage=fluid_storage,
-                                  alternate_sweep=alternate_sweep)
+                                  alternate_sweep=alternate_sweep, thermal_lattice=thermal_lattice)
         self.batch.load(0, mask, heat=heat, tau=tau, tau_th_fluid=tau_th_fluid, tau_th_solid=tau_th_solid,
                         heat_power=heat_power, T_in=T_in, rho_out=rho_out, u_in=u_in, periodic=periodic,
                         force_x=force_x, u_wall=u_wall, dirichlet=dirichlet, frozen_velocity=frozen_velocity)
@@ -347,7 +352,11 @@ class CoupledSolver(LatticeSolver):
     """The production solve: D3Q19 fluid + D3Q19 thermal on one cold-plate lattice.
 
     Constructed like src/simulation.py:14-90 from a mask, a heat-source map
-    (None: the z = 0 plane) and an inlet velocity.
+    (None: the z = 0 plane) and an inlet velocity.  ``thermal_lattice="D3Q7"``
+    (keyword) swaps in the seven-velocity thermal lattice, which has no
+    reference counterpart: same relaxation times in (the reference's
+    diffusivities and heat input per step), 56 instead of 152 bytes per cell
+    and step.
     """
 
     def __init__(self, mask, heat_source=None, *, tau_fluid=0.52, heat_power=0.1, t_inlet=25.0, u_inlet_val=0.03,

@@ -172,8 +172,12 @@ def run_simulation(mask,
                    *,
                    min_steps_rule=True,
                    init_fields=None,
-                   precision=32):
-    """Run one coupled solve; returns {"u","rho","T","mask","steps","d_ke","dT_conv","dQ_conv","converged"}."""
+                   precision=32,
+                   thermal_lattice="D3Q19"):
+    """Run one coupled solve; returns {"u","rho","T","mask","steps","d_ke","dT_conv","dQ_conv","converged"}.
+
+    ``thermal_lattice="D3Q7"`` runs the seven-velocity thermal lattice (no reference counterpart; the same
+    relaxation times, so the reference's diffusivities and heat input per step)."""
     if not str(device).startswith("cuda"):
         raise ValueError(f"device={device!r}: this solver runs on the GPU only (use device='cuda')")
     mask = np.asarray(mask)
@@ -185,7 +189,7 @@ def run_simulation(mask,
     graph_chunk = stats_every if stats_every % 2 == 0 else 0
     solver = CoupledSolver(mask, heat_source, tau_fluid=tau_fluid, heat_power=heat_power, t_inlet=t_inlet,
                            u_inlet_val=u_inlet_val, Pr=Pr, k_ratio=k_ratio, precision=precision,
-                           graph_chunk=graph_chunk)
+                           graph_chunk=graph_chunk, thermal_lattice=thermal_lattice)
     try:
         if init_fields is not None:
             solver.init_equilibrium(*init_fields)
