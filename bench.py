@@ -312,7 +312,8 @@ def batch_bench(n_designs, slots, max_steps, tol, seed0=2000, groups=1, **shape)
 
 def config1_bench(cpu_steps_1=100, cpu_steps_all=500):
     """Plain small cold plate (74x64x16, 2,000 steps): the drop-in run_simulation on the GPU and the oracle
-    on this host's cores at 1 thread and at all threads (bounded step counts, MLUPS)."""
+    on this host's cores at 1 thread and at all threads (bounded step counts, MLUPS), with the reference's
+    D3Q19 thermal lattice and, reported separately, the D3Q7 variant (same relaxation times and heat input)."""
     import torch
     from oracle import oracle as orc
     from xadvcool_amd import geometry as G
@@ -321,20 +322,38 @@ def config1_bench(cpu_steps_1=100, cpu_steps_all=500):
     m = G.config1_mask()
     kw = dict(tau_fluid=c["tau_fluid"], heat_power=c["heat_power"], t_inlet=c["t_inlet"], u_inlet_val=c["u_inlet_val"],
               tol_ke=0.0, tol_T=0.0, Pr=c["Pr"], k_ratio=c["k_ratio"])
-    run_simulation(m, max_steps=200, device="cuda", silent=True, **kw)  # load kernels, build graphs
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    r = run_simulation(m, max_steps=c["steps"], device="cuda", silent=True, **kw)
-    gpu_s = time.perf_counter() - t0
-    out = {"mode": "config1", "lattice": list(m.shape), "steps": r["steps"],
-           "gpu": {"wall_s": round(gpu_s, 4), "mlups": round(m.size * r["steps"] / gpu_s / 1e6, 1),
-                   "note": "run_simulation end to end: init, graph replays, stats every 100 steps, field copies"}}
-    for threads, n in ((1, cpu_steps_1), (orc.max_threads(), cpu_steps_all)):
-        orc.set_threads(threads)
+    fluid = m != 1
+
+    def summary(r):  # pressure drop over the fluid cells of x = 1 and x = nx-2, temperature rise
+        dp = (float(r["rho"][1][fluid[1]].mean()) - float(r["rho"][-2][fluid[-2]].mean())) / 3.0
+        rise = r["T"] - c["t_inlet"]
+        return {"pressure_drop": dp, "T_rise_max": float(rise.max()), "T_rise_mean": float(rise.mean())}
+
+    out = {"mode": "config1", "lattice": list(m.shape)}
+    for lat in ("D3Q19", "D3Q7"):
+        run_simulation(m, max_steps=200, device="cuda", silent=True, thermal_lattice=lat, **kw)  # kernels, graphs
+        torch.cuda.synchronize()
         t0 = time.perf_counter()
-        orc.run_simulation(m, max_steps=n, **kw)
-        dt = time.perf_counter() - t0
-        out[f"oracle_{threads}t"] = {"steps": n, "wall_s": round(dt, 3), "mlups": round(m.size * n / dt / 1e6, 2)}
+        r = run_simulation(m, max_steps=c["steps"], device="cuda", silent=True, thermal_lattice=lat, **kw)
+        gpu_s = time.perf_counter() - t0
+        key = "" if lat == "D3Q19" else "_d3q7"
+        out["steps"] = r["steps"]
+        out["gpu" + key] = {"wall_s": round(gpu_s, 4), "mlups": round(m.size * r["steps"] / gpu_s / 1e6, 1),
+                            **{k: round(v, 7) for k, v in summary(r).items()},
+                            "note": "run_simulation end to end: init, graph replays, stats every 100 steps, field "
+                                    "copies" + ("" if lat == "D3Q19" else "; thermal lattice D3Q7 (no reference "
+                                                "counterpart; reported beside the D3Q19 result)")}
+        for threads, n in ((1, cpu_steps_1), (orc.max_threads(), cpu_steps_all)):
+            orc.set_threads(threads)
+            t0 = time.perf_counter()
+            orc.run_simulation(m, max_steps=n, thermal_lattice=lat, **kw)
+            dt = time.perf_counter() - t0
+            out[f"oracle{key}_{threads}t"] = {"steps": n, "wall_s": round(dt, 3), "mlups": round(m.size * n / dt / 1e6, 2)}
+    g19, g7 = out["gpu"], out["gpu_d3q7"]
+    out["d3q7_vs_d3q19"] = {"T_rise_max": round(g7["T_rise_max"] / g19["T_rise_max"] - 1, 5),
+                            "T_rise_mean": round(g7["T_rise_mean"] / g19["T_rise_mean"] - 1, 5),
+                            "pressure_drop": round(g7["pressure_drop"] / g19["pressure_drop"] - 1, 7),
+                            "speed": round(g7["mlups"] / g19["mlups"], 3)}
     out["host_threads"] = orc.max_threads()
     print(json.dumps(out), flush=True)
 
