@@ -60,4 +60,11 @@ def test_bad_arguments_are_errors_not_crashes():
     d = _lib.BatchDesc(32, 1, 1, 0, 1, 4, 4, 4, 256, 0, 0, 0, 0, 2)  # alternate_sweep is a 0/1 flag
     assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
     assert b"alternate_sweep" in lib.xadv_last_error()
+    d = _lib.BatchDesc(32, 1, 1, 0, 1, 4, 4, 4, 256, 0, 0, 0, 0, 1, 9)  # thermal lattice D3Q9 does not exist
+    assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
+    assert b"thermal_q" in lib.xadv_last_error()
+    for fluid, general in ((0, 0), (1, 1)):  # the D3Q7 lattice runs coupled, channel boundaries only
+        d = _lib.BatchDesc(32, fluid, 1, general, 1, 4, 4, 4, 256, 0, 0, 0, 0, 1, 7)
+        assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
+        assert b"D3Q7" in lib.xadv_last_error()
     assert lib.xadv_step(None, 1, 0) == _lib.XADV_E_ARG
