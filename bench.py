@@ -427,6 +427,8 @@ def main():
     ap.add_argument("--block", type=int, default=256, help="threads per block of the fused pass")
     ap.add_argument("--cpt", type=int, default=2, help="lattice cells per thread of the fused pass")
     ap.add_argument("--xcd", action="store_true", help="map consecutive lattice blocks to one XCD")
+    ap.add_argument("--slab", type=int, default=0,
+                    help="passes/batch modes: x-planes per slab of the wavefront pass pairs (0: whole passes)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
     args = ap.parse_args()
     if args.mode == "solves" and args.designs is None:
@@ -439,7 +441,7 @@ def main():
 
     if args.mode == "passes":
         return profile_passes(args.passes, args.slots or 1, block=args.block, cells_per_thread=args.cpt,
-                              xcd_remap=args.xcd, alternate_sweep=args.sweep == "alternate")
+                              xcd_remap=args.xcd, alternate_sweep=args.sweep == "alternate", slab_planes=args.slab)
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
     if args.mode == "ceilings":
@@ -463,7 +465,8 @@ def main():
         print(json.dumps({"mode": "batch", **batch_bench(args.designs or 64, args.slots or 64, args.max_steps,
                                                          args.tol, groups=args.groups, block=args.block,
                                                          cells_per_thread=args.cpt,
-                                                         alternate_sweep=args.sweep == "alternate")}),
+                                                         alternate_sweep=args.sweep == "alternate",
+                                                         slab_planes=args.slab)}),
               flush=True)
         return
 

@@ -68,6 +68,12 @@ typedef struct {
                            tau_th and heated with the reference step's factor 1 - 1/(2 tau_th);
                            needs the fluid lattice and general = 0.  A step is the fused fluid
                            pass (storing u) followed by one D3Q7 kernel. */
+  int slab_planes;      /* 0: each pass is one launch over whole lattices.  n > 0: runs of an even
+                           number of steps starting on an odd pass go as pairs of passes split into
+                           x-slabs of n planes, launched as a wavefront (odd pass on slab k, then the
+                           even pass on slab k - 1), so a slab's even pass finds the lines its odd pass
+                           touched still in the Infinity Cache; the sweep direction alternates between
+                           pairs.  Same bits.  Needs thermal_q 19 and general = 0. */
 } xadv_batch_desc;
 
 #define XADV_STORAGE_AUTO 0

@@ -649,10 +649,14 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // are separate arrays and each slot read here is written only by this cell,
 // so the order of the fluid stores and thermal loads does not matter).
 // Cell k of a thread is block_base + k*blockDim + threadIdx, so every load
-// instruction of a wavefront covers 64 consecutive cells.
+// instruction of a wavefront covers 64 consecutive cells.  A launch covers the
+// x-planes [x0, x1) of every slot (cells x0*P .. x1*P - 1; the whole lattice
+// for x0 = 0, x1 >= nx); clear_raw says whether this even launch may clear the
+// raw flag (only the last launch of an even pass split over x-slabs does, so
+// every launch of the odd pass before it has read the flag).
 template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP>
 __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int order,
-                                                             int want_fields) {
+                                                             int want_fields, int x0, int x1, int clear_raw) {
   static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
   constexpr bool PRE = XADV_TH_PREFETCH && FL && TH;
   const int b = map_block(bps, order);
@@ -667,11 +671,11 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
   if (TH) {
     if (NODD)
       raw = S.raw;
-    else if (lb == 0 && threadIdx.x == 0 && S.raw)
+    else if (clear_raw && lb == 0 && threadIdx.x == 0 && S.raw)
       S.raw = 0;
   }
-  const uint32_t ncell = (uint32_t)S.ncell;
-  const uint32_t base = (uint32_t)lb * blockDim.x * CPT + threadIdx.x;
+  const uint32_t ncell = min((uint32_t)S.ncell, (uint32_t)min(x1, S.nx) * (uint32_t)S.P);
+  const uint32_t base = (uint32_t)x0 * (uint32_t)S.P + (uint32_t)lb * blockDim.x * CPT + threadIdx.x;
   if (base >= ncell) return;
 
   Cell cell[CPT];

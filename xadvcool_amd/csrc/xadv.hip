@@ -107,34 +107,43 @@ int push_field(xadv_batch* b, int s, size_t off, size_t bytes) {
 
 // ----------------------------------------------------------- dispatch -----
 
+// The x-planes a pass launch covers (all of them, or one slab of a pass split over x-slabs) and its grid.
+struct PassRange {
+  int x0 = 0, x1 = 1 << 30;  // planes [x0, x1) of every slot
+  int clear_raw = 1;         // an even launch may clear the raw flag
+  int bps = 0, G = 0, G_launch = 0;  // 0: the batch's whole-lattice grid
+  bool slab = false;
+};
+
 template <class R, int FL, int TH, int GEN, int CPT, int CMP>
-void launch_pass_c(xadv_batch* b, int nodd, int want_fields) {
+void launch_pass_c(xadv_batch* b, int nodd, int want_fields, const PassRange& r) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
-  dim3 grid(b->G_launch), block(b->d.block);
-  const int order = (b->d.xcd_remap ? 1 : 0) | ((b->d.alternate_sweep && !nodd) ? 2 : 0);
+  const int bps = r.bps ? r.bps : b->bps, G = r.G ? r.G : b->G;
+  dim3 grid(r.G_launch ? r.G_launch : b->G_launch), block(b->d.block);
+  const int order = (b->d.xcd_remap ? 1 : 0) | ((b->d.alternate_sweep && !nodd && !r.slab) ? 2 : 0);
   if (nodd)
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT, CMP>), grid, block, 0, b->stream, sl, b->bps, b->G,
-                       order, want_fields);
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT, CMP>), grid, block, 0, b->stream, sl, bps, G, order,
+                       want_fields, r.x0, r.x1, r.clear_raw);
   else
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT, CMP>), grid, block, 0, b->stream, sl, b->bps, b->G,
-                       order, want_fields);
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT, CMP>), grid, block, 0, b->stream, sl, bps, G, order,
+                       want_fields, r.x0, r.x1, r.clear_raw);
 }
 
 // Compact storage exists for the channel kernels with the fluid lattice on.
 template <class R, int FL, int TH, int GEN, int CPT>
-void launch_pass_s(xadv_batch* b, int nodd, int want_fields) {
+void launch_pass_s(xadv_batch* b, int nodd, int want_fields, const PassRange& r) {
   if constexpr (FL && !GEN) {
-    if (b->compact) return launch_pass_c<R, FL, TH, GEN, CPT, 1>(b, nodd, want_fields);
+    if (b->compact) return launch_pass_c<R, FL, TH, GEN, CPT, 1>(b, nodd, want_fields, r);
   }
-  launch_pass_c<R, FL, TH, GEN, CPT, 0>(b, nodd, want_fields);
+  launch_pass_c<R, FL, TH, GEN, CPT, 0>(b, nodd, want_fields, r);
 }
 
 template <class R, int FL, int TH, int GEN>
-void launch_pass_t(xadv_batch* b, int nodd, int want_fields) {
+void launch_pass_t(xadv_batch* b, int nodd, int want_fields, const PassRange& r) {
   if (b->d.cells_per_thread == 2)
-    launch_pass_s<R, FL, TH, GEN, 2>(b, nodd, want_fields);
+    launch_pass_s<R, FL, TH, GEN, 2>(b, nodd, want_fields, r);
   else
-    launch_pass_s<R, FL, TH, GEN, 1>(b, nodd, want_fields);
+    launch_pass_s<R, FL, TH, GEN, 1>(b, nodd, want_fields, r);
 }
 
 template <class R, int FL, int TH, int GEN>
@@ -181,18 +190,52 @@ void launch_init_t(xadv_batch* b, int slot, int64_t ncell, int frozen) {
     else FN<R, 0, 1, 0>(__VA_ARGS__);                                    \
   } while (0)
 
-void launch_pass(xadv_batch* b, int nodd, int want_fields) {
+void launch_pass(xadv_batch* b, int nodd, int want_fields, const PassRange& r = PassRange{}) {
   // With the D3Q7 thermal variant every fluid pass stores u, which its thermal step collides with.
   if (b->q7) want_fields = 1;
   if (b->d.precision == 64)
-    XADV_DISPATCH(launch_pass_t, double, b, nodd, want_fields);
+    XADV_DISPATCH(launch_pass_t, double, b, nodd, want_fields, r);
   else
-    XADV_DISPATCH(launch_pass_t, float, b, nodd, want_fields);
+    XADV_DISPATCH(launch_pass_t, float, b, nodd, want_fields, r);
   if (b->q7) {
     if (b->d.precision == 64)
       launch_thermal7<double>(b, nodd);
     else
       launch_thermal7<float>(b, nodd);
+  }
+}
+
+// Two passes (odd, then even) split over x-slabs of slab_planes planes and run as a wavefront: A_k is the
+// odd pass on slab k, B_k the even pass on slab k, launched A_0, A_1, B_0, A_2, B_1, ..., B_{K-1} (or the
+// mirror order).  Pass n touches the neighbour slots of x +- 1 only, so B_k needs A_{k-1}, A_k and A_{k+1},
+// all launched before it, and writes nothing that a later A launch reads (the odd pass reads only the
+// slots of its own cells and their x - 1 / x + 1 planes, which B has not touched yet or no longer needs).
+// Every cell computes what it computes in the whole-lattice pass, so the bits are the same; the even pass
+// of a slab runs while the odd pass's lines of that slab are still in the Infinity Cache.
+void launch_pass_pair_slabs(xadv_batch* b, int want_fields, bool descending) {
+  const int S = b->d.slab_planes;
+  const int K = (b->d.cap_nx + S - 1) / S;
+  const int cpb = b->d.block * b->d.cells_per_thread;
+  PassRange r;
+  r.slab = true;
+  r.bps = (int)(((int64_t)b->cap_P * S + cpb - 1) / cpb);
+  r.G = r.bps * b->d.slots;
+  r.G_launch = b->d.xcd_remap ? xadv::xcd_grid(r.bps, b->d.slots) : r.G;
+  for (int t = 0; t <= K; ++t) {
+    if (t < K) {
+      const int k = descending ? K - 1 - t : t;
+      r.x0 = k * S;
+      r.x1 = (k + 1) * S;
+      r.clear_raw = 0;
+      launch_pass(b, 1, 0, r);
+    }
+    if (t >= 1) {
+      const int k = descending ? K - t : t - 1;
+      r.x0 = k * S;
+      r.x1 = (k + 1) * S;
+      r.clear_raw = t == K;  // the last launch of the pair: every odd launch has read the raw flag
+      launch_pass(b, 0, want_fields, r);
+    }
   }
 }
 
@@ -543,6 +586,9 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   if (q7 && (!d.fluid || d.general))
     return fail(XADV_E_ARG, "the D3Q7 thermal lattice runs coupled to the fluid lattice, channel boundaries only");
   if (q7) d.thermal = 0;  // the fused passes run the fluid lattice only; k_thermal7 does the thermal step
+  if (d.slab_planes < 0) return fail(XADV_E_ARG, "slab_planes must be >= 0");
+  if (d.slab_planes > 0 && (q7 || d.general))
+    return fail(XADV_E_ARG, "x-slab pairs of passes need the D3Q19 lattices and general = 0 (no periodic x axis)");
   const bool can_compact = d.fluid && !d.general;
   if (d.fluid_storage == XADV_STORAGE_COMPACT && !can_compact)
     return fail(XADV_E_ARG, "compact fluid storage needs the fluid lattice on and general = 0");
@@ -809,6 +855,8 @@ int xadv_step(xadv_batch* b, int nsteps, int want_fields) {
   if (nsteps < 0) return fail(XADV_E_ARG, "nsteps must be >= 0");
   if (nsteps == 0) return XADV_OK;
   const int chunk = b->d.graph_chunk;
+  // x-slab pairs (slab_planes > 0) need pairs of passes that start on an odd pass; other runs take whole passes.
+  const bool slabs = b->d.slab_planes > 0 && b->npass % 2 == 0 && nsteps % 2 == 0;
   if (chunk && nsteps % chunk == 0 && b->npass % 2 == 0) {
     const std::pair<int, int> key(chunk, want_fields ? 1 : 0);
     const std::pair<int, int> key0(chunk, 0);
@@ -819,7 +867,12 @@ int xadv_step(xadv_batch* b, int nsteps, int want_fields) {
       if (it == b->graphs.end()) {
         hipGraph_t gr;
         HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        for (int j = 0; j < chunk; ++j) launch_pass(b, (j + 1) & 1, (kk.second && j == chunk - 1) ? 1 : 0);
+        if (b->d.slab_planes > 0) {  // chunk is even: chunk / 2 pairs, the sweep direction alternating
+          for (int j = 0; j < chunk / 2; ++j)
+            launch_pass_pair_slabs(b, (kk.second && j == chunk / 2 - 1) ? 1 : 0, (j & 1) != 0);
+        } else {
+          for (int j = 0; j < chunk; ++j) launch_pass(b, (j + 1) & 1, (kk.second && j == chunk - 1) ? 1 : 0);
+        }
         hipError_t le = hipGetLastError();
         HIPCHK(hipStreamEndCapture(b->stream, &gr));
         if (le != hipSuccess) return fail(XADV_E_HIP, std::string("launch in capture: ") + hipGetErrorString(le));
@@ -831,6 +884,12 @@ int xadv_step(xadv_batch* b, int nsteps, int want_fields) {
       HIPCHK(hipGraphLaunch(it->second, b->stream));
       b->npass += chunk;
     }
+  } else if (slabs) {
+    for (int k = 0; k < nsteps / 2; ++k) {
+      launch_pass_pair_slabs(b, (want_fields && k == nsteps / 2 - 1) ? 1 : 0, ((b->npass / 2) & 1) != 0);
+      b->npass += 2;
+    }
+    HIPCHK(hipGetLastError());
   } else {
     for (int k = 0; k < nsteps; ++k) {
       b->npass += 1;

@@ -41,7 +41,7 @@ class LatticeBatch:
 
     def __init__(self, *, slots=1, cap_shape, fluid=True, thermal=True, general=False, precision=32, block=256,
                  xcd_remap=False, graph_chunk=100, cells_per_thread=1, fluid_storage="auto", alternate_sweep=True,
-                 thermal_lattice="D3Q19"):
+                 thermal_lattice="D3Q19", slab_planes=0):
         _require_gpu()
         self._lib = _lib.load()
         if graph_chunk and graph_chunk % 2:
@@ -54,7 +54,7 @@ class LatticeBatch:
         d = _lib.BatchDesc(int(precision), int(bool(fluid)), int(bool(thermal)), int(bool(general)), int(slots),
                            int(cap_shape[0]), int(cap_shape[1]), int(cap_shape[2]), int(block), int(bool(xcd_remap)),
                            int(graph_chunk), int(cells_per_thread), storage[fluid_storage], int(bool(alternate_sweep)),
-                           7 if thermal_lattice == "D3Q7" else 19)
+                           7 if thermal_lattice == "D3Q7" else 19, int(slab_planes))
         h = ctypes.c_void_p()
         torch.cuda.init()
         _lib.check(self._lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)), "xadv_batch_create")
