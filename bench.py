@@ -112,6 +112,20 @@ def load_design(batch, d, Pr=7.0, k_ratio=628.0, T_in=25.0, slot=0):
     batch.init(slot)
 
 
+def host_cpu():
+    """The host CPU the oracle ran on: model name, logical CPUs in the machine and in this process's affinity
+    mask, and the oracle's OpenMP thread count."""
+    from oracle import oracle as orc
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), None)
+    except OSError:
+        pass
+    return {"model": model, "logical_cpus": os.cpu_count(), "cpus_in_affinity": len(os.sched_getaffinity(0)),
+            "oracle_threads": orc.max_threads()}
+
+
 def cpu_baseline(design, seconds, threads):
     """The oracle (reference six-kernel schedule, FP32) on the same lattice, timed on this host."""
     from oracle import oracle as orc
@@ -354,7 +368,7 @@ def config1_bench(cpu_steps_1=100, cpu_steps_all=500):
                             "T_rise_mean": round(g7["T_rise_mean"] / g19["T_rise_mean"] - 1, 5),
                             "pressure_drop": round(g7["pressure_drop"] / g19["pressure_drop"] - 1, 7),
                             "speed": round(g7["mlups"] / g19["mlups"], 3)}
-    out["host_threads"] = orc.max_threads()
+    out["host"] = host_cpu()
     print(json.dumps(out), flush=True)
 
 
@@ -388,7 +402,7 @@ def config2_bench(seed=0, tol=1e-6, max_steps=30000, cpu_seconds=12.0):
         out[f"oracle_{threads}t"] = {"steps": n, "wall_s": round(t, 2), "mlups": round(mlups, 3),
                                      "s_per_step": round(t / max(n, 1), 4),
                                      "solve_s_at_this_rate": round(r["steps"] * t / max(n, 1), 1)}
-    out["host_threads"] = orc.max_threads()
+    out["host"] = host_cpu()
     print(json.dumps(out), flush=True)
 
 
@@ -584,6 +598,7 @@ def main():
             "threads": nt, "mlups": round(mlups_n, 3),
             "solves_per_hr": round(3600.0 * summ["cold"]["converged"] / (sum(c_steps) * cells / (mlups_n * 1e6)), 4),
             "sample": f"{n_n} steps in {t_n:.1f} s"}
+        line["cpu_baseline"]["host"] = host_cpu()
     if args.rows:
         with open(args.rows, "w") as f:
             json.dump({"line": line, "rows": rows}, f)
