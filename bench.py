@@ -97,10 +97,12 @@ def pass_counters():
         return None
 
 
-def algorithmic_bytes_per_pass(mask, rsize=4):
+def algorithmic_bytes_per_pass(mask, rsize=4, thermal_q=19):
+    """Populations read and written once per pass: 19 per non-solid cell (compact fluid lattice) and
+    ``thermal_q`` per cell (dense thermal lattice)."""
     n_all = mask.size
     n_fluid = int(np.sum(mask != 1))
-    return 19 * rsize * 2 * (n_fluid + n_all)
+    return rsize * 2 * (19 * n_fluid + thermal_q * n_all)
 
 
 def load_design(batch, d, Pr=7.0, k_ratio=628.0, T_in=25.0, slot=0):
@@ -149,16 +151,18 @@ def cpu_baseline(design, seconds, threads):
     return mask.size * steps / dt / 1e6, steps, dt
 
 
-def profile_passes(npasses, slots=1, seed=1000, **launch):
+def profile_passes(npasses, slots=1, seed=1000, thermal_lattice="D3Q19", **launch):
     """Only fused passes on ``slots`` config-2 designs (seeds from ``seed``; graph chunks of 100, fewer when
     npasses is smaller), for kernel traces and counters.  One slot (~140 MB touched per pass) stays in the
-    256 MB Infinity Cache; many slots are HBM-resident."""
+    256 MB Infinity Cache; many slots are HBM-resident.  With the D3Q7 thermal lattice a pass is the fluid
+    pass followed by the D3Q7 thermal kernel (56 B per cell of thermal populations)."""
     import torch
     from xadvcool_amd.engine import LatticeBatch
     from xadvcool_amd.geometry import config2_design
     designs = [config2_design(seed + i) for i in range(slots)]
     chunk = 100 if npasses >= 100 else max(2, npasses - npasses % 2)
-    batch = LatticeBatch(slots=slots, cap_shape=LATTICE, graph_chunk=chunk, **launch)
+    batch = LatticeBatch(slots=slots, cap_shape=LATTICE, graph_chunk=chunk, thermal_lattice=thermal_lattice,
+                         **launch)
     for s, d in enumerate(designs):
         load_design(batch, d, slot=s)
     batch.step(200, want_fields=False)  # past the start-up transient
@@ -172,8 +176,10 @@ def profile_passes(npasses, slots=1, seed=1000, **launch):
         timer.end(n)
         done += n
     ms = timer.total_ms() / max(done, 1)
-    nb = sum(algorithmic_bytes_per_pass(d["mask"]) for d in designs)
-    print(json.dumps({"mode": "passes", "slots": slots, **launch, "passes": done, "avg_pass_us": round(1000 * ms, 2),
+    q = 7 if thermal_lattice == "D3Q7" else 19
+    nb = sum(algorithmic_bytes_per_pass(d["mask"], thermal_q=q) for d in designs)
+    print(json.dumps({"mode": "passes", "slots": slots, "thermal_lattice": thermal_lattice, **launch,
+                      "passes": done, "avg_pass_us": round(1000 * ms, 2),
                       "alg_bytes_per_pass": nb, "achieved_GBs": round(nb / (ms * 1e-3) / 1e9, 1)}), flush=True)
     batch.close()
 
@@ -443,6 +449,8 @@ def main():
     ap.add_argument("--xcd", action="store_true", help="map consecutive lattice blocks to one XCD")
     ap.add_argument("--slab", type=int, default=0,
                     help="passes/batch modes: x-planes per slab of the wavefront pass pairs (0: whole passes)")
+    ap.add_argument("--thermal-lattice", choices=["D3Q19", "D3Q7"], default="D3Q19",
+                    help="passes mode: thermal lattice (D3Q7: the fluid pass then the D3Q7 thermal kernel)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
     args = ap.parse_args()
     if args.mode == "solves" and args.designs is None:
@@ -454,7 +462,8 @@ def main():
         sys.exit(2)
 
     if args.mode == "passes":
-        return profile_passes(args.passes, args.slots or 1, block=args.block, cells_per_thread=args.cpt,
+        return profile_passes(args.passes, args.slots or 1, thermal_lattice=args.thermal_lattice,
+                              block=args.block, cells_per_thread=args.cpt,
                               xcd_remap=args.xcd, alternate_sweep=args.sweep == "alternate", slab_planes=args.slab)
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
