@@ -66,8 +66,8 @@ typedef struct {
                            (no reference counterpart): seven velocities, weights 1/4 and 1/8,
                            g_eq = w T (1 + 4 e.u), relaxed at the diffusivity of the instance's
                            tau_th and heated with the reference step's factor 1 - 1/(2 tau_th);
-                           needs the fluid lattice and general = 0.  A step is the fused fluid
-                           pass (storing u) followed by one D3Q7 kernel. */
+                           needs the fluid lattice and general = 0.  A step is one fused pass:
+                           the fluid lattice, then the D3Q7 step with the velocity just computed. */
   int slab_planes;      /* 0: each pass is one launch over whole lattices.  n > 0: runs of an even
                            number of steps starting on an odd pass go as pairs of passes split into
                            x-slabs of n planes, launched as a wavefront (odd pass on slab k, then the

@@ -1,4 +1,4 @@
-"""The D3Q7 thermal variant on the GPU (k_thermal7 after the fused fluid pass) against the oracle's D3Q7
+"""The D3Q7 thermal variant on the GPU (the fused pass with thermal lattice 7) against the oracle's D3Q7
 schedule (thermal schedule 3 of oracle/lbm_oracle.cpp, pinned by tests/test_oracle_d3q7.py): rho, u and T
 bit for bit in FP32 and FP64, cold and warm starts, every launch shape, graph chunks, two slots of
 different lattices with one refilled late, and config 1 through the drop-in run_simulation."""

@@ -608,6 +608,101 @@ __device__ __forceinline__ void thermal_scatter(const Slot<R>& S, const Cell& k,
   }
 }
 
+// ---------------------------------------------------------- D3Q7 thermal ----
+// D3Q7 variant of the thermal lattice.  Not a reference kernel: the
+// reference's thermal lattice is D3Q19.  It solves the same advection-diffusion
+// problem on directions 0..6 of the D3Q19 set (rest and the six faces, with
+// their opposite pairs), weights 1/4 and 1/8 (c_s^2 = 1/4), the linear
+// equilibrium g_eq_i = w_i T (1 + 4 e_i.u) (the quadratic terms are not
+// isotropic on seven velocities).  A slot's relaxation times are the
+// reference's: omega_tf / omega_ts hold 1/tau7 with tau7 = 1/2 + 4/3 (tau - 1/2)
+// (the same diffusivity) and sf_tf / sf_ts the reference step's source factor
+// 1 - 1/(2 tau), so the heat injected per step is the reference's.  Streaming
+// follows thermal_engine3d.py:147-211 for these directions.  Two buffers of 7
+// planes per slot (g7, g7 + 7 npad): odd steps read the first and write the
+// second, even steps the reverse.  The fused pass (thermal lattice 7) runs a
+// cell's D3Q7 step after the fluid half of the same step, with the velocity that
+// half has just computed (u = 0 in solids).  The first odd step after an init
+// takes the initial g and T as given (raw), as the D3Q19 thermal half does; the
+// next even step clears the flag.
+template <class R>
+__device__ __forceinline__ R w7(int i) {
+  return i == 0 ? (R)0.25 : (R)0.125;
+}
+
+// Streamed D3Q7 populations of cell c from the post-collision buffer src.
+template <class R>
+__device__ __forceinline__ void gather7(const Slot<R>& S, const R* __restrict__ src, uint32_t c, uint32_t fl,
+                                        R* g) {
+  const int type = (int)(fl & F_TYPE);
+  const uint32_t N = (uint32_t)S.npad;
+  if (type == BC_INLET) {  // thermal_engine3d.py:160-165
+#pragma unroll
+    for (int i = 0; i < 7; ++i) g[i] = w7<R>(i) * S.T_in;
+    return;
+  }
+  if (type == BC_OUTLET) {  // :170-177 (x - 1, or the cell itself on x = 0)
+    const uint32_t s = (fl & F_FX0) ? c : c - (uint32_t)S.P;
+#pragma unroll
+    for (int i = 0; i < 7; ++i) g[i] = ld(src, i * N + s);
+    return;
+  }
+  // :180-211: the source c - e_i; off the lattice: x < 0 -> w_i T_in, x >= nx -> own value, y/z -> opposite.
+  // Every load's index is selected first (the neighbour, or the cell's own slot), so no load leaves the slot.
+  const uint32_t P = (uint32_t)S.P, nz = (uint32_t)S.nz;
+  g[0] = ld(src, c);
+  const R v1 = ld(src, N + ((fl & F_FX0) ? c : c - P));
+  g[1] = (fl & F_FX0) ? w7<R>(1) * S.T_in : v1;
+  g[2] = ld(src, 2 * N + ((fl & F_FXN) ? c : c + P));
+  g[3] = ld(src, (fl & F_FY0) ? 4 * N + c : 3 * N + c - nz);
+  g[4] = ld(src, (fl & F_FYN) ? 3 * N + c : 4 * N + c + nz);
+  g[5] = ld(src, (fl & F_FZ0) ? 6 * N + c : 5 * N + c - 1);
+  g[6] = ld(src, (fl & F_FZN) ? 5 * N + c : 6 * N + c + 1);
+}
+
+// The streamed populations of a thread's cells from the post-collision buffer
+// src (or, on the raw step, the initial g and T as given) and T = sum of g in
+// order from g_0.
+template <class R, int CPT>
+__device__ __forceinline__ void thermal7_load(const Slot<R>& S, const R* __restrict__ src, const Cell* cell, int raw,
+                                              R (*g)[7], R* T) {
+  const uint32_t N = (uint32_t)S.npad;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k) {
+    if (!cell[k].valid) continue;
+    const uint32_t c = cell[k].c;
+    if (raw) {
+#pragma unroll
+      for (int i = 0; i < 7; ++i) g[k][i] = ld(src, i * N + c);
+      T[k] = ld(S.T, c);
+    } else {
+      gather7<R>(S, src, c, cell[k].fl, g[k]);
+      T[k] = 0;
+#pragma unroll
+      for (int i = 0; i < 7; ++i) T[k] += g[k][i];
+    }
+  }
+}
+
+// BGK relaxation towards g_eq(T, u) at the relaxation time of the cell's kind,
+// plus w_i (1 - 1/(2 tau)) q on heated cells, stored into dst.
+template <class R>
+__device__ __forceinline__ void thermal7_collide(const Slot<R>& S, R* __restrict__ dst, const Cell& k, const R* g, R T,
+                                                 R ux, R uy, R uz) {
+  const uint32_t N = (uint32_t)S.npad;
+  const R om = k.type == BC_SOLID ? S.omega_ts : S.omega_tf;
+  const R sf = k.type == BC_SOLID ? S.sf_ts : S.sf_tf;
+  const R q = (k.fl & F_HEAT) ? ld(S.q, k.c) : (R)0;
+  const R eu[7] = {(R)0, ux, -ux, uy, -uy, uz, -uz};
+#pragma unroll
+  for (int i = 0; i < 7; ++i) {
+    const R geq = w7<R>(i) * T * ((R)1.0 + (R)4.0 * eu[i]);
+    R v = g[i] - om * (g[i] - geq);
+    if (q > (R)0.0) v += w7<R>(i) * sf * q;
+    st(dst, i * N + k.c, v);
+  }
+}
+
 // ------------------------------------------------------------- the pass ----
 
 #ifndef XADV_TH_PREFETCH
@@ -653,12 +748,16 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // x-planes [x0, x1) of every slot (cells x0*P .. x1*P - 1; the whole lattice
 // for x0 = 0, x1 >= nx); clear_raw says whether this even launch may clear the
 // raw flag (only the last launch of an even pass split over x-slabs does, so
-// every launch of the odd pass before it has read the flag).
+// every launch of the odd pass before it has read the flag).  TH = 1: the D3Q19
+// thermal lattice one step behind the fluid (AA in place); TH = 7: the D3Q7
+// variant of the same step (two buffers), its 7 loads per cell issued right
+// after the fluid gather so they overlap the fluid lattice's.
 template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP>
 __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int order,
                                                              int want_fields, int x0, int x1, int clear_raw) {
   static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
-  constexpr bool PRE = XADV_TH_PREFETCH && FL && TH;
+  static_assert(TH == 0 || TH == 1 || (TH == 7 && FL && !GEN), "thermal lattice: none, D3Q19, or D3Q7 with the fluid");
+  constexpr bool PRE = XADV_TH_PREFETCH && FL && TH == 1;
   const int b = map_block(bps, order);
   if (b >= G) return;
   const int si = b / bps;
@@ -696,6 +795,11 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
   for (int k = 0; k < CPT; ++k) ux[k] = uy[k] = uz[k] = 0;
   R g[CPT][19];
   R Tl[CPT];
+  R g7[TH == 7 ? CPT : 1][7];
+  R T7[TH == 7 ? CPT : 1];
+  const uint32_t N7 = TH == 7 ? (uint32_t)S.npad : 0u;
+  const R* __restrict__ src7 = TH == 7 ? (NODD ? S.g7 : S.g7 + 7 * (size_t)N7) : nullptr;
+  R* __restrict__ dst7 = TH == 7 ? (NODD ? S.g7 + 7 * (size_t)N7 : S.g7) : nullptr;
 
   if (FL) {
     R f[CPT][19];
@@ -707,6 +811,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
         fluid_gather<R, GEN, NODD, CMP>(S, cell[k], J[k], f[k]);
       }
     if (PRE) thermal_load<R, GEN, NODD, CPT>(S, cell, raw, g, Tl);
+    if constexpr (TH == 7) thermal7_load<R, CPT>(S, src7, cell, raw, g7, T7);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
       const Cell& q = cell[k];
@@ -737,7 +842,7 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
         st(S.u, 3 * q.c + 2, sz);
       }
     }
-  } else if (TH) {
+  } else if (TH == 1) {
     // Thermal lattice alone: collide with the velocity field given at init.
 #pragma unroll
     for (int k = 0; k < CPT; ++k)
@@ -748,7 +853,13 @@ __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict
       }
   }
 
-  if (TH) {
+  if constexpr (TH == 7) {
+#pragma unroll
+    for (int k = 0; k < CPT; ++k)
+      if (cell[k].valid) thermal7_collide<R>(S, dst7, cell[k], g7[k], T7[k], ux[k], uy[k], uz[k]);
+  }
+
+  if (TH == 1) {
     if (!PRE) thermal_load<R, GEN, NODD, CPT>(S, cell, raw, g, Tl);
 #pragma unroll
     for (int k = 0; k < CPT; ++k) {
@@ -882,103 +993,7 @@ __global__ void k_fill(R* __restrict__ p, int64_t n, R v) {
   if (i < n) p[i] = v;
 }
 
-// ---------------------------------------------------------- D3Q7 thermal ----
-// D3Q7 variant of the thermal lattice.  Not a reference kernel: the
-// reference's thermal lattice is D3Q19.  It solves the same advection-diffusion
-// problem on directions 0..6 of the D3Q19 set (rest and the six faces, with
-// their opposite pairs), weights 1/4 and 1/8 (c_s^2 = 1/4), the linear
-// equilibrium g_eq_i = w_i T (1 + 4 e_i.u) (the quadratic terms are not
-// isotropic on seven velocities).  A slot's relaxation times are the
-// reference's: omega_tf / omega_ts hold 1/tau7 with tau7 = 1/2 + 4/3 (tau - 1/2)
-// (the same diffusivity) and sf_tf / sf_ts the reference step's source factor
-// 1 - 1/(2 tau), so the heat injected per step is the reference's.  Streaming
-// follows thermal_engine3d.py:147-211 for these directions.  Two buffers of 7
-// planes per slot (g7, g7 + 7 npad): odd steps read the first and write the
-// second, even steps the reverse; a step runs after the fluid pass of the same
-// step and collides with the velocity that pass stored (u = 0 in solids).  The
-// first odd step after an init takes the initial g and T as given (raw), as the
-// fused pass's thermal half does; the next even step clears the flag.
-template <class R>
-__device__ __forceinline__ R w7(int i) {
-  return i == 0 ? (R)0.25 : (R)0.125;
-}
-
-// Streamed D3Q7 populations of cell c from the post-collision buffer src.
-template <class R>
-__device__ __forceinline__ void gather7(const Slot<R>& S, const R* __restrict__ src, uint32_t c, uint32_t fl,
-                                        R* g) {
-  const int type = (int)(fl & F_TYPE);
-  const uint32_t N = (uint32_t)S.npad;
-  if (type == BC_INLET) {  // thermal_engine3d.py:160-165
-#pragma unroll
-    for (int i = 0; i < 7; ++i) g[i] = w7<R>(i) * S.T_in;
-    return;
-  }
-  if (type == BC_OUTLET) {  // :170-177 (x - 1, or the cell itself on x = 0)
-    const uint32_t s = (fl & F_FX0) ? c : c - (uint32_t)S.P;
-#pragma unroll
-    for (int i = 0; i < 7; ++i) g[i] = ld(src, i * N + s);
-    return;
-  }
-  // :180-211: the source c - e_i; off the lattice: x < 0 -> w_i T_in, x >= nx -> own value, y/z -> opposite.
-  // Every load's index is selected first (the neighbour, or the cell's own slot), so no load leaves the slot.
-  const uint32_t P = (uint32_t)S.P, nz = (uint32_t)S.nz;
-  g[0] = ld(src, c);
-  const R v1 = ld(src, N + ((fl & F_FX0) ? c : c - P));
-  g[1] = (fl & F_FX0) ? w7<R>(1) * S.T_in : v1;
-  g[2] = ld(src, 2 * N + ((fl & F_FXN) ? c : c + P));
-  g[3] = ld(src, (fl & F_FY0) ? 4 * N + c : 3 * N + c - nz);
-  g[4] = ld(src, (fl & F_FYN) ? 3 * N + c : 4 * N + c + nz);
-  g[5] = ld(src, (fl & F_FZ0) ? 6 * N + c : 5 * N + c - 1);
-  g[6] = ld(src, (fl & F_FZN) ? 5 * N + c : 6 * N + c + 1);
-}
-
-template <class R>
-__global__ __launch_bounds__(256) void k_thermal7(Slot<R>* __restrict__ slots, int bps, int G, int order, int nodd) {
-  const int b = map_block(bps, order);
-  if (b >= G) return;
-  const int si = b / bps;
-  const int lb = b - si * bps;
-  Slot<R>& S = slots[si];
-  if (!S.active) return;
-  const bool raw = nodd && S.raw;
-  if (!nodd && lb == 0 && threadIdx.x == 0) S.raw = 0;  // read by odd steps only
-  const uint32_t c = (uint32_t)lb * blockDim.x + threadIdx.x;
-  if (c >= (uint32_t)S.ncell) return;
-  const uint32_t N = (uint32_t)S.npad;
-  const R* __restrict__ src = nodd ? S.g7 : S.g7 + 7 * (size_t)N;
-  R* __restrict__ dst = nodd ? S.g7 + 7 * (size_t)N : S.g7;
-  const uint32_t fl = ld(S.flags, c);
-  const int type = (int)(fl & F_TYPE);
-  R g[7], T;
-  if (raw) {
-#pragma unroll
-    for (int i = 0; i < 7; ++i) g[i] = ld(src, i * N + c);
-    T = ld(S.T, c);
-  } else {
-    gather7<R>(S, src, c, fl, g);
-    T = 0;
-#pragma unroll
-    for (int i = 0; i < 7; ++i) T += g[i];
-  }
-  R ux = 0, uy = 0, uz = 0;
-  if (type != BC_SOLID) {
-    ux = ld(S.u, 3 * c);
-    uy = ld(S.u, 3 * c + 1);
-    uz = ld(S.u, 3 * c + 2);
-  }
-  const R om = type == BC_SOLID ? S.omega_ts : S.omega_tf;
-  const R sf = type == BC_SOLID ? S.sf_ts : S.sf_tf;
-  const R q = (fl & F_HEAT) ? ld(S.q, c) : (R)0;
-  const R eu[7] = {(R)0, ux, -ux, uy, -uy, uz, -uz};
-#pragma unroll
-  for (int i = 0; i < 7; ++i) {
-    const R geq = w7<R>(i) * T * ((R)1.0 + (R)4.0 * eu[i]);
-    R v = g[i] - om * (g[i] - geq);
-    if (q > (R)0.0) v += w7<R>(i) * sf * q;
-    st(dst, i * N + c, v);
-  }
-}
+// ---------------------------------------------------------- D3Q7 init ----
 
 // Initial D3Q7 populations g_eq(T, u) (u = 0 in solids) into the buffer the first odd step reads.
 template <class R>

@@ -53,7 +53,7 @@ struct xadv_batch {
   void* slots_host = nullptr;  // pinned copy of the slot table
   size_t slot_bytes = 0;
   int compact = 0;  // fluid lattice stored for non-solid cells only
-  int q7 = 0;       // thermal lattice is the D3Q7 variant (k_thermal7 after each fluid pass; d.thermal is 0)
+  int q7 = 0;       // thermal lattice is the D3Q7 variant (fused into the fluid pass; d.thermal is 0)
   int64_t cols_per_slot = 0;  // padded column-table entries per slot
   // arenas (bytes)
   char *f = nullptr, *g = nullptr, *flags = nullptr, *q = nullptr, *fin = nullptr, *fsnap = nullptr, *gsnap = nullptr,
@@ -158,13 +158,6 @@ void launch_readout_t(xadv_batch* b, int nodd, int with_stats) {
                        b->d.xcd_remap, with_stats, b->partials, b->q7);
 }
 
-// D3Q7 thermal step of every slot (after the fluid pass of the same step), one cell per thread.
-template <class R>
-void launch_thermal7(xadv_batch* b, int nodd) {
-  hipLaunchKernelGGL((xadv::k_thermal7<R>), dim3(b->G_r_launch), dim3(b->d.block), 0, b->stream,
-                     reinterpret_cast<Slot<R>*>(b->slots_dev), b->bps_r, b->G_r, b->d.xcd_remap, nodd);
-}
-
 template <class R, int FL, int TH, int GEN>
 void launch_init_t(xadv_batch* b, int slot, int64_t ncell, int frozen) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
@@ -191,18 +184,17 @@ void launch_init_t(xadv_batch* b, int slot, int64_t ncell, int frozen) {
   } while (0)
 
 void launch_pass(xadv_batch* b, int nodd, int want_fields, const PassRange& r = PassRange{}) {
-  // With the D3Q7 thermal variant every fluid pass stores u, which its thermal step collides with.
-  if (b->q7) want_fields = 1;
+  if (b->q7) {  // fluid + D3Q7 thermal in one pass (d.thermal is 0 for the other kernels)
+    if (b->d.precision == 64)
+      launch_pass_t<double, 1, 7, 0>(b, nodd, want_fields, r);
+    else
+      launch_pass_t<float, 1, 7, 0>(b, nodd, want_fields, r);
+    return;
+  }
   if (b->d.precision == 64)
     XADV_DISPATCH(launch_pass_t, double, b, nodd, want_fields, r);
   else
     XADV_DISPATCH(launch_pass_t, float, b, nodd, want_fields, r);
-  if (b->q7) {
-    if (b->d.precision == 64)
-      launch_thermal7<double>(b, nodd);
-    else
-      launch_thermal7<float>(b, nodd);
-  }
 }
 
 // Two passes (odd, then even) split over x-slabs of slab_planes planes and run as a wavefront: A_k is the
@@ -585,7 +577,7 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   const bool q7 = d.thermal && d.thermal_q == 7;
   if (q7 && (!d.fluid || d.general))
     return fail(XADV_E_ARG, "the D3Q7 thermal lattice runs coupled to the fluid lattice, channel boundaries only");
-  if (q7) d.thermal = 0;  // the fused passes run the fluid lattice only; k_thermal7 does the thermal step
+  if (q7) d.thermal = 0;  // init and readout run their fluid-only kernels and the D3Q7 ones beside them
   if (d.slab_planes < 0) return fail(XADV_E_ARG, "slab_planes must be >= 0");
   if (d.slab_planes > 0 && (q7 || d.general))
     return fail(XADV_E_ARG, "x-slab pairs of passes need the D3Q19 lattices and general = 0 (no periodic x axis)");
