@@ -18,8 +18,8 @@ designs on the GPU from their parameters, the solves, metrics, checks and,
 for the warm arm, the surrogate inference and any fallback.  ``--steps``
 does not change the workload (a step is one design solved to the stop
 rule); ``--designs`` does.  The fused pass runs 256 threads x 2 cells,
-linear block order, alternating sweep (``--launch tune`` runs the project's
-autotuner first instead).
+linear block order, alternating sweep, thermal loads issued right after the
+fluid gather (``--launch tune`` runs the project's autotuner first instead).
 
 Printed (one JSON line): warm-start converged solves/hour (``value``), the
 cold-start figure and their ratio, the agreement of the converged pressure
@@ -449,6 +449,10 @@ def main():
     ap.add_argument("--xcd", action="store_true", help="map consecutive lattice blocks to one XCD")
     ap.add_argument("--slab", type=int, default=0,
                     help="passes/batch modes: x-planes per slab of the wavefront pass pairs (0: whole passes)")
+    ap.add_argument("--thermal-prefetch", choices=["auto", "on", "off"], default="auto",
+                    help="fused D3Q19 pass: issue the thermal loads right after the fluid gather (faster at a few "
+                         "lattices per launch, slower when the state is HBM-resident); auto: on in the solves "
+                         "mode (three lattices per launch), off elsewhere")
     ap.add_argument("--thermal-lattice", choices=["D3Q19", "D3Q7"], default="D3Q19",
                     help="passes mode: thermal lattice (D3Q7: the fluid pass then the D3Q7 thermal kernel)")
     ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
@@ -463,8 +467,9 @@ def main():
 
     if args.mode == "passes":
         return profile_passes(args.passes, args.slots or 1, thermal_lattice=args.thermal_lattice,
-                              block=args.block, cells_per_thread=args.cpt,
-                              xcd_remap=args.xcd, alternate_sweep=args.sweep == "alternate", slab_planes=args.slab)
+                              thermal_prefetch=args.thermal_prefetch == "on", block=args.block,
+                              cells_per_thread=args.cpt, xcd_remap=args.xcd,
+                              alternate_sweep=args.sweep == "alternate", slab_planes=args.slab)
     if args.mode == "copy":
         return profile_copy(args.copy_mb)
     if args.mode == "ceilings":
@@ -504,7 +509,7 @@ def main():
     net = S.UNet3D.load_file(args.weights)
     nslots = args.slots or 3
     launch = dict(block=args.block, cells_per_thread=args.cpt, xcd_remap=args.xcd,
-                  alternate_sweep=args.sweep == "alternate")
+                  alternate_sweep=args.sweep == "alternate", thermal_prefetch=args.thermal_prefetch != "off")
     tuned_us = None
     if args.launch == "tune":
         # Untimed set-up, like a compile step: the project's autotuner times each launch shape on ``nslots``
@@ -513,7 +518,8 @@ def main():
         tune_designs = [config2_design(sd) for sd in range(900, 900 + nslots)]
         best, times = autotune.tune(tune_designs, LATTICE, passes=300, save=False,
                                     candidates=[c for c in autotune.CANDIDATES if c["block"] >= 128])
-        launch = {k: best[k] for k in ("block", "cells_per_thread", "xcd_remap", "alternate_sweep")}
+        launch = dict({k: best[k] for k in ("block", "cells_per_thread", "xcd_remap", "alternate_sweep")},
+                      thermal_prefetch=args.thermal_prefetch != "off")
         tuned_us = round(times[json.dumps(best, sort_keys=True)], 2)
         print(f"[bench] autotuned launch {launch}: {tuned_us} us/pass", file=sys.stderr, flush=True)
     kw = dict(slots=nslots, tol=args.tol, max_steps=args.max_steps, launch=launch, stop_rule=args.stop_rule,

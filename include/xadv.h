@@ -74,6 +74,12 @@ typedef struct {
                            even pass on slab k - 1), so a slab's even pass finds the lines its odd pass
                            touched still in the Infinity Cache; the sweep direction alternates between
                            pairs.  Same bits.  Needs thermal_q 19 and general = 0. */
+  int thermal_prefetch; /* 1: the fused D3Q19 pass issues its thermal loads right after the fluid
+                           gather (38 loads per cell in flight, more registers).  Measured on
+                           148x128x32 designs: 1.6 % less time per pass at three lattices per
+                           launch (partly resident in the Infinity Cache), 3.6 % more at 64
+                           (HBM).  Same bits.  Needs the D3Q19 thermal lattice and compact fluid
+                           storage. */
 } xadv_batch_desc;
 
 #define XADV_STORAGE_AUTO 0

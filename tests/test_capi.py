@@ -67,4 +67,10 @@ def test_bad_arguments_are_errors_not_crashes():
         d = _lib.BatchDesc(32, fluid, 1, general, 1, 4, 4, 4, 256, 0, 0, 0, 0, 1, 7)
         assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
         assert b"D3Q7" in lib.xadv_last_error()
+    # the thermal prefetch is a 0/1 flag of the fused D3Q19 pass on compact storage: not with dense storage,
+    # the thermal lattice off, or the D3Q7 lattice
+    for storage, thermal, q, flag in ((0, 1, 19, 2), (_lib.STORAGE_DENSE, 1, 19, 1), (0, 0, 19, 1), (0, 1, 7, 1)):
+        d = _lib.BatchDesc(32, 1, thermal, 0, 1, 4, 4, 4, 256, 0, 0, 0, storage, 1, q, 0, flag)
+        assert lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)) == _lib.XADV_E_ARG
+        assert b"thermal_prefetch" in lib.xadv_last_error()
     assert lib.xadv_step(None, 1, 0) == _lib.XADV_E_ARG

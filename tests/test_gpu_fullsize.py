@@ -3,8 +3,8 @@
 test_gpu_parity.py pins every launch shape on small lattices.  This test pins the configuration
 bench.py measures: three 148x128x32 gyroid designs per launch (seeds 1000-1002, the benchmark's
 first three), compact fluid storage, graph-captured chunks, 256 threads x 2 cells, linear order,
-alternating sweep (and the XCD-contiguous order).  After 24 steps every slot's FP32 rho, u and T
-equal the oracle's bit for bit.
+alternating sweep, thermal loads issued right after the fluid gather (and the XCD-contiguous order
+without that prefetch).  After 24 steps every slot's FP32 rho, u and T equal the oracle's bit for bit.
 """
 import numpy as np
 import pytest
@@ -37,7 +37,8 @@ def cases():
     return out
 
 
-@pytest.mark.parametrize("shape", [dict(block=256, cells_per_thread=2, xcd_remap=False, alternate_sweep=True),
+@pytest.mark.parametrize("shape", [dict(block=256, cells_per_thread=2, xcd_remap=False, alternate_sweep=True,
+                                        thermal_prefetch=True),
                                    dict(block=256, cells_per_thread=1, xcd_remap=True, alternate_sweep=True)],
                          ids=["bench-shape", "xcd-order"])
 def test_three_fullsize_designs_per_launch_match_oracle(gpu, cases, shape):

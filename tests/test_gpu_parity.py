@@ -106,33 +106,37 @@ def test_coupled_fp32_matches_oracle(gpu, steps, storage):
     assert_match("T", T, ref.T)
 
 
-@pytest.mark.parametrize("block,cpt,xcd,alt,storage", [(64, 1, False, False, "compact"),
-                                                       (128, 2, True, False, "compact"),
-                                                       (256, 1, True, False, "compact"),
-                                                       (64, 2, False, False, "compact"),
-                                                       (128, 2, True, False, "dense"),
-                                                       (256, 1, False, False, "dense"),
-                                                       (256, 1, False, True, "compact"),
-                                                       (64, 2, True, True, "compact"),
-                                                       (128, 1, False, True, "dense")])
-def test_coupled_launch_shapes(gpu, block, cpt, xcd, alt, storage):
-    """Every launch shape the autotuner may pick computes the same bits, in either block order."""
-    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd} alt={alt} storage={storage}", flush=True)
+@pytest.mark.parametrize("block,cpt,xcd,alt,storage,pre", [(64, 1, False, False, "compact", False),
+                                                           (128, 2, True, False, "compact", False),
+                                                           (256, 1, True, False, "compact", False),
+                                                           (64, 2, False, False, "compact", False),
+                                                           (128, 2, True, False, "dense", False),
+                                                           (256, 1, False, False, "dense", False),
+                                                           (256, 1, False, True, "compact", False),
+                                                           (64, 2, True, True, "compact", False),
+                                                           (128, 1, False, True, "dense", False),
+                                                           (256, 2, False, True, "compact", True),
+                                                           (64, 1, True, False, "compact", True)])
+def test_coupled_launch_shapes(gpu, block, cpt, xcd, alt, storage, pre):
+    """Every launch shape the autotuner may pick computes the same bits, in either block order, with the thermal
+    loads issued after the fluid gather or with the fluid stores."""
+    print(f"[parity] coupled shape block={block} cpt={cpt} xcd={xcd} alt={alt} storage={storage} prefetch={pre}",
+          flush=True)
     p = coupled_params(seed=2, shape=(23, 13, 11))
     ref = oracle_coupled(p, 13, np.float32)
     r, u, T = hip_coupled(p, 13, 32, block=block, cells_per_thread=cpt, xcd_remap=xcd, alternate_sweep=alt,
-                          fluid_storage=storage)
+                          fluid_storage=storage, thermal_prefetch=pre)
     assert_match("rho", r, ref.rho)
     assert_match("u", u, ref.u)
     assert_match("T", T, ref.T)
 
 
-@pytest.mark.parametrize("storage", ["compact", "dense"])
-def test_coupled_fp64_matches_oracle(gpu, storage):
-    print(f"[parity] coupled fp64 steps=20 storage={storage}", flush=True)
+@pytest.mark.parametrize("storage,pre", [("compact", False), ("dense", False), ("compact", True)])
+def test_coupled_fp64_matches_oracle(gpu, storage, pre):
+    print(f"[parity] coupled fp64 steps=20 storage={storage} prefetch={pre}", flush=True)
     p = coupled_params(seed=3)
     ref = oracle_coupled(p, 20, np.float64)
-    r, u, T = hip_coupled(p, 20, 64, fluid_storage=storage)
+    r, u, T = hip_coupled(p, 20, 64, fluid_storage=storage, thermal_prefetch=pre)
     assert_match("rho", r, ref.rho, tol=1e-12)
     assert_match("u", u, ref.u, tol=1e-12)
     assert_match("T", T, ref.T, tol=1e-12)

@@ -30,7 +30,7 @@ class BatchDesc(ctypes.Structure):
         ("slots", ctypes.c_int), ("cap_nx", ctypes.c_int), ("cap_ny", ctypes.c_int), ("cap_nz", ctypes.c_int),
         ("block", ctypes.c_int), ("xcd_remap", ctypes.c_int), ("graph_chunk", ctypes.c_int),
         ("cells_per_thread", ctypes.c_int), ("fluid_storage", ctypes.c_int), ("alternate_sweep", ctypes.c_int),
-        ("thermal_q", ctypes.c_int), ("slab_planes", ctypes.c_int),
+        ("thermal_q", ctypes.c_int), ("slab_planes", ctypes.c_int), ("thermal_prefetch", ctypes.c_int),
     ]
 
 

@@ -65,13 +65,14 @@ class BatchedSolver:
                  max_steps=30000, min_steps_rule=False, Pr=7.0, k_ratio=628.0, t_inlet=25.0, precision=32,
                  block=256, cells_per_thread=1, xcd_remap=False, alternate_sweep=True, fluid_storage="auto",
                  collect_fields=False,
-                 metrics=True, stall_check=None, confirm=1, stop_rule="reference", checks=False, slab_planes=0):
+                 metrics=True, stall_check=None, confirm=1, stop_rule="reference", checks=False, slab_planes=0,
+                 thermal_prefetch=False):
         self.slots = int(slots)
         self.cap_shape = tuple(cap_shape)
         self.batch = LatticeBatch(slots=self.slots, cap_shape=self.cap_shape, precision=precision, block=block,
                                   cells_per_thread=cells_per_thread, xcd_remap=xcd_remap, graph_chunk=stats_every,
                                   fluid_storage=fluid_storage, alternate_sweep=alternate_sweep,
-                                  slab_planes=slab_planes)
+                                  slab_planes=slab_planes, thermal_prefetch=thermal_prefetch)
         self.every = int(stats_every)
         self.tol_ke, self.tol_T = tol_ke, tol_T
         self.max_steps = int(max_steps)

@@ -705,9 +705,6 @@ __device__ __forceinline__ void thermal7_collide(const Slot<R>& S, R* __restrict
 
 // ------------------------------------------------------------- the pass ----
 
-#ifndef XADV_TH_PREFETCH
-#define XADV_TH_PREFETCH 0
-#endif
 #ifndef XADV_PASS_WAVES
 #define XADV_PASS_WAVES 0
 #endif
@@ -739,10 +736,11 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // One time step for CPT cells per thread.  The phases (fluid gather, fluid
 // collide + scatter, thermal gather, thermal collide + scatter) run for all
 // of a thread's cells before the next phase, so each thread keeps
-// CPT x 19 loads in flight.  With XADV_TH_PREFETCH the thermal gather is
-// issued right after the fluid gather (38 loads in flight; the two lattices
-// are separate arrays and each slot read here is written only by this cell,
-// so the order of the fluid stores and thermal loads does not matter).
+// CPT x 19 loads in flight.  With PRE the D3Q19 thermal gather is issued
+// right after the fluid gather (38 loads in flight, more registers; the two
+// lattices are separate arrays and each slot read here is written only by
+// this cell, so the order of the fluid stores and thermal loads does not
+// matter).
 // Cell k of a thread is block_base + k*blockDim + threadIdx, so every load
 // instruction of a wavefront covers 64 consecutive cells.  A launch covers the
 // x-planes [x0, x1) of every slot (cells x0*P .. x1*P - 1; the whole lattice
@@ -752,12 +750,12 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // thermal lattice one step behind the fluid (AA in place); TH = 7: the D3Q7
 // variant of the same step (two buffers), its 7 loads per cell issued right
 // after the fluid gather so they overlap the fluid lattice's.
-template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP>
+template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP, int PRE = 0>
 __global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int order,
                                                              int want_fields, int x0, int x1, int clear_raw) {
   static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
   static_assert(TH == 0 || TH == 1 || (TH == 7 && FL && !GEN), "thermal lattice: none, D3Q19, or D3Q7 with the fluid");
-  constexpr bool PRE = XADV_TH_PREFETCH && FL && TH == 1;
+  static_assert(!PRE || (FL && TH == 1), "the thermal prefetch is for the fused D3Q19 pass");
   const int b = map_block(bps, order);
   if (b >= G) return;
   const int si = b / bps;

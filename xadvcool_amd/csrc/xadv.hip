@@ -115,24 +115,29 @@ struct PassRange {
   bool slab = false;
 };
 
-template <class R, int FL, int TH, int GEN, int CPT, int CMP>
+template <class R, int FL, int TH, int GEN, int CPT, int CMP, int PRE = 0>
 void launch_pass_c(xadv_batch* b, int nodd, int want_fields, const PassRange& r) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
   const int bps = r.bps ? r.bps : b->bps, G = r.G ? r.G : b->G;
   dim3 grid(r.G_launch ? r.G_launch : b->G_launch), block(b->d.block);
   const int order = (b->d.xcd_remap ? 1 : 0) | ((b->d.alternate_sweep && !nodd && !r.slab) ? 2 : 0);
   if (nodd)
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT, CMP>), grid, block, 0, b->stream, sl, bps, G, order,
-                       want_fields, r.x0, r.x1, r.clear_raw);
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 1, CPT, CMP, PRE>), grid, block, 0, b->stream, sl, bps, G,
+                       order, want_fields, r.x0, r.x1, r.clear_raw);
   else
-    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT, CMP>), grid, block, 0, b->stream, sl, bps, G, order,
-                       want_fields, r.x0, r.x1, r.clear_raw);
+    hipLaunchKernelGGL((xadv::k_pass<R, FL, TH, GEN, 0, CPT, CMP, PRE>), grid, block, 0, b->stream, sl, bps, G,
+                       order, want_fields, r.x0, r.x1, r.clear_raw);
 }
 
-// Compact storage exists for the channel kernels with the fluid lattice on.
+// Compact storage exists for the channel kernels with the fluid lattice on; the thermal prefetch for the
+// fused D3Q19 pass on compact storage.
 template <class R, int FL, int TH, int GEN, int CPT>
 void launch_pass_s(xadv_batch* b, int nodd, int want_fields, const PassRange& r) {
   if constexpr (FL && !GEN) {
+    if constexpr (TH == 1) {
+      if (b->compact && b->d.thermal_prefetch)
+        return launch_pass_c<R, FL, TH, GEN, CPT, 1, 1>(b, nodd, want_fields, r);
+    }
     if (b->compact) return launch_pass_c<R, FL, TH, GEN, CPT, 1>(b, nodd, want_fields, r);
   }
   launch_pass_c<R, FL, TH, GEN, CPT, 0>(b, nodd, want_fields, r);
@@ -584,11 +589,15 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   const bool can_compact = d.fluid && !d.general;
   if (d.fluid_storage == XADV_STORAGE_COMPACT && !can_compact)
     return fail(XADV_E_ARG, "compact fluid storage needs the fluid lattice on and general = 0");
+  const bool compact = can_compact && (d.fluid_storage == XADV_STORAGE_COMPACT ||
+                                       (d.fluid_storage == XADV_STORAGE_AUTO && d.cap_nz <= xadv::kCompactMaxNz));
+  if (d.thermal_prefetch != 0 && d.thermal_prefetch != 1) return fail(XADV_E_ARG, "thermal_prefetch must be 0 or 1");
+  if (d.thermal_prefetch && !(compact && d.thermal))
+    return fail(XADV_E_ARG, "thermal_prefetch is for the fused D3Q19 pass with compact fluid storage");
   auto* b = new xadv_batch();
   b->d = d;
   b->q7 = q7 ? 1 : 0;
-  b->compact = can_compact && (d.fluid_storage == XADV_STORAGE_COMPACT ||
-                               (d.fluid_storage == XADV_STORAGE_AUTO && d.cap_nz <= xadv::kCompactMaxNz));
+  b->compact = compact;
   b->rsize = d.precision == 64 ? 8 : 4;
   b->cap_cells = (int64_t)d.cap_nx * d.cap_ny * d.cap_nz;
   b->cap_npad = round_up(b->cap_cells, 64);

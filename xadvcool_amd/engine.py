@@ -41,7 +41,7 @@ class LatticeBatch:
 
     def __init__(self, *, slots=1, cap_shape, fluid=True, thermal=True, general=False, precision=32, block=256,
                  xcd_remap=False, graph_chunk=100, cells_per_thread=1, fluid_storage="auto", alternate_sweep=True,
-                 thermal_lattice="D3Q19", slab_planes=0):
+                 thermal_lattice="D3Q19", slab_planes=0, thermal_prefetch=False):
         _require_gpu()
         self._lib = _lib.load()
         if graph_chunk and graph_chunk % 2:
@@ -54,7 +54,7 @@ class LatticeBatch:
         d = _lib.BatchDesc(int(precision), int(bool(fluid)), int(bool(thermal)), int(bool(general)), int(slots),
                            int(cap_shape[0]), int(cap_shape[1]), int(cap_shape[2]), int(block), int(bool(xcd_remap)),
                            int(graph_chunk), int(cells_per_thread), storage[fluid_storage], int(bool(alternate_sweep)),
-                           7 if thermal_lattice == "D3Q7" else 19, int(slab_planes))
+                           7 if thermal_lattice == "D3Q7" else 19, int(slab_planes), int(bool(thermal_prefetch)))
         h = ctypes.c_void_p()
         torch.cuda.init()
         _lib.check(self._lib.xadv_batch_create(ctypes.byref(d), ctypes.byref(h)), "xadv_batch_create")
@@ -241,7 +241,8 @@ class LatticeSolver:
     def __init__(self, mask, *, fluid=True, thermal=True, heat=None, tau=0.6, tau_th_fluid=0.6, tau_th_solid=0.6,
                  heat_power=0.0, T_in=0.0, rho_out=1.0, u_in=0.0, periodic=(0, 0, 0), force_x=0.0, u_wall=0.0,
                  dirichlet=None, frozen_velocity=False, precision=32, block=256, xcd_remap=False, graph_chunk=100,
-                 cells_per_thread=1, fluid_storage="auto", alternate_sweep=True, thermal_lattice="D3Q19"):
+                 cells_per_thread=1, fluid_storage="auto", alternate_sweep=True, thermal_lattice="D3Q19",
+                 thermal_prefetch=False):
         mask = np.asarray(mask)
         general = any(periodic) or force_x != 0.0 or u_wall != 0.0 or dirichlet is not None
         self.mask = mask
@@ -249,7 +250,8 @@ class LatticeSolver:
         self.batch = LatticeBatch(slots=1, cap_shape=self.shape, fluid=fluid, thermal=thermal, general=general,
                                   precision=precision, block=block, xcd_remap=xcd_remap, graph_chunk=graph_chunk,
                                   cells_per_thread=cells_per_thread, fluid_storage=fluid_storage,
-                                  alternate_sweep=alternate_sweep, thermal_lattice=thermal_lattice)
+                                  alternate_sweep=alternate_sweep, thermal_lattice=thermal_lattice,
+                                  thermal_prefetch=thermal_prefetch)
         self.batch.load(0, mask, heat=heat, tau=tau, tau_th_fluid=tau_th_fluid, tau_th_solid=tau_th_solid,
                         heat_power=heat_power, T_in=T_in, rho_out=rho_out, u_in=u_in, periodic=periodic,
                         force_x=force_x, u_wall=u_wall, dirichlet=dirichlet, frozen_velocity=frozen_velocity)
