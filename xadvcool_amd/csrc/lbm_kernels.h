@@ -705,15 +705,6 @@ __device__ __forceinline__ void thermal7_collide(const Slot<R>& S, R* __restrict
 
 // ------------------------------------------------------------- the pass ----
 
-#ifndef XADV_PASS_WAVES
-#define XADV_PASS_WAVES 0
-#endif
-#if XADV_PASS_WAVES > 0
-#define XADV_PASS_ATTR __attribute__((amdgpu_waves_per_eu(XADV_PASS_WAVES)))
-#else
-#define XADV_PASS_ATTR
-#endif
-
 // Thermal gather for a thread's cells: post-collision g of the previous step
 // through the streaming rules, or (first thermal half after an init) the
 // initial populations and T as given (src/simulation.py:73,89).
@@ -751,8 +742,8 @@ __device__ __forceinline__ void thermal_load(const Slot<R>& S, const Cell* cell,
 // variant of the same step (two buffers), its 7 loads per cell issued right
 // after the fluid gather so they overlap the fluid lattice's.
 template <class R, int FL, int TH, int GEN, int NODD, int CPT, int CMP, int PRE = 0>
-__global__ __launch_bounds__(256) XADV_PASS_ATTR void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int order,
-                                                             int want_fields, int x0, int x1, int clear_raw) {
+__global__ __launch_bounds__(256) void k_pass(Slot<R>* __restrict__ slots, int bps, int G, int order,
+                                              int want_fields, int x0, int x1, int clear_raw) {
   static_assert(!(CMP && (GEN || !FL)), "compact fluid storage is built for the channel kernels with the fluid on");
   static_assert(TH == 0 || TH == 1 || (TH == 7 && FL && !GEN), "thermal lattice: none, D3Q19, or D3Q7 with the fluid");
   static_assert(!PRE || (FL && TH == 1), "the thermal prefetch is for the fused D3Q19 pass");
