@@ -455,7 +455,9 @@ def main():
                          "mode (three lattices per launch), off elsewhere")
     ap.add_argument("--thermal-lattice", choices=["D3Q19", "D3Q7"], default="D3Q19",
                     help="passes mode: thermal lattice (D3Q7: the fluid pass then the D3Q7 thermal kernel)")
-    ap.add_argument("--weights", default=os.path.join(ROOT, "xadvcool_amd", "warmstart_weights.npz"))
+    ap.add_argument("--weights", default=None,
+                    help="surrogate weights: one .npz or several joined by commas, whose predictions are averaged "
+                         "(default: the shipped pair)")
     args = ap.parse_args()
     if args.mode == "solves" and args.designs is None:
         args.designs = 256
@@ -498,7 +500,6 @@ def main():
               flush=True)
         return
 
-    from xadvcool_amd import surrogate as S
     from xadvcool_amd import warmstart as W
     from xadvcool_amd.geometry import config2_design
 
@@ -506,7 +507,7 @@ def main():
     # maps on 148x128x32, parameters only (each arm builds its designs on the GPU inside its wall time).
     # --steps does not change it: a "step" here is one design solved to the stop rule.
     timed = W.make_design_params(W.COMPARE_SEED0, args.designs)
-    net = S.UNet3D.load_file(args.weights)
+    net = W.load_surrogate(args.weights or W.DEFAULT_ENSEMBLE)
     nslots = args.slots or 3
     launch = dict(block=args.block, cells_per_thread=args.cpt, xcd_remap=args.xcd,
                   alternate_sweep=args.sweep == "alternate", thermal_prefetch=args.thermal_prefetch != "off")
@@ -559,7 +560,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded unseen designs (seeds from 1,000,000), all five families and all ten heat maps; "
-                "surrogate trained only on the solver's own converged solutions of other seeds",
+                "surrogate (two networks, predictions averaged) trained only on the solver's own converged "
+                "solutions of other seeds",
         "config": {"workload": f"surrogate-warm-started batched solving: {len(timed)} unseen 148x128x32 designs "
                                "(straight, pin fin, gyroid, Schwarz P, Schwarz D; ten heat maps), each solved cold "
                                "and warm to the stop rule",

@@ -3,9 +3,10 @@
 * a warm start that blows up (temperatures far above the divergence limit) is abandoned at
   its first residual check and the design is re-run cold: the result equals
   the cold run of the same design bit for bit (same steps, same metrics);
-* the trained surrogate, on designs it never saw, reaches the same stop rule in
-  fewer steps than the cold start, with converged pressure drop and thermal
-  resistance within 0.5 % of the cold result.
+* the trained surrogate (the shipped pair of networks, predictions averaged), on
+  designs it never saw, reaches the same stop rule in fewer steps than the cold
+  start, with converged pressure drop and thermal resistance within 0.5 % of the
+  cold result.
 """
 import os
 
@@ -86,11 +87,12 @@ def test_trained_surrogate_cuts_steps_on_unseen_designs(gpu):
     heat).  A rate-based rule stops an exponential approach with time constant tau about tol * tau from its
     steady state; on these lattices tau reaches ~15,000 steps, so at 1e-6 a cold run can stop 0.7 % away and
     could not judge the warm one at 0.5 %; at 3e-7 it stops within ~0.2 %."""
-    from xadvcool_amd import surrogate as S
     from xadvcool_amd import warmstart as W
 
-    assert os.path.exists(W.DEFAULT_WEIGHTS), "trained weights missing from the package"
-    net = S.UNet3D.load_file(W.DEFAULT_WEIGHTS)
+    for p in W.DEFAULT_ENSEMBLE:
+        assert os.path.exists(p), f"trained weights missing from the package: {p}"
+    net = W.load_surrogate()
+    assert isinstance(net, W.Ensemble) and len(net.nets) == 2
     n = 8
     designs = _designs(n, 5000)
     print("[warmstart] cold vs warm, 8 designs, rise residuals at 3e-7", flush=True)
@@ -105,3 +107,19 @@ def test_trained_surrogate_cuts_steps_on_unseen_designs(gpu):
     warm_steps = np.mean([r["warm_steps"] for r in both])
     assert warm_steps < 0.5 * cold_steps, (warm_steps, cold_steps)
     assert summ["speedup_solves_per_hr"] >= 2.0, summ
+
+
+def test_ensemble_prediction_is_the_mean_of_its_networks(gpu):
+    """The shipped surrogate averages its two networks' de-normalised fields; solids and INLET cells keep the
+    cold-start state in every member and so in the mean."""
+    from xadvcool_amd import warmstart as W
+    ens = W.load_surrogate()
+    d = _designs(1, 7200)[0]
+    print("[warmstart] ensemble mean", flush=True)
+    a, b = (W.predict_fields(n, d) for n in ens.nets)
+    m = W.predict_fields(ens, d)
+    for x, y, z in zip(a, b, m):
+        assert torch.equal(z, (x + y) / 2)
+    solid = torch.from_numpy(d["mask"] == 1).cuda()
+    assert torch.all(m[0][solid] == 1.0) and torch.all(m[1][solid] == 0.0)
+    assert not torch.equal(a[2], b[2])  # two different networks

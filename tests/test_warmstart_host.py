@@ -107,3 +107,18 @@ def test_compare_summary_counts_and_ratios():
     g = s["by_family"]["gyroid"]
     assert (g["cold_converged"], g["warm_converged"], g["both"], g["within_0.5pct"]) == (1, 2, 1, 1)
     assert g["converged_per_step_ratio"] == round((2 / 60000) / (1 / 180000), 3)
+
+
+def test_shipped_ensemble_members_share_the_architecture():
+    """The two shipped weight files hold the same arrays (names and shapes) and the same normalisation."""
+    import os
+    a, b = (np.load(p, allow_pickle=False) for p in W.DEFAULT_ENSEMBLE)
+    assert all(os.path.exists(p) for p in W.DEFAULT_ENSEMBLE)
+    assert sorted(a.files) == sorted(b.files)
+    for k in a.files:
+        assert a[k].shape == b[k].shape, k
+    norm = [k for k in a.files if "norm" in k or "mean" in k or "std" in k]
+    assert norm, a.files
+    for k in norm:
+        assert np.array_equal(a[k], b[k]), k
+    assert any(not np.array_equal(a[k], b[k]) for k in a.files)  # two different networks

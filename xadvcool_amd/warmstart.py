@@ -36,6 +36,27 @@ PAD = (S.round8(CORE[0] + 20), S.round8(CORE[1]), S.round8(CORE[2]))
 CONFIRM = 5  # consecutive passing residual checks before a run counts as converged (both arms)
 COMPARE_SEED0 = 1_000_000  # evaluation designs: seeds far from every training seed
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights.npz")
+# The shipped surrogate: two networks of the same architecture (the second fine-tuned from the first on more
+# solver solutions) whose predicted fields are averaged.
+DEFAULT_ENSEMBLE = (DEFAULT_WEIGHTS,
+                    os.path.join(os.path.dirname(os.path.abspath(__file__)), "warmstart_weights_2.npz"))
+
+
+class Ensemble:
+    """Surrogates whose predicted (rho, u, T) are averaged, each de-normalised with its own statistics."""
+
+    def __init__(self, nets):
+        self.nets = list(nets)
+        if not self.nets:
+            raise ValueError("an ensemble needs at least one network")
+
+
+def load_surrogate(paths=DEFAULT_ENSEMBLE):
+    """One network (a path) or the average of several (a sequence of paths, or paths joined by commas)."""
+    if isinstance(paths, str):
+        paths = [p for p in paths.split(",") if p]
+    nets = [S.UNet3D.load_file(p) for p in paths]
+    return nets[0] if len(nets) == 1 else Ensemble(nets)
 
 
 def make_designs(seed0, n, heats=HEAT_SOURCE_NAMES, families=None):
@@ -230,7 +251,8 @@ def evaluate(net, samples, batch=8):
 
 @torch.no_grad()
 def predict_fields(net, d):
-    """Surrogate (rho, u, T) for a design, on the GPU (FP32, lattice shape).
+    """Surrogate (rho, u, T) for a design, on the GPU (FP32, lattice shape); an Ensemble gives the mean of its
+    networks' fields.
 
     Solid and INLET cells keep the cold-start values (rho 1, u 0).  Solid
     populations never change after the init.  INLET cells never read a
@@ -238,6 +260,9 @@ def predict_fields(net, d):
     post-collision set), so their state is a memory of the initial one: the
     inlet density, hence the pressure level at x = 1 and the converged pressure
     drop, would otherwise depend on the prediction (measured: up to 13 %)."""
+    if isinstance(net, Ensemble):
+        outs = [predict_fields(n, d) for n in net.nets]
+        return tuple((sum(o[i] for o in outs) / len(outs)).contiguous() for i in range(3))
     mask, heat = device_inputs(dict(d))
     x = S.features_device(mask, heat, d["u_inlet_val"], d["tau_fluid"], d["heat_power"], PAD).to(torch.bfloat16)[None]
     sc = torch.from_numpy(S.scalars(d["tau_fluid"], d["heat_power"], d["u_inlet_val"])).cuda()[None]
@@ -487,7 +512,9 @@ def main():
                     % COMPARE_SEED0)
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=5e-4)
-    ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--weights", default=None,
+                    help="train: where the trained weights go (default: the first shipped network); compare: the "
+                         "surrogate, one .npz or several joined by commas (default: the shipped ensemble)")
     ap.add_argument("--out", default="warmstart_out")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-steps", type=int, default=30000)
@@ -552,16 +579,17 @@ def main():
             per_epoch = step_s * max(1, len(tr) // 8) * 1.1 + 0.5
             epochs = max(1, min(epochs, int((deadline - time.perf_counter()) / per_epoch)))
             log(f"[train] {step_s * 1e3:.1f} ms per step: {epochs} epochs fit the time budget")
-        hist = train(net, tr, epochs=epochs, lr=args.lr, val=val, log=log, ckpt=args.weights,
+        out_weights = args.weights or DEFAULT_WEIGHTS
+        hist = train(net, tr, epochs=epochs, lr=args.lr, val=val, log=log, ckpt=out_weights,
                      warmup=args.warmup, deadline=deadline)
-        net.save(args.weights)
+        net.save(out_weights)
         log(f"[train] validation MSE per channel (u_x, u_y, u_z, rho, T): {evaluate_channels(net, val)}")
         with open(f"{args.out}/train_history.json", "w") as f:
             json.dump({"history": hist, "datagen": [{k: r[k] for k in ("key", "steps", "status")} for r in results]},
                       f)
-        log(f"[train] saved {args.weights}")
+        log(f"[train] saved {out_weights}")
     else:
-        net = S.UNet3D.load_file(args.weights)
+        net = load_surrogate(args.weights or DEFAULT_ENSEMBLE)
         fams = set(args.families.split(",")) if args.families else None
         designs = make_designs(COMPARE_SEED0 if args.seed0 is None else args.seed0, args.n, families=fams)
         summ, rows = compare(net, designs, slots=args.slots, tol=args.tol, max_steps=args.max_steps, log=log,
