@@ -151,18 +151,18 @@ def cpu_baseline(design, seconds, threads):
     return mask.size * steps / dt / 1e6, steps, dt
 
 
-def profile_passes(npasses, slots=1, seed=1000, thermal_lattice="D3Q19", **launch):
+def profile_passes(npasses, slots=1, seed=1000, thermal_lattice="D3Q19", precision=32, **launch):
     """Only fused passes on ``slots`` config-2 designs (seeds from ``seed``; graph chunks of 100, fewer when
     npasses is smaller), for kernel traces and counters.  One slot (~140 MB touched per pass) stays in the
-    256 MB Infinity Cache; many slots are HBM-resident.  With the D3Q7 thermal lattice a pass is the fluid
-    pass followed by the D3Q7 thermal kernel (56 B per cell of thermal populations)."""
+    256 MB Infinity Cache; many slots are HBM-resident.  With the D3Q7 thermal lattice the fused pass moves
+    56 B per cell of thermal populations; precision 64 moves 8-byte populations."""
     import torch
     from xadvcool_amd.engine import LatticeBatch
     from xadvcool_amd.geometry import config2_design
     designs = [config2_design(seed + i) for i in range(slots)]
     chunk = 100 if npasses >= 100 else max(2, npasses - npasses % 2)
     batch = LatticeBatch(slots=slots, cap_shape=LATTICE, graph_chunk=chunk, thermal_lattice=thermal_lattice,
-                         **launch)
+                         precision=precision, **launch)
     for s, d in enumerate(designs):
         load_design(batch, d, slot=s)
     batch.step(200, want_fields=False)  # past the start-up transient
@@ -177,8 +177,9 @@ def profile_passes(npasses, slots=1, seed=1000, thermal_lattice="D3Q19", **launc
         done += n
     ms = timer.total_ms() / max(done, 1)
     q = 7 if thermal_lattice == "D3Q7" else 19
-    nb = sum(algorithmic_bytes_per_pass(d["mask"], thermal_q=q) for d in designs)
-    print(json.dumps({"mode": "passes", "slots": slots, "thermal_lattice": thermal_lattice, **launch,
+    nb = sum(algorithmic_bytes_per_pass(d["mask"], rsize=precision // 8, thermal_q=q) for d in designs)
+    print(json.dumps({"mode": "passes", "slots": slots, "thermal_lattice": thermal_lattice, "precision": precision,
+                      **launch,
                       "passes": done, "avg_pass_us": round(1000 * ms, 2),
                       "alg_bytes_per_pass": nb, "achieved_GBs": round(nb / (ms * 1e-3) / 1e9, 1)}), flush=True)
     batch.close()
@@ -453,8 +454,9 @@ def main():
                     help="fused D3Q19 pass: issue the thermal loads right after the fluid gather (faster at a few "
                          "lattices per launch, slower when the state is HBM-resident); auto: on in the solves "
                          "mode (three lattices per launch), off elsewhere")
+    ap.add_argument("--precision", type=int, choices=[32, 64], default=32, help="passes mode: FP32 or FP64 lattices")
     ap.add_argument("--thermal-lattice", choices=["D3Q19", "D3Q7"], default="D3Q19",
-                    help="passes mode: thermal lattice (D3Q7: the fluid pass then the D3Q7 thermal kernel)")
+                    help="passes mode: thermal lattice (D3Q7: the seven-velocity variant in the same fused pass)")
     ap.add_argument("--weights", default=None,
                     help="surrogate weights: one .npz or several joined by commas, whose predictions are averaged "
                          "(default: the shipped pair)")
@@ -469,6 +471,7 @@ def main():
 
     if args.mode == "passes":
         return profile_passes(args.passes, args.slots or 1, thermal_lattice=args.thermal_lattice,
+                              precision=args.precision,
                               thermal_prefetch=args.thermal_prefetch == "on", block=args.block,
                               cells_per_thread=args.cpt, xcd_remap=args.xcd,
                               alternate_sweep=args.sweep == "alternate", slab_planes=args.slab)
