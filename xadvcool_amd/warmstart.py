@@ -513,8 +513,9 @@ def main():
     ap.add_argument("--epochs", type=int, default=40)
     ap.add_argument("--lr", type=float, default=5e-4)
     ap.add_argument("--weights", default=None,
-                    help="train: where the trained weights go (default: the first shipped network); compare: the "
-                         "surrogate, one .npz or several joined by commas (default: the shipped ensemble)")
+                    help="train: where the trained weights go (default: weights.npz in --out, so the shipped files "
+                         "are never overwritten by accident); compare: the surrogate, one .npz or several joined "
+                         "by commas (default: the shipped pair)")
     ap.add_argument("--out", default="warmstart_out")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--max-steps", type=int, default=30000)
@@ -579,7 +580,7 @@ def main():
             per_epoch = step_s * max(1, len(tr) // 8) * 1.1 + 0.5
             epochs = max(1, min(epochs, int((deadline - time.perf_counter()) / per_epoch)))
             log(f"[train] {step_s * 1e3:.1f} ms per step: {epochs} epochs fit the time budget")
-        out_weights = args.weights or DEFAULT_WEIGHTS
+        out_weights = args.weights or os.path.join(args.out, "weights.npz")
         hist = train(net, tr, epochs=epochs, lr=args.lr, val=val, log=log, ckpt=out_weights,
                      warmup=args.warmup, deadline=deadline)
         net.save(out_weights)
