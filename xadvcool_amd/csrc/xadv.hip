@@ -47,7 +47,8 @@ struct xadv_batch {
   int64_t cap_cells = 0, cap_npad = 0;
   int cap_P = 0;
   int bps = 0, G = 0, G_launch = 0;        // pass grid (block * cpt cells per block)
-  int bps_r = 0, G_r = 0, G_r_launch = 0;  // readout grid (one cell per thread)
+  int bps_r = 0, G_r = 0, G_r_launch = 0;  // one cell per thread (D3Q7 readout)
+  int bps_s = 0, G_s = 0, G_s_launch = 0;  // k_readout: kReadCells cells per thread, one stats row per block
   size_t rsize = 4;
   void* slots_dev = nullptr;
   void* slots_host = nullptr;  // pinned copy of the slot table
@@ -154,12 +155,12 @@ void launch_pass_t(xadv_batch* b, int nodd, int want_fields, const PassRange& r)
 template <class R, int FL, int TH, int GEN>
 void launch_readout_t(xadv_batch* b, int nodd, int with_stats) {
   auto* sl = reinterpret_cast<Slot<R>*>(b->slots_dev);
-  dim3 grid(b->G_r_launch), block(b->d.block);
+  dim3 grid(b->G_s_launch), block(b->d.block);
   if (nodd)
-    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps_r, b->G_r,
+    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 1>), grid, block, 0, b->stream, sl, b->bps_s, b->G_s,
                        b->d.xcd_remap, with_stats, b->partials, b->q7);
   else
-    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps_r, b->G_r,
+    hipLaunchKernelGGL((xadv::k_readout<R, FL, TH, GEN, 0>), grid, block, 0, b->stream, sl, b->bps_s, b->G_s,
                        b->d.xcd_remap, with_stats, b->partials, b->q7);
 }
 
@@ -615,6 +616,9 @@ int xadv_batch_create(const xadv_batch_desc* desc, xadv_batch** out) {
   b->bps_r = (int)((b->cap_cells + d.block - 1) / d.block);
   b->G_r = b->bps_r * d.slots;
   b->G_r_launch = d.xcd_remap ? xadv::xcd_grid(b->bps_r, d.slots) : b->G_r;
+  b->bps_s = (int)((b->cap_cells + (int64_t)d.block * xadv::kReadCells - 1) / ((int64_t)d.block * xadv::kReadCells));
+  b->G_s = b->bps_s * d.slots;
+  b->G_s_launch = d.xcd_remap ? xadv::xcd_grid(b->bps_s, d.slots) : b->G_s;
   b->slot_bytes = d.precision == 64 ? sizeof(Slot<double>) : sizeof(Slot<float>);
   b->loaded.assign(d.slots, 0);
   b->inited.assign(d.slots, 0);
@@ -920,10 +924,10 @@ int xadv_readout(xadv_batch* b, int with_stats) {
   if (with_stats) {
     if (b->d.precision == 64)
       hipLaunchKernelGGL((xadv::k_stats_final<double>), dim3(b->d.slots), dim3(xadv::kFoldThreads), 0, b->stream,
-                         reinterpret_cast<const Slot<double>*>(b->slots_dev), b->bps_r, b->partials, b->stats_dev);
+                         reinterpret_cast<const Slot<double>*>(b->slots_dev), b->bps_s, b->partials, b->stats_dev);
     else
       hipLaunchKernelGGL((xadv::k_stats_final<float>), dim3(b->d.slots), dim3(xadv::kFoldThreads), 0, b->stream,
-                         reinterpret_cast<const Slot<float>*>(b->slots_dev), b->bps_r, b->partials, b->stats_dev);
+                         reinterpret_cast<const Slot<float>*>(b->slots_dev), b->bps_s, b->partials, b->stats_dev);
     HIPCHK(hipGetLastError());
   }
   return XADV_OK;
