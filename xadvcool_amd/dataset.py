@@ -13,6 +13,14 @@ its attributes as JSON, next to ``metadata.parquet`` and
 The gaussian_hotspots maps of the reference used Python's salted string hash as
 seed (generate_dataset.py:406), which no run can reproduce; here the seed is
 the first 8 hex digits of the sample id's md5.
+
+With ``surrogate`` (not a reference option) every design starts from the
+surrogate's predicted fields under the same protocol (the reference's
+residuals at 1e-5 on every check, the flow-through floor, the step cap); a
+warm run that stalls, diverges or converges to fields failing the physics
+gate is re-run cold (``warmstart.solve_designs``).  Samples, metadata and
+attributes have the cold protocol's form; their stop step, wall time and
+MLUPS are the warm run's.
 """
 from __future__ import annotations
 
@@ -86,10 +94,16 @@ def plausibility(met, converged):
 
 
 def generate_dataset(output_dir, samples_per_design=50, max_steps=30000, seed=42, resume=True, slots=64,
-                     core=(G.CORE_NX, G.CORE_NY, G.CORE_NZ), limit=None, log=print):
-    """Solve the design points (up to ``limit``) and write samples, metadata.parquet and failed_samples.json."""
+                     core=(G.CORE_NX, G.CORE_NY, G.CORE_NZ), limit=None, log=print, surrogate=None, warm_chunk=256):
+    """Solve the design points (up to ``limit``) and write samples, metadata.parquet and failed_samples.json.
+
+    ``surrogate`` (``warmstart.load_surrogate()``): warm-start every design, ``warm_chunk`` designs per
+    batched run (their predicted fields are held on the GPU until their slot starts); the lattice must be the
+    surrogate's (the default core).  Returns (rows, failed) and, with a surrogate, logs the fallbacks."""
     import pandas as pd
     import torch
+    if surrogate is not None and tuple(core) != (G.CORE_NX, G.CORE_NY, G.CORE_NZ):
+        raise ValueError("the surrogate predicts fields on the default core only")
     Pr, k_ratio = COOLANT_PR, COOLANT_K_RATIO
     os.makedirs(output_dir, exist_ok=True)
     meta_path = os.path.join(output_dir, "metadata.parquet")
@@ -120,9 +134,12 @@ def generate_dataset(output_dir, samples_per_design=50, max_steps=30000, seed=42
             break
     by_key = {d["key"]: d for d in designs}
     log(f"[dataset] {len(designs)} designs to solve, {len(existing)} present, {len(failed)} failed before")
-    bs = BatchedSolver(slots, tuple(max(d["mask"].shape[k] for d in designs) for k in range(3)) if designs else
-                       (core[0] + 2 * BUF, core[1], core[2]), tol_ke=1e-5, tol_T=1e-5, max_steps=max_steps,
-                       min_steps_rule=True, Pr=Pr, k_ratio=k_ratio, collect_fields=True, metrics=True)
+    bs = None
+    if surrogate is None:
+        cap = (tuple(max(d["mask"].shape[k] for d in designs) for k in range(3)) if designs else
+               (core[0] + 2 * BUF, core[1], core[2]))
+        bs = BatchedSolver(slots, cap, tol_ke=1e-5, tol_T=1e-5, max_steps=max_steps, min_steps_rule=True, Pr=Pr,
+                           k_ratio=k_ratio, collect_fields=True, metrics=True)
 
     def on_result(r):
         d = by_key[r["key"]]
@@ -156,11 +173,23 @@ def generate_dataset(output_dir, samples_per_design=50, max_steps=30000, seed=42
         rows.append(_metadata_row(sid, p, d, r, met, r["wall_s"], gpu_name, Pr, k_ratio, max_v, attrs))
 
     t0 = time.perf_counter()
-    try:
-        if designs:
-            bs.run(designs, on_result=on_result)
-    finally:
-        bs.close()
+    if surrogate is not None:
+        from . import warmstart as W
+        nfb = 0
+        for i in range(0, len(designs), warm_chunk):
+            out = W.solve_designs(designs[i:i + warm_chunk], surrogate, slots=slots, tol=1e-5, max_steps=max_steps,
+                                  log=None, confirm=1, stop_rule="reference", min_steps_rule=True,
+                                  collect_fields=True, on_final=on_result)
+            nfb += len(out["fallbacks"])
+            for k, why, steps in out["fallbacks"]:
+                log(f"[dataset] {k}: warm run abandoned after {steps} steps ({why}), solved cold")
+        log(f"[dataset] warm-started {len(designs)} designs, {nfb} re-run cold")
+    else:
+        try:
+            if designs:
+                bs.run(designs, on_result=on_result)
+        finally:
+            bs.close()
     if rows:
         df = pd.DataFrame(rows).drop_duplicates(subset=["sample_id"], keep="last")
         df.to_parquet(meta_path, index=False)
