@@ -499,7 +499,8 @@ def main():
                                                          args.tol, groups=args.groups, block=args.block,
                                                          cells_per_thread=args.cpt,
                                                          alternate_sweep=args.sweep == "alternate",
-                                                         slab_planes=args.slab)}),
+                                                         slab_planes=args.slab,
+                                                         thermal_prefetch=args.thermal_prefetch == "on")}),
               flush=True)
         return
 
