@@ -51,6 +51,10 @@ int xnn_add_bf16(const void* a, const void* b, void* y, int64_t n, void* stream)
 /* 2x2x2 average pooling (coarse dims given) and its gradient (fine dims given). */
 int xnn_pool_fwd(const void* in, void* out, int N, int CX, int CY, int CZ, int C, void* stream);
 int xnn_pool_bwd(const float* gcoarse, float* gfine, int N, int FX, int FY, int FZ, int C, void* stream);
+/* out = acc + the pooling gradient of gcoarse (fine dims given): a skip connection's gradient and the one
+   through the pooling summed in one pass, the bits of xnn_add_f32(acc, xnn_pool_bwd(...)). */
+int xnn_pool_bwd_add(const float* gcoarse, const float* acc, float* out, int N, int FX, int FY, int FZ, int C,
+                     void* stream);
 /* nearest x2 upsampling (fine dims given) and its gradient (coarse dims given). */
 int xnn_up_fwd(const void* in, void* out, int N, int FX, int FY, int FZ, int C, void* stream);
 int xnn_up_bwd(const float* gfine, float* gcoarse, int N, int CX, int CY, int CZ, int C, void* stream);

@@ -192,6 +192,11 @@ def test_vectorised_elementwise_kernels_match_scalar(gpu, shape):
     gc = S.up_bwd(g, cs)
     assert torch.equal(gc.cpu(), S.up_bwd(_offset_copy(g), cs).cpu())
     assert torch.equal(S.pool_bwd(gc, shape[:4]).cpu(), S.pool_bwd(_offset_copy(gc), shape[:4]).cpu())
+    # the fused skip-gradient sum: the bits of add_f32(acc, pool_bwd(...)), vectorised and scalar
+    acc = torch.randn(*shape, generator=gen).cuda()
+    two = S.add_f32(acc, S.pool_bwd(gc, shape[:4]))
+    assert torch.equal(S.pool_bwd_add(gc, acc, shape[:4]).cpu(), two.cpu())
+    assert torch.equal(S.pool_bwd_add(_offset_copy(gc), acc, shape[:4]).cpu(), two.cpu())
     for n in (g.numel(), 1001, 4099):
         x = torch.randn(n, generator=gen).cuda() * 4
         gy = torch.randn(n, generator=gen).cuda()

@@ -1115,6 +1115,32 @@ __global__ void k_pool_bwd4(const float* __restrict__ gc_, float* __restrict__ g
   reinterpret_cast<f32x4*>(gf)[i] = 0.125f * g;
 }
 
+// acc + the pooling gradient (a skip connection's gradient plus the one through the pooling), in one
+// pass: the bits of k_add_f32 of acc and k_pool_bwd's output, without the fine temporary.
+__global__ void k_pool_bwd_add(const float* __restrict__ gc_, const float* __restrict__ acc, float* __restrict__ out,
+                               Geo gf_geo, int C) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= gf_geo.V() * C) return;
+  const int c = (int)(i % C);
+  int n, x, y, z;
+  decompose(gf_geo, i / C, n, x, y, z);
+  const int CX = gf_geo.X / 2, CY = gf_geo.Y / 2, CZ = gf_geo.Z / 2;
+  out[i] = acc[i] + 0.125f * gc_[((((int64_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2) * C + c];
+}
+__global__ void k_pool_bwd_add4(const float* __restrict__ gc_, const float* __restrict__ acc, float* __restrict__ out,
+                                Geo gfg, int C) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t C4 = (uint32_t)C >> 2;
+  if (i >= (uint32_t)gfg.V() * C4) return;
+  const uint32_t v = i / C4, c4 = i - v * C4;
+  int n, x, y, z;
+  dec32(v, gfg.X, gfg.Y, gfg.Z, n, x, y, z);
+  const int CX = gfg.X / 2, CY = gfg.Y / 2, CZ = gfg.Z / 2;
+  const f32x4 g = reinterpret_cast<const f32x4*>(gc_)[((((uint32_t)n * CX + x / 2) * CY + y / 2) * CZ + z / 2) * C4 +
+                                                      c4];
+  reinterpret_cast<f32x4*>(out)[i] = reinterpret_cast<const f32x4*>(acc)[i] + 0.125f * g;
+}
+
 // grad of upsampling: g_coarse = sum of the 8 fine children (fp32)
 __global__ void k_up_bwd(const float* __restrict__ gf, float* __restrict__ gc_, Geo gc, int C) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1925,6 +1951,17 @@ int xnn_pool_bwd(const float* gcoarse, float* gfine, int N, int FX, int FY, int 
   else
     hipLaunchKernelGGL(k_pool_bwd, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, gcoarse, gfine, gf,
                        C);
+  return hipGetLastError() == hipSuccess ? 0 : 2;
+}
+int xnn_pool_bwd_add(const float* gcoarse, const float* acc, float* out, int N, int FX, int FY, int FZ, int C,
+                     void* stream) {
+  Geo gf{FX, FY, FZ, N};
+  if (C % 4 == 0 && FX % 2 == 0 && FY % 2 == 0 && FZ % 2 == 0 && vec_ok(gf.V() * C, {gcoarse, acc, out}))
+    hipLaunchKernelGGL(k_pool_bwd_add4, dim3(nblk(gf.V() * (C / 4))), dim3(256), 0, (hipStream_t)stream, gcoarse,
+                       acc, out, gf, C);
+  else
+    hipLaunchKernelGGL(k_pool_bwd_add, dim3(nblk(gf.V() * C)), dim3(256), 0, (hipStream_t)stream, gcoarse, acc,
+                       out, gf, C);
   return hipGetLastError() == hipSuccess ? 0 : 2;
 }
 int xnn_up_fwd(const void* in, void* out, int N, int FX, int FY, int FZ, int C, void* stream) {

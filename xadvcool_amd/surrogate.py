@@ -66,6 +66,7 @@ def lib():
         L.xnn_add_bf16.argtypes = [vp, vp, vp, i64, vp]
         L.xnn_pool_fwd.argtypes = [vp, vp, i, i, i, i, i, vp]
         L.xnn_pool_bwd.argtypes = [vp, vp, i, i, i, i, i, vp]
+        L.xnn_pool_bwd_add.argtypes = [vp, vp, vp, i, i, i, i, i, vp]
         L.xnn_up_fwd.argtypes = [vp, vp, i, i, i, i, i, vp]
         L.xnn_up_bwd.argtypes = [vp, vp, i, i, i, i, i, vp]
         L.xnn_mse.argtypes = [vp, vp, vp, vp, vp, i64, i, i, f, vp]
@@ -316,6 +317,17 @@ def pool_bwd(gc, shape_f):
     return g
 
 
+def pool_bwd_add(gc, acc, shape_f):
+    """acc + pool_bwd(gc, shape_f) in one pass (the same bits as add_f32 of the two)."""
+    N, X, Y, Z = shape_f
+    C = gc.shape[-1]
+    if tuple(acc.shape) != (N, X, Y, Z, C) or acc.dtype != torch.float32:
+        raise ValueError("pool_bwd_add: acc must be the fp32 fine-level gradient")
+    y = torch.empty_like(acc)
+    _ok(lib().xnn_pool_bwd_add(_p(gc), _p(acc), _p(y), N, X, Y, Z, C, _st()), "xnn_pool_bwd_add")
+    return y
+
+
 def up_fwd(x, shape_f):
     N, X, Y, Z = shape_f
     C = x.shape[-1]
@@ -535,13 +547,13 @@ class UNet3D:
         g_a4b = add_f32(g_r4, lin("c4c", t["a4b"], dact("p4c", g_a4c), sh[3]))
         g_a4a = lin("c4b", t["a4a"], dact("p4b", g_a4b), sh[3])
         g_q3 = lin("c4a", t["q3"], dact("p4a", g_a4a), sh[3])
-        g_s3 = add_f32(g_s3, pool_bwd(g_q3, sh[2]))
+        g_s3 = pool_bwd_add(g_q3, g_s3, sh[2])
         g_a3a = lin("c3b", t["a3a"], dact("p3b", g_s3), sh[2])
         g_q2 = lin("c3a", t["q2"], dact("p3a", g_a3a), sh[2])
-        g_s2 = add_f32(g_s2, pool_bwd(g_q2, sh[1]))
+        g_s2 = pool_bwd_add(g_q2, g_s2, sh[1])
         g_a2a = lin("c2b", t["a2a"], dact("p2b", g_s2), sh[1])
         g_q1 = lin("c2a", t["q1"], dact("p2a", g_a2a), sh[1])
-        g_s1 = add_f32(g_s1, pool_bwd(g_q1, sh[0]))
+        g_s1 = pool_bwd_add(g_q1, g_s1, sh[0])
         g_a1a = lin("c1b", t["a1a"], dact("p1b", g_s1), sh[0])
         g1a, db1a = dact("p1a", g_a1a)
         cv["c1a"].wgrad(t["x"], g1a, sh[0], db=db1a)
