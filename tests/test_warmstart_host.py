@@ -122,3 +122,19 @@ def test_shipped_ensemble_members_share_the_architecture():
     for k in norm:
         assert np.array_equal(a[k], b[k]), k
     assert any(not np.array_equal(a[k], b[k]) for k in a.files)  # two different networks
+
+
+def test_load_surrogate_forms_one_network_or_an_ensemble(monkeypatch):
+    """A path gives one network; several (a sequence, or paths joined by commas) an Ensemble in that order."""
+    monkeypatch.setattr(W.S.UNet3D, "load_file", classmethod(lambda cls, p, device="cuda": ("net", p)))
+    assert W.load_surrogate("a.npz") == ("net", "a.npz")
+    e = W.load_surrogate("a.npz,b.npz")
+    assert isinstance(e, W.Ensemble) and e.nets == [("net", "a.npz"), ("net", "b.npz")]
+    e = W.load_surrogate(W.DEFAULT_ENSEMBLE)
+    assert [p for _, p in e.nets] == list(W.DEFAULT_ENSEMBLE)
+    try:
+        W.Ensemble([])
+    except ValueError:
+        pass
+    else:
+        raise AssertionError("an empty ensemble must be refused")
